@@ -1,0 +1,155 @@
+"""Shared test helpers: small seeded particle configurations and brute-force
+checks written from the mathematics (not from the oracle's code)."""
+import math
+
+import numpy as np
+
+
+def derived(L, Z):
+    zh16 = 2.0 ** (1.0 / 6.0)
+    nc = int(math.floor(L / zh16))
+    B = L / nc
+    return zh16, nc, B, L / Z
+
+
+def rsa_positions(n, L, dmin, seed, max_tries=200000):
+    """Uniform random sequential insertion with a hard exclusion dmin
+    (periodic).  Grid-accelerated, small n only."""
+    rng = np.random.default_rng(seed)
+    ncell = max(1, int(L // dmin))
+    cw = L / ncell
+    grid = {}
+    pts = []
+    tries = 0
+    while len(pts) < n:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("RSA jammed")
+        p = rng.uniform(0.0, L, 2)
+        cx, cy = int(p[0] / cw) % ncell, int(p[1] / cw) % ncell
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for q in grid.get(((cx + dx) % ncell, (cy + dy) % ncell), ()):
+                    d = p - q
+                    d -= L * np.round(d / L)
+                    if d @ d < dmin * dmin:
+                        ok = False
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid.setdefault((cx, cy), []).append(p)
+            pts.append(p)
+    return np.array(pts)
+
+
+def to_git_rel(x, B, nc):
+    """(git, rel) with git = floor(x/B) and rel = x - git*B in [0, B)."""
+    git = np.floor(x / B).astype(np.int64)
+    git = np.clip(git, 0, nc - 1)
+    rel = x - git * B
+    return git.astype(np.int32), rel
+
+
+def min_image(d, L):
+    return d - L * np.round(d / L)
+
+
+def wca_bruteforce(x, n1, L, s22=1.0, s12=1.0, e22=1.0, e12=1.0, cap=1.0e5, git=None, nc=None):
+    """All-pair WCA forces from the closed form F = 24 eps/sigma ... evaluated
+    exactly as the reference scales it: for a pair of type t with sigma
+    ratio s and eps ratio e, r2' = r2 s^2, d' = d s, |F|/r = e s 24 (2 r'^-14
+    - r'^-8) capped at `cap`, F = (|F|/r) d'.  The 11 pair has s = e = 1."""
+    n = len(x)
+    F = np.zeros_like(x)
+    cut2 = 2.0 ** (1.0 / 3.0)
+    for i in range(n):
+        d = min_image(x[i] - x, L)
+        r2 = (d * d).sum(1)
+        for j in np.nonzero(r2 < 4.0)[0]:
+            if j == i:
+                continue
+            if git is not None:
+                dc = (git[i] - git[j]) % nc
+                if not np.all((dc <= 1) | (dc == nc - 1)):
+                    continue
+            si, sj = i >= n1, j >= n1
+            if not si and not sj:
+                s, e = 1.0, None
+            elif si and sj:
+                s, e = s22, e22
+            else:
+                s, e = s12, e12
+            rr = r2[j] * s * s
+            if rr > cut2:
+                continue
+            c = 24.0 * (2.0 * rr ** -7 - rr ** -4)
+            if e is not None:
+                c *= e * s
+            c = min(c, cap)
+            F[i] += c * d[j] * s
+    return F
+
+
+def rel_rms(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    nb = np.sqrt((np.abs(b) ** 2).sum())
+    if nb == 0.0:
+        return float(np.sqrt((np.abs(a) ** 2).sum()))
+    return float(np.sqrt((np.abs(a - b) ** 2).sum()) / nb)
+
+
+def textbook_weights(x, h, scheme):
+    """Mesh index and 1D weights of NGP/CIC/TSC (Hockney & Eastwood,
+    Computer Simulation Using Particles, ch. 5): cell k = floor(x/h), centre
+    (k+1/2)h, d = (x - centre)/h; CIC (0 if d>=0 else -d, 1-|d|, d if d>=0);
+    TSC ((1/2)(1/2-d)^2, 3/4-d^2, (1/2)(1/2+d)^2)."""
+    k = np.floor(x / h).astype(np.int64)
+    d = x / h - (k + 0.5)
+    if scheme == 0:
+        w = np.stack([np.zeros_like(d), np.ones_like(d), np.zeros_like(d)], -1)
+    elif scheme == 1:
+        w = np.stack([np.where(d < 0, -d, 0.0), 1.0 - np.abs(d), np.where(d >= 0, d, 0.0)], -1)
+    else:
+        w = np.stack([0.5 * (0.5 - d) ** 2, 0.75 - d * d, 0.5 * (0.5 + d) ** 2], -1)
+    return k, w
+
+
+def deposit_numpy(x, q, Z, h, scheme):
+    rho = np.zeros((Z, Z))
+    kx, wx = textbook_weights(x[:, 0], h, scheme)
+    ky, wy = textbook_weights(x[:, 1], h, scheme)
+    for a in range(3):
+        for b in range(3):
+            np.add.at(rho, ((kx + a - 1) % Z, (ky + b - 1) % Z), q * wx[:, a] * wy[:, b] / (h * h))
+    return rho
+
+
+def gather_numpy(x, field, h, scheme):
+    Z = field.shape[0]
+    kx, wx = textbook_weights(x[:, 0], h, scheme)
+    ky, wy = textbook_weights(x[:, 1], h, scheme)
+    out = np.zeros(len(x))
+    for a in range(3):
+        for b in range(3):
+            out += wx[:, a] * wy[:, b] * field[(kx + a - 1) % Z, (ky + b - 1) % Z]
+    return out
+
+
+def symbols(Z, L, lam, kap):
+    """Closed-form spectral symbols: discrete-Laplacian Green's function and
+    central-difference gradient, FFT index order."""
+    h = L / Z
+    j = np.arange(Z)
+    q = 2 * np.pi / L * np.where(j < Z // 2, j, j - Z)
+    lap = 4 * np.sin(0.5 * q * h) ** 2 / h ** 2
+    G = 1.0 / (lap[:, None] + lap[None, :] + 1.0 / lam ** 2)
+    if lam > 0.2 * L:
+        G[0, 0] = 0.0
+    gx = kap * np.sin(q * h)[:, None] / h * G / Z / Z
+    gy = kap * np.sin(q * h)[None, :] / h * G / Z / Z
+    return gx, gy
