@@ -1,0 +1,354 @@
+"""Python host binding of libbdk_hip.so (the C-ABI of include/bdk.h).
+
+`RunDynamik` mirrors RunZustand::RunDynamik (signaturen2.h:33-252): same
+entry points (init, zeitschritt, berechne_WCA*, berechne_kapkraefte,
+berechne_zufallskraefte, optimaler_zeitschritt, refresh_erwNachbar, getFKap,
+getF), argument meaning and error behaviour, over the HIP library.  The HIP
+library is required: loading fails loudly if it is missing or no HIP device
+is present; there is no CPU fallback.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbdk_hip.so")
+
+NGP, CIC, TSC = 0, 1, 2
+FP64, FP32 = 0, 1
+NOISE_DISC, NOISE_GAUSS, NOISE_UPLOAD = 0, 1, 2
+GREEN_DISCRETE, GREEN_CONTINUOUS = 0, 1
+MESH_RHO, MESH_FX, MESH_FY = 0, 1, 2
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_NONFINITE = range(6)
+
+
+class BdkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bdk error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    """bd_params of include/bdk.h (parameter.h:14-99 user constants)."""
+    _fields_ = [
+        ("n1", C.c_int32), ("n2", C.c_int32), ("scheme", C.c_int32), ("Z", C.c_int32),
+        ("L", C.c_double), ("lam", C.c_double), ("kap", C.c_double), ("T", C.c_double),
+        ("dt_max", C.c_double), ("max_travel", C.c_double),
+        ("sigma11_22", C.c_double), ("sigma11_12", C.c_double), ("eps22_11", C.c_double),
+        ("eps12_11", C.c_double), ("f2_f1", C.c_double), ("max_pair_wca", C.c_double),
+        ("idgas_only", C.c_int32), ("no_wca", C.c_int32), ("restrict_radial", C.c_int32),
+        ("ngp_fix", C.c_int32), ("precision", C.c_int32), ("noise_mode", C.c_int32),
+        ("green_mode", C.c_int32), ("device", C.c_int32),
+        ("noise_var", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libbdk_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libbdk_hip.so not built: run `python bd-simulations_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, dbl, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint64
+        sig = {
+            "bdk_params_default": (C.c_int, [vp]),
+            "bdk_params_derived": (C.c_int, [vp, vp]),
+            "bdk_create": (C.c_int, [vp, C.POINTER(vp)]),
+            "bdk_destroy": (None, [vp]),
+            "bdk_last_error": (C.c_char_p, [vp]),
+            "bdk_set_positions": (C.c_int, [vp, C.c_int, i32, vp, vp]),
+            "bdk_get_positions": (C.c_int, [vp, C.c_int, i32, vp, vp]),
+            "bdk_compute_wca": (C.c_int, [vp, vp, vp]),
+            "bdk_compute_capillary": (C.c_int, [vp, vp, vp]),
+            "bdk_set_noise": (C.c_int, [vp, vp, vp]),
+            "bdk_get_noise": (C.c_int, [vp, vp, vp]),
+            "bdk_optimal_dt": (C.c_int, [vp, vp]),
+            "bdk_step": (C.c_int, [vp, dbl, vp]),
+            "bdk_run_steps": (C.c_int, [vp, i64, vp]),
+            "bdk_run_interval": (C.c_int, [vp, dbl, vp]),
+            "bdk_steps_for_interval": (C.c_int, [dbl, dbl, vp]),
+            "bdk_refresh": (C.c_int, [vp]),
+            "bdk_sync": (C.c_int, [vp]),
+            "bdk_step_counter": (i64, [vp]),
+            "bdk_get_cells": (C.c_int, [vp, vp, vp]),
+            "bdk_get_mesh": (C.c_int, [vp, C.c_int, vp]),
+            "bdk_footprint": (C.c_int, [vp, vp]),
+            "bdk_profile_steps": (C.c_int, [vp, i64, i32, vp, C.c_char_p, i32, vp]),
+            "bdk_mt_create": (C.c_int, [u64, C.POINTER(vp)]),
+            "bdk_mt_destroy": (None, [vp]),
+            "bdk_mt_uniform": (dbl, [vp, dbl, dbl]),
+            "bdk_mt_init_positions": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+            "bdk_mt_disc_noise": (C.c_int, [vp, i32, i32, vp, vp]),
+            "bdk_init_uniform": (C.c_int, [vp, u64, dbl, vp, vp]),
+            "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
+            "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return ["bdk_params_default", "bdk_params_derived", "bdk_create", "bdk_destroy", "bdk_last_error",
+            "bdk_set_positions", "bdk_get_positions", "bdk_compute_wca", "bdk_compute_capillary",
+            "bdk_set_noise", "bdk_get_noise", "bdk_optimal_dt", "bdk_step", "bdk_run_steps",
+            "bdk_run_interval", "bdk_steps_for_interval", "bdk_refresh", "bdk_sync", "bdk_step_counter",
+            "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_mt_create",
+            "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
+            "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos"]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_params(**kw):
+    p = Params()
+    lib().bdk_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def derived(p):
+    out = np.zeros(6)
+    lib().bdk_params_derived(C.byref(p), _p(out))
+    return dict(zh16=out[0], nc=int(out[1]), B=out[2], h=out[3], dq=out[4], drho=out[5])
+
+
+def footprint(p):
+    b = C.c_uint64(0)
+    rc = lib().bdk_footprint(C.byref(p), C.byref(b))
+    if rc:
+        raise BdkError(rc, lib().bdk_last_error(None).decode())
+    return b.value
+
+
+def steps_for_interval(obs_dt, dt):
+    n = C.c_int64(0)
+    lib().bdk_steps_for_interval(obs_dt, dt, C.byref(n))
+    return n.value
+
+
+def init_uniform(p, seed, dmin=0.9):
+    N = p.n1 + p.n2
+    git = np.zeros((N, 2), np.int32)
+    rel = np.zeros((N, 2))
+    rc = lib().bdk_init_uniform(C.byref(p), seed, dmin, _p(git), _p(rel))
+    if rc:
+        raise BdkError(rc, lib().bdk_last_error(None).decode())
+    return git, rel
+
+
+def init_blob(p, seed, width, background=0.1, dmin=0.9):
+    N = p.n1 + p.n2
+    git = np.zeros((N, 2), np.int32)
+    rel = np.zeros((N, 2))
+    clipped = C.c_double(0.0)
+    rc = lib().bdk_init_blob(C.byref(p), seed, width, background, dmin, _p(git), _p(rel), C.byref(clipped))
+    if rc:
+        raise BdkError(rc, lib().bdk_last_error(None).decode())
+    return git, rel, clipped.value
+
+
+class MT:
+    """The host mt19937_64 stream of zufall.cpp:18-39."""
+
+    def __init__(self, seed):
+        h = C.c_void_p()
+        lib().bdk_mt_create(seed, C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        try:
+            lib().bdk_mt_destroy(self.h)
+        except Exception:
+            pass
+
+    def uniform(self, lo, hi):
+        return lib().bdk_mt_uniform(self.h, lo, hi)
+
+    def init_positions(self, p):
+        g1 = np.zeros((p.n1, 2), np.int32)
+        r1 = np.zeros((p.n1, 2))
+        g2 = np.zeros((max(p.n2, 1), 2), np.int32)
+        r2 = np.zeros((max(p.n2, 1), 2))
+        lib().bdk_mt_init_positions(self.h, C.byref(p), _p(g1), _p(r1), _p(g2), _p(r2))
+        return np.concatenate([g1, g2[: p.n2]]), np.concatenate([r1, r2[: p.n2]])
+
+    def disc_noise(self, n1, n2):
+        a = np.zeros((max(n1, 1), 2))
+        b = np.zeros((max(n2, 1), 2))
+        lib().bdk_mt_disc_noise(self.h, n1, n2, _p(a), _p(b))
+        return np.concatenate([a[:n1], b[:n2]])
+
+
+class RunDynamik:
+    """One simulation on the GPU, with the entry points of
+    RunZustand::RunDynamik (signaturen2.h:68-188)."""
+
+    def __init__(self, params):
+        self.p = params
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.bdk_create(C.byref(params), C.byref(h))
+        if rc:
+            raise BdkError(rc, self.L.bdk_last_error(None).decode())
+        self.h = h
+        d = derived(params)
+        self.nc, self.B, self.hmesh = d["nc"], d["B"], d["h"]
+        self._F = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bdk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            raise BdkError(rc, self.L.bdk_last_error(self.h).decode())
+
+    @property
+    def N(self):
+        return self.p.n1 + self.p.n2
+
+    # --- positions (the aliases of am:305-308), species 1 first
+    def set_positions(self, git, rel):
+        git = np.ascontiguousarray(git, dtype=np.int32).reshape(-1, 2)
+        rel = np.ascontiguousarray(rel, dtype=np.float64).reshape(-1, 2)
+        n1 = self.p.n1
+        g1, r1 = np.ascontiguousarray(git[:n1]), np.ascontiguousarray(rel[:n1])
+        g2, r2 = np.ascontiguousarray(git[n1:]), np.ascontiguousarray(rel[n1:])
+        self._chk(self.L.bdk_set_positions(self.h, 1, n1, _p(g1), _p(r1)))
+        self._chk(self.L.bdk_set_positions(self.h, 2, self.p.n2, _p(g2), _p(r2)))
+
+    init = set_positions  # RunDynamik::init builds lists from the current positions (am:361-364)
+
+    def positions(self):
+        out_g, out_r = [], []
+        for sp, n in ((1, self.p.n1), (2, self.p.n2)):
+            g = np.zeros((max(n, 1), 2), np.int32)
+            r = np.zeros((max(n, 1), 2))
+            self._chk(self.L.bdk_get_positions(self.h, sp, n, _p(g), _p(r)))
+            out_g.append(g[:n])
+            out_r.append(r[:n])
+        return np.concatenate(out_g), np.concatenate(out_r)
+
+    def absolute(self):
+        g, r = self.positions()
+        return r + self.B * g
+
+    def _two(self, fn):
+        a = np.zeros((max(self.p.n1, 1), 2))
+        b = np.zeros((max(self.p.n2, 1), 2))
+        self._chk(fn(self.h, _p(a), _p(b)))
+        return np.concatenate([a[: self.p.n1], b[: self.p.n2]])
+
+    # --- force entry points
+    def berechne_WCA(self):
+        """berechne_WCA11 + berechne_WCA22 + addiere_WCA12 (dm:617-1059)."""
+        return self._two(self.L.bdk_compute_wca)
+
+    def berechne_kapkraefte(self):
+        """dm:1180-1233."""
+        return self._two(self.L.bdk_compute_capillary)
+
+    def set_noise(self, noise):
+        noise = np.ascontiguousarray(noise, dtype=np.float64).reshape(-1, 2)
+        a = np.ascontiguousarray(noise[: self.p.n1])
+        b = np.ascontiguousarray(noise[self.p.n1:])
+        self._chk(self.L.bdk_set_noise(self.h, _p(a), _p(b)))
+
+    def berechne_zufallskraefte(self):
+        """Noise the next step uses (dm:1317-1339)."""
+        return self._two(self.L.bdk_get_noise)
+
+    def optimaler_zeitschritt(self):
+        dt = C.c_double(0.0)
+        self._chk(self.L.bdk_optimal_dt(self.h, C.byref(dt)))
+        return dt.value
+
+    def zeitschritt(self, tmax=1.0):
+        """dm:74-163; tmax <= 0 returns 1e10 without stepping (dm:80-83)."""
+        dt = C.c_double(0.0)
+        rc = self.L.bdk_step(self.h, tmax, C.byref(dt))
+        if rc == ERR_ARG and not tmax > 0.0:
+            return dt.value
+        self._chk(rc)
+        return dt.value
+
+    def run_steps(self, k):
+        t = C.c_double(0.0)
+        self._chk(self.L.bdk_run_steps(self.h, k, C.byref(t)))
+        return t.value
+
+    def zeitschritte_bis_obs(self, obs_dt):
+        n = C.c_int64(0)
+        self._chk(self.L.bdk_run_interval(self.h, obs_dt, C.byref(n)))
+        return n.value
+
+    def refresh_erwNachbar(self):
+        self._chk(self.L.bdk_refresh(self.h))
+
+    def sync(self):
+        self._chk(self.L.bdk_sync(self.h))
+
+    def step_counter(self):
+        return self.L.bdk_step_counter(self.h)
+
+    def cells(self):
+        ncell = self.nc * self.nc
+        start = np.zeros(ncell + 1, np.int32)
+        ids = np.zeros(max(self.N, 1), np.int32)
+        self._chk(self.L.bdk_get_cells(self.h, _p(start), _p(ids)))
+        return start, ids[: self.N]
+
+    def mesh(self, which):
+        Z = self.p.Z
+        out = np.zeros((Z, Z))
+        self._chk(self.L.bdk_get_mesh(self.h, which, _p(out)))
+        return out
+
+    def getFKap(self):
+        """y capillary force on species-1 particle 1 (s2:182-185)."""
+        return self.berechne_kapkraefte()[1, 1]
+
+    def getF(self):
+        """capillary + WCA, y, species-1 particle 1 (s2:186-188)."""
+        return self.berechne_kapkraefte()[1, 1] + self.berechne_WCA()[1, 1]
+
+    def profile(self, k):
+        ms = np.zeros(64)
+        names = C.create_string_buffer(4096)
+        n = C.c_int32(0)
+        self._chk(self.L.bdk_profile_steps(self.h, k, 64, _p(ms), names, 4096, C.byref(n)))
+        labels = names.value.decode().split("\n")
+        return list(zip(labels, ms[: n.value].tolist()))
+
+    def write_pos(self, path1, path2, nr, obs_dt):
+        self._chk(self.L.bdk_write_pos(self.h, path1.encode(), path2.encode() if path2 else None, nr, obs_dt))
+
+
+def has_gpu():
+    try:
+        import subprocess
+        out = subprocess.run(["rocm-smi", "--showid"], capture_output=True, timeout=20).stdout.decode()
+        return "GPU" in out
+    except Exception:
+        return os.path.exists("/dev/kfd")

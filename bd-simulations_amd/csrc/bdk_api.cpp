@@ -1,0 +1,1029 @@
+// C-ABI of the MI355X Brownian-dynamics timestep (include/bdk.h).
+//
+// Host side of RunZustand::RunDynamik (signaturen2.h:33-252): parameter
+// validation, device state, the particle storage permutation (particles are
+// stored grouped by cell so that every kernel reads contiguous memory; the
+// boundary always speaks the reference's per-species order), the step
+// launch sequence, hipGraph replay of fixed-dt steps, the long-double
+// checkpoint clock of zeitschritte_bis_obs (dm:444-455), initial states and
+// the pos1_/pos2_ writer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "bdk_internal.hpp"
+
+using bdkx::DevScalars;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(bdk_t *h, int code, const std::string &msg) {
+    if (h)
+        h->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+int hipfail(bdk_t *h, hipError_t e, const char *where) {
+    return fail(h, BDK_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(h, call)                                      \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hipfail(h, e_, #call);  \
+    } while (0)
+
+bool is_pow2(int z) { return z > 0 && (z & (z - 1)) == 0; }
+
+int ilog2(int z) {
+    int l = 0;
+    while ((1 << l) < z) ++l;
+    return l;
+}
+
+// parameter.h:170-191
+struct Derived {
+    double zh16, B, h, dq, drho;
+    int nc;
+};
+
+Derived derive(const bd_params &p) {
+    Derived d;
+    d.zh16 = std::pow(2.0, 1.0 / 6.0);
+    d.nc = (int)std::floor(p.L / d.zh16);
+    d.B = p.L / ((double)d.nc);
+    d.h = p.L / p.Z;
+    d.dq = 2.0 * M_PI / p.L;
+    d.drho = 1.0 / (d.h * d.h);
+    return d;
+}
+
+std::string validate(const bd_params &p) {
+    if (p.n1 < 0 || p.n2 < 0 || (int64_t)p.n1 + p.n2 < 1) return "n1, n2 must be >= 0 with n1+n2 >= 1";
+    if ((int64_t)p.n1 + p.n2 > (1LL << 30)) return "n1+n2 too large";
+    if (!(p.L > 0.0) || !std::isfinite(p.L)) return "L must be positive";
+    if (p.scheme < 0 || p.scheme > 2) return "densGrid_Schema must be 0, 1 or 2 (NGP/CIC/TSC)";
+    if (!is_pow2(p.Z) || p.Z < 8 || p.Z > 4096) return "Z (FFT_samples) must be a power of two in [8, 4096]";
+    Derived d = derive(p);
+    if (d.nc < 3) return "the box must hold at least 3 list cells per side (L >= 3*2^(1/6))";
+    if ((int64_t)d.nc * d.nc > 4096LL * 4096LL) return "too many list cells (nc^2 > 4096^2)";
+    if (!(p.lambda > 0.0)) return "lambda must be positive";
+    if (!(p.T >= 0.0)) return "T must be >= 0";
+    if (!(p.dt_max > 0.0)) return "dt_max must be positive";
+    if (!(p.max_travel > 0.0)) return "max_travel must be positive (+inf for fixed dt)";
+    if (p.precision != BDK_FP64 && p.precision != BDK_FP32) return "precision must be BDK_FP64 or BDK_FP32";
+    if (p.noise_mode < 0 || p.noise_mode > 2) return "noise_mode must be DISC, GAUSS or UPLOAD";
+    if (p.green_mode < 0 || p.green_mode > 1) return "green_mode must be DISCRETE or CONTINUOUS";
+    if (p.noise_mode == BDK_NOISE_GAUSS && !(p.noise_var >= 0.0)) return "noise_var must be >= 0";
+    return "";
+}
+
+size_t real_size(const bd_params &p) { return p.precision == BDK_FP32 ? 4 : 8; }
+
+struct Footprint {
+    size_t particles, cells, mesh, total;
+};
+
+Footprint footprint_of(const bd_params &p) {
+    Derived d = derive(p);
+    const size_t N = (size_t)p.n1 + p.n2, R = real_size(p);
+    const size_t ncell = (size_t)d.nc * d.nc, Z = (size_t)p.Z;
+    Footprint f;
+    // cell, rel, pid, fwca, fkap, noise_up, noise_out, rank, list
+    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4);
+    f.cells = ncell * 8 + 4 + 4096 * 4;
+    // rho, ht, gt, fmesh, tables
+    f.mesh = Z * Z * R + (Z / 2) * Z * 2 * R + (Z / 2) * Z * 4 * R + Z * Z * 2 * R + Z * (2 * R + 3 * 8 + 3 * 4);
+    f.total = f.particles + f.cells + f.mesh + sizeof(DevScalars);
+    return f;
+}
+
+void free_dev(bdk_t *h) {
+    void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
+                    h->d.rank, h->d.count, h->d.start, h->d.list, h->d.bsum, h->d.rho, h->d.ht,
+                    h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
+                    h->d.qvf, h->d.sc};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    h->d = bdkx::DevBufs();
+}
+
+template <typename T> hipError_t dalloc(T *&p, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T) > 0 ? n * sizeof(T) : 16);
+    p = (T *)q;
+    return e;
+}
+
+hipError_t dalloc_v(void *&p, size_t bytes) { return hipMalloc(&p, bytes > 0 ? bytes : 16); }
+
+// spectral tables (dm:578-594, dm:1384-1394), built in FP64 on the host
+int build_tables_host(bdk_t *h) {
+    const bd_params &p = h->p;
+    const int Z = p.Z;
+    Derived dv = derive(p);
+    const double dx = dv.h;
+    std::vector<double> sq(Z), lap(Z), qv(Z);
+    for (int j = 0; j < Z; ++j) {
+        double q = dv.dq * ((j + (int)(Z * 0.5)) % Z - 0.5 * Z);
+        qv[j] = q;
+        if (p.green_mode == BDK_GREEN_DISCRETE) {
+            sq[j] = p.kap * std::sin(q * dx) / dx;
+            lap[j] = 4 * std::sin(0.5 * q * dx) * std::sin(0.5 * q * dx) / dx / dx;
+        } else {
+            sq[j] = (j == Z / 2) ? 0.0 : p.kap * q;
+            lap[j] = q * q;
+        }
+    }
+    std::vector<float> sqf(sq.begin(), sq.end()), lapf(lap.begin(), lap.end()), qvf(qv.begin(), qv.end());
+    HIPCHK(h, hipMemcpy(h->d.sq, sq.data(), Z * 8, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d.lap, lap.data(), Z * 8, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d.qv, qv.data(), Z * 8, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d.sqf, sqf.data(), Z * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d.lapf, lapf.data(), Z * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d.qvf, qvf.data(), Z * 4, hipMemcpyHostToDevice));
+    // twiddles exp(-2 pi i k / Z)
+    if (p.precision == BDK_FP32) {
+        std::vector<float> tw(2 * Z);
+        for (int k = 0; k < Z; ++k) {
+            double a = 2.0 * M_PI * k / Z;
+            tw[2 * k] = (float)std::cos(a);
+            tw[2 * k + 1] = (float)-std::sin(a);
+        }
+        HIPCHK(h, hipMemcpy(h->d.tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> tw(2 * Z);
+        for (int k = 0; k < Z; ++k) {
+            double a = 2.0 * M_PI * k / Z;
+            tw[2 * k] = std::cos(a);
+            tw[2 * k + 1] = -std::sin(a);
+        }
+        HIPCHK(h, hipMemcpy(h->d.tw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+    }
+    return BDK_OK;
+}
+
+int check_nonfinite(bdk_t *h) {
+    DevScalars sc;
+    HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (sc.nonfinite)
+        return fail(h, BDK_ERR_NONFINITE,
+                    "a particle position became non-finite or ran away (|rel| >= 1e9); it was held in place");
+    return BDK_OK;
+}
+
+// upload both species in storage order (grouped by cell, ascending id) and
+// rebuild the cell lists (WCA_init, dm:459-526)
+int upload(bdk_t *h) {
+    const int n1 = h->p.n1, n2 = h->p.n2, N = n1 + n2, nc = h->g.nc;
+    std::vector<int> cell(N);
+    std::vector<double> rel(2 * (size_t)N);
+    for (int sp = 0; sp < 2; ++sp) {
+        const int n = sp == 0 ? n1 : n2, off = sp == 0 ? 0 : n1;
+        for (int i = 0; i < n; ++i) {
+            int gx = h->hgit[sp][2 * i], gy = h->hgit[sp][2 * i + 1];
+            if (gx < 0 || gx >= nc || gy < 0 || gy >= nc)
+                return fail(h, BDK_ERR_ARG, "git out of range [0, nachList_Zellen) for species " +
+                                                std::to_string(sp + 1) + " particle " + std::to_string(i));
+            double rx = h->hrel[sp][2 * i], ry = h->hrel[sp][2 * i + 1];
+            if (!(std::fabs(rx) < 1e9 && std::fabs(ry) < 1e9))
+                return fail(h, BDK_ERR_NONFINITE, "non-finite rel for species " + std::to_string(sp + 1) +
+                                                      " particle " + std::to_string(i));
+            cell[off + i] = gx * nc + gy;
+            rel[2 * (size_t)(off + i)] = rx;
+            rel[2 * (size_t)(off + i) + 1] = ry;
+        }
+    }
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell[a] < cell[b]; });
+    std::vector<int> scell(N), spid(N);
+    std::vector<double> srel(2 * (size_t)N);
+    for (int s = 0; s < N; ++s) {
+        int p = order[s];
+        scell[s] = cell[p];
+        spid[s] = p;
+        srel[2 * (size_t)s] = rel[2 * (size_t)p];
+        srel[2 * (size_t)s + 1] = rel[2 * (size_t)p + 1];
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d.cell, scell.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.pid, spid.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.rel, srel.data(), (size_t)N * 16, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d.count, 0, (size_t)h->g.ncell * 4, h->stream));
+    HIPCHK(h, bdkx::launch_bin(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->dev_valid = true;
+    return BDK_OK;
+}
+
+int ensure(bdk_t *h) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    if (h->dev_valid) return BDK_OK;
+    if (!h->pos_set[0] || (h->p.n2 > 0 && !h->pos_set[1]))
+        return fail(h, BDK_ERR_STATE, "positions of both species must be set first (bdk_set_positions)");
+    return upload(h);
+}
+
+// slot-ordered device array of real2 -> per-species [n][2] doubles
+template <typename R>
+int download_r2(bdk_t *h, const void *src, double *f1, double *f2) {
+    const int N = h->g.N, n1 = h->p.n1;
+    std::vector<R> buf(2 * (size_t)N);
+    std::vector<int> pid(N);
+    HIPCHK(h, hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(R), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int s = 0; s < N; ++s) {
+        int p = pid[s];
+        double *dst = p < n1 ? (f1 ? f1 + 2 * (size_t)p : nullptr) : (f2 ? f2 + 2 * (size_t)(p - n1) : nullptr);
+        if (!dst) continue;
+        dst[0] = (double)buf[2 * (size_t)s];
+        dst[1] = (double)buf[2 * (size_t)s + 1];
+    }
+    return BDK_OK;
+}
+
+int download_forces(bdk_t *h, const void *src, double *f1, double *f2) {
+    return h->p.precision == BDK_FP32 ? download_r2<float>(h, src, f1, f2) : download_r2<double>(h, src, f1, f2);
+}
+
+void drop_graph(bdk_t *h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+}
+
+int ensure_graph(bdk_t *h) {
+    if (h->gexec) return BDK_OK;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = bdkx::enqueue_step(h, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) return hipfail(h, e, "enqueue_step (capture)");
+    if (e2 != hipSuccess) return hipfail(h, e2, "hipStreamEndCapture");
+    h->graph = g;
+    HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    return BDK_OK;
+}
+
+double dt_from_bits(unsigned long long b) {
+    double v;
+    std::memcpy(&v, &b, 8);
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bdk_params_default(bd_params *p) {
+    if (!p) return BDK_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->n1 = 1000;
+    p->n2 = 1000;
+    p->scheme = BDK_CIC;
+    p->Z = 128;
+    p->L = 64.0;
+    p->lambda = 8.0;
+    p->kap = 1.0;
+    p->T = 1.0;
+    p->dt_max = 1e-4;
+    p->max_travel = INFINITY;
+    p->sigma11_22 = 1.0;
+    p->sigma11_12 = 1.0;
+    p->eps22_11 = 1.0;
+    p->eps12_11 = 1.0;
+    p->f2_f1 = 0.5;
+    p->max_pair_wca = 1.0e5;
+    p->precision = BDK_FP32;
+    p->noise_mode = BDK_NOISE_DISC;
+    p->green_mode = BDK_GREEN_DISCRETE;
+    p->noise_var = 1.0;
+    p->seed = 1;
+    return BDK_OK;
+}
+
+int bdk_params_derived(const bd_params *p, double *o) {
+    if (!p || !o) return BDK_ERR_ARG;
+    Derived d = derive(*p);
+    o[0] = d.zh16;
+    o[1] = d.nc;
+    o[2] = d.B;
+    o[3] = d.h;
+    o[4] = d.dq;
+    o[5] = d.drho;
+    return BDK_OK;
+}
+
+int bdk_footprint(const bd_params *p, uint64_t *bytes) {
+    if (!p || !bytes) return fail(nullptr, BDK_ERR_ARG, "null argument");
+    std::string v = validate(*p);
+    if (!v.empty()) return fail(nullptr, BDK_ERR_ARG, v);
+    *bytes = footprint_of(*p).total;
+    return BDK_OK;
+}
+
+int bdk_create(const bd_params *p, bdk_t **out) {
+    if (!p || !out) return fail(nullptr, BDK_ERR_ARG, "null argument");
+    *out = nullptr;
+    std::string v = validate(*p);
+    if (!v.empty()) return fail(nullptr, BDK_ERR_ARG, v);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, BDK_ERR_HIP, "no HIP device available");
+    if (p->device < 0 || p->device >= ndev) return fail(nullptr, BDK_ERR_ARG, "device ordinal out of range");
+    bdk_t *h = new bdk();
+    h->p = *p;
+    h->dev = p->device;
+    hipError_t e = hipSetDevice(h->dev);
+    if (e != hipSuccess) {
+        int rc = hipfail(nullptr, e, "hipSetDevice");
+        delete h;
+        return rc;
+    }
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && footprint_of(*p).total > freeb) {
+        delete h;
+        return fail(nullptr, BDK_ERR_NOMEM, "device footprint exceeds free device memory");
+    }
+    Derived dv = derive(*p);
+    bdkx::Geo &g = h->g;
+    std::memset(&g, 0, sizeof(g));
+    g.n1 = p->n1;
+    g.n2 = p->n2;
+    g.N = p->n1 + p->n2;
+    g.nc = dv.nc;
+    g.ncell = dv.nc * dv.nc;
+    g.Z = p->Z;
+    g.logZ = ilog2(p->Z);
+    g.scheme = p->scheme;
+    g.idgas = p->idgas_only != 0;
+    g.nowca = p->no_wca != 0;
+    g.restrict_radial = p->restrict_radial != 0;
+    g.ngp_fix = p->ngp_fix != 0;
+    g.noise_mode = p->noise_mode;
+    g.green_mode = p->green_mode;
+    g.adaptive = std::isfinite(p->max_travel) ? 1 : 0;
+    g.B = dv.B;
+    g.L = p->L;
+    g.h = dv.h;
+    g.drho = dv.drho;
+    g.cut2 = dv.zh16 * dv.zh16;
+    g.s22 = p->sigma11_22;
+    g.s12 = p->sigma11_12;
+    g.e22f = p->eps22_11 * p->sigma11_22;
+    g.e12f = p->eps12_11 * p->sigma11_12;
+    g.maxpair = p->max_pair_wca;
+    g.f2_f1 = p->f2_f1;
+    g.T = p->T;
+    g.dt_max = p->dt_max;
+    g.mrd = p->max_travel;
+    g.noise_var = p->noise_var;
+    g.lam2inv = 1.0 / (p->lambda * p->lambda);
+    g.kap = p->kap;
+    g.zero_mode_g = p->lambda > 0.2 * p->L;
+    g.seed = p->seed;
+
+    const size_t N = g.N, R = real_size(*p), ncell = g.ncell, Z = g.Z;
+    bdkx::DevBufs &d = h->d;
+    int rc = BDK_OK;
+#define ALLOC(expr)                                                \
+    if (rc == BDK_OK) {                                            \
+        hipError_t ee = (expr);                                    \
+        if (ee != hipSuccess) rc = hipfail(nullptr, ee, #expr);    \
+    }
+    ALLOC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    ALLOC(dalloc(d.cell, N));
+    ALLOC(dalloc(d.rel, N));
+    ALLOC(dalloc(d.pid, N));
+    ALLOC(dalloc_v(d.fwca, N * 2 * R));
+    ALLOC(dalloc_v(d.fkap, N * 2 * R));
+    ALLOC(dalloc(d.noise_up, N));
+    ALLOC(dalloc(d.noise_out, N));
+    ALLOC(dalloc(d.rank, N));
+    ALLOC(dalloc(d.count, ncell));
+    ALLOC(dalloc(d.start, ncell + 1));
+    ALLOC(dalloc(d.list, N));
+    ALLOC(dalloc(d.bsum, 4096));
+    ALLOC(dalloc_v(d.rho, Z * Z * R));
+    ALLOC(dalloc_v(d.ht, (Z / 2) * Z * 2 * R));
+    ALLOC(dalloc_v(d.gt, (Z / 2) * Z * 4 * R));
+    ALLOC(dalloc_v(d.fmesh, Z * Z * 2 * R));
+    ALLOC(dalloc_v(d.tw, Z * 2 * R));
+    ALLOC(dalloc(d.sq, Z));
+    ALLOC(dalloc(d.lap, Z));
+    ALLOC(dalloc(d.qv, Z));
+    ALLOC(dalloc(d.sqf, Z));
+    ALLOC(dalloc(d.lapf, Z));
+    ALLOC(dalloc(d.qvf, Z));
+    ALLOC(dalloc(d.sc, 1));
+    ALLOC(bdkx::set_kernel_attributes());
+#undef ALLOC
+    if (rc != BDK_OK) {
+        free_dev(h);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
+    }
+    DevScalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    e = hipMemcpy(d.sc, &sc, sizeof(sc), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d.count, 0, ncell * 4);
+    if (e == hipSuccess) e = hipMemset(d.fwca, 0, N * 2 * R);
+    if (e == hipSuccess) e = hipMemset(d.fkap, 0, N * 2 * R);
+    if (e == hipSuccess) e = hipMemset(d.noise_up, 0, N * 16);
+    if (e == hipSuccess) e = hipMemset(d.rho, 0, Z * Z * R);
+    if (e == hipSuccess) e = hipMemset(d.fmesh, 0, Z * Z * 2 * R);
+    if (e != hipSuccess) {
+        rc = hipfail(nullptr, e, "initialising device state");
+        free_dev(h);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
+    }
+    rc = build_tables_host(h);
+    if (rc != BDK_OK) {
+        g_create_error = h->err;
+        free_dev(h);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
+    }
+    h->hgit[0].assign(2 * (size_t)p->n1, 0);
+    h->hgit[1].assign(2 * (size_t)p->n2, 0);
+    h->hrel[0].assign(2 * (size_t)p->n1, 0.0);
+    h->hrel[1].assign(2 * (size_t)p->n2, 0.0);
+    h->pos_set[1] = p->n2 == 0;
+    *out = h;
+    return BDK_OK;
+}
+
+void bdk_destroy(bdk_t *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->dev);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    free_dev(h);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *bdk_last_error(const bdk_t *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int bdk_set_positions(bdk_t *h, int species, int32_t n, const int32_t *git, const double *rel) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    if (species != 1 && species != 2) return fail(h, BDK_ERR_ARG, "species must be 1 or 2");
+    const int sp = species - 1;
+    const int want = sp == 0 ? h->p.n1 : h->p.n2;
+    if (n != want) return fail(h, BDK_ERR_ARG, "n does not match N" + std::to_string(species));
+    if (n > 0 && (!git || !rel)) return fail(h, BDK_ERR_ARG, "null position array");
+    if (n > 0) {
+        std::memcpy(h->hgit[sp].data(), git, sizeof(int32_t) * 2 * (size_t)n);
+        std::memcpy(h->hrel[sp].data(), rel, sizeof(double) * 2 * (size_t)n);
+    }
+    h->pos_set[sp] = true;
+    h->dev_valid = false;
+    return BDK_OK;
+}
+
+int bdk_get_positions(bdk_t *h, int species, int32_t n, int32_t *git, double *rel) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (species != 1 && species != 2) return fail(h, BDK_ERR_ARG, "species must be 1 or 2");
+    const int n1 = h->p.n1, N = h->g.N, nc = h->g.nc;
+    if (n != (species == 1 ? n1 : h->p.n2)) return fail(h, BDK_ERR_ARG, "n does not match");
+    std::vector<int> cell(N), pid(N);
+    std::vector<double> r(2 * (size_t)N);
+    HIPCHK(h, hipMemcpyAsync(cell.data(), h->d.cell, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(r.data(), h->d.rel, (size_t)N * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int s = 0; s < N; ++s) {
+        int p = pid[s];
+        int i = species == 1 ? p : p - n1;
+        if ((species == 1) != (p < n1)) continue;
+        if (git) {
+            git[2 * (size_t)i] = cell[s] / nc;
+            git[2 * (size_t)i + 1] = cell[s] % nc;
+        }
+        if (rel) {
+            rel[2 * (size_t)i] = r[2 * (size_t)s];
+            rel[2 * (size_t)i + 1] = r[2 * (size_t)s + 1];
+        }
+    }
+    return check_nonfinite(h);
+}
+
+int bdk_compute_wca(bdk_t *h, double *f1, double *f2) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (h->g.idgas) {
+        HIPCHK(h, hipMemsetAsync(h->d.fwca, 0, (size_t)h->g.N * 2 * real_size(h->p), h->stream));
+    } else {
+        HIPCHK(h, bdkx::launch_force(h, 1));
+    }
+    return download_forces(h, h->d.fwca, f1, f2);
+}
+
+int bdk_compute_capillary(bdk_t *h, double *f1, double *f2) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (h->g.idgas) {
+        HIPCHK(h, hipMemsetAsync(h->d.fkap, 0, (size_t)h->g.N * 2 * real_size(h->p), h->stream));
+    } else {
+        HIPCHK(h, bdkx::launch_force(h, 2));
+        HIPCHK(h, bdkx::launch_fft(h));
+        HIPCHK(h, bdkx::launch_gather(h, 0));
+    }
+    return download_forces(h, h->d.fkap, f1, f2);
+}
+
+int bdk_set_noise(bdk_t *h, const double *n1, const double *n2) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    if (h->p.noise_mode != BDK_NOISE_UPLOAD) return fail(h, BDK_ERR_STATE, "noise_mode is not BDK_NOISE_UPLOAD");
+    if ((h->p.n1 && !n1) || (h->p.n2 && !n2)) return fail(h, BDK_ERR_ARG, "null noise array");
+    if (h->p.n1)
+        HIPCHK(h, hipMemcpyAsync(h->d.noise_up, n1, (size_t)h->p.n1 * 16, hipMemcpyHostToDevice, h->stream));
+    if (h->p.n2)
+        HIPCHK(h, hipMemcpyAsync(h->d.noise_up + h->p.n1, n2, (size_t)h->p.n2 * 16, hipMemcpyHostToDevice,
+                                 h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return BDK_OK;
+}
+
+int bdk_get_noise(bdk_t *h, double *n1, double *n2) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    HIPCHK(h, bdkx::launch_noise_out(h));
+    std::vector<double> buf(2 * (size_t)h->g.N);
+    HIPCHK(h, hipMemcpyAsync(buf.data(), h->d.noise_out, buf.size() * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (n1) std::memcpy(n1, buf.data(), (size_t)h->p.n1 * 16);
+    if (n2) std::memcpy(n2, buf.data() + 2 * (size_t)h->p.n1, (size_t)h->p.n2 * 16);
+    return BDK_OK;
+}
+
+int bdk_optimal_dt(bdk_t *h, double *dt) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (!dt) return fail(h, BDK_ERR_ARG, "null dt");
+    if (!h->g.adaptive) {
+        *dt = h->p.dt_max;
+        return BDK_OK;
+    }
+    if (h->g.idgas) {
+        HIPCHK(h, hipMemsetAsync(h->d.fwca, 0, (size_t)h->g.N * 2 * real_size(h->p), h->stream));
+        HIPCHK(h, hipMemsetAsync(h->d.fmesh, 0, (size_t)h->g.Z * h->g.Z * 2 * real_size(h->p), h->stream));
+        HIPCHK(h, bdkx::launch_force(h, 4));
+    } else {
+        HIPCHK(h, bdkx::launch_force(h, 1 | 2 | 4));
+        HIPCHK(h, bdkx::launch_fft(h));
+    }
+    HIPCHK(h, bdkx::launch_gather(h, 1));
+    DevScalars sc;
+    HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *dt = dt_from_bits(sc.dt_bits);
+    return BDK_OK;
+}
+
+int bdk_step(bdk_t *h, double tmax, double *dt_out) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    if (!(tmax > 0.0)) {  // dm:80-83
+        if (dt_out) *dt_out = 1.0e10;
+        return fail(h, BDK_ERR_ARG, "Fehler: Negative Zeitschrittweite gefordert (tmax <= 0)");
+    }
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (h->g.idgas) {
+        HIPCHK(h, hipMemsetAsync(h->d.fmesh, 0, (size_t)h->g.Z * h->g.Z * 2 * real_size(h->p), h->stream));
+    }
+    HIPCHK(h, bdkx::enqueue_step(h, nullptr, nullptr));
+    h->host_step++;
+    if (h->g.adaptive) {
+        DevScalars sc;
+        HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (dt_out) *dt_out = dt_from_bits(sc.dt_last);
+    } else if (dt_out) {
+        *dt_out = h->p.dt_max;
+    }
+    return BDK_OK;
+}
+
+int bdk_run_steps(bdk_t *h, int64_t k, double *t_out) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (k < 0) return fail(h, BDK_ERR_ARG, "k must be >= 0");
+    long double t = 0.0L;
+    if (!h->g.adaptive && h->p.noise_mode != BDK_NOISE_UPLOAD) {
+        rc = ensure_graph(h);
+        if (rc) return rc;
+        for (int64_t i = 0; i < k; ++i) {
+            HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+            t += (long double)h->p.dt_max;
+        }
+        h->host_step += k;
+    } else {
+        for (int64_t i = 0; i < k; ++i) {
+            double dt = 0.0;
+            rc = bdk_step(h, 1.0, &dt);
+            if (rc) return rc;
+            t += (long double)dt;
+        }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (t_out) *t_out = (double)t;
+    return check_nonfinite(h);
+}
+
+int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out) {
+    if (!steps_out || !(dt > 0.0)) return BDK_ERR_ARG;
+    long double t = 0.0L;
+    int64_t n = 0;
+    while (t < obs_dt) {
+        t += (long double)dt;
+        ++n;
+    }
+    *steps_out = n;
+    return BDK_OK;
+}
+
+int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    int64_t n = 0;
+    if (!h->g.adaptive) {
+        bdk_steps_for_interval(obs_dt, h->p.dt_max, &n);
+        rc = bdk_run_steps(h, n, nullptr);
+        if (rc) return rc;
+    } else {
+        long double t = 0.0L;
+        while (t < obs_dt) {
+            double dt = 0.0;
+            rc = bdk_step(h, (double)((long double)obs_dt - t), &dt);
+            if (rc) return rc;
+            t += (long double)dt;
+            ++n;
+        }
+        rc = check_nonfinite(h);
+        if (rc) return rc;
+    }
+    if (steps_out) *steps_out = n;
+    return BDK_OK;
+}
+
+int bdk_refresh(bdk_t *h) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    HIPCHK(h, bdkx::launch_refresh(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return check_nonfinite(h);
+}
+
+int bdk_sync(bdk_t *h) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return BDK_OK;
+}
+
+int64_t bdk_step_counter(const bdk_t *h) { return h ? h->host_step : -1; }
+
+int bdk_get_cells(bdk_t *h, int32_t *start, int32_t *ids) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    const int N = h->g.N, ncell = h->g.ncell;
+    std::vector<int> st(ncell + 1), list(N), pid(N);
+    HIPCHK(h, hipMemcpyAsync(st.data(), h->d.start, (size_t)(ncell + 1) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(list.data(), h->d.list, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (start) std::memcpy(start, st.data(), (size_t)(ncell + 1) * 4);
+    if (ids) {
+        for (int q = 0; q < N; ++q) ids[q] = pid[list[q]];
+        for (int c = 0; c < ncell; ++c) std::sort(ids + st[c], ids + st[c + 1]);
+    }
+    return BDK_OK;
+}
+
+int bdk_get_mesh(bdk_t *h, int which, double *out) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    if (!out || which < 0 || which > 2) return fail(h, BDK_ERR_ARG, "bad mesh request");
+    const size_t Z2 = (size_t)h->g.Z * h->g.Z;
+    const bool f32 = h->p.precision == BDK_FP32;
+    if (which == BDK_MESH_RHO) {
+        if (f32) {
+            std::vector<float> b(Z2);
+            HIPCHK(h, hipMemcpyAsync(b.data(), h->d.rho, Z2 * 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < Z2; ++i) out[i] = b[i];
+        } else {
+            HIPCHK(h, hipMemcpyAsync(out, h->d.rho, Z2 * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+        }
+    } else {
+        const int c = which == BDK_MESH_FX ? 0 : 1;
+        if (f32) {
+            std::vector<float> b(2 * Z2);
+            HIPCHK(h, hipMemcpyAsync(b.data(), h->d.fmesh, Z2 * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < Z2; ++i) out[i] = b[2 * i + c];
+        } else {
+            std::vector<double> b(2 * Z2);
+            HIPCHK(h, hipMemcpyAsync(b.data(), h->d.fmesh, Z2 * 16, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < Z2; ++i) out[i] = b[2 * i + c];
+        }
+    }
+    return BDK_OK;
+}
+
+int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, int32_t names_cap,
+                      int32_t *n_out) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (h->p.noise_mode == BDK_NOISE_UPLOAD) return fail(h, BDK_ERR_STATE, "profiling needs a Philox noise mode");
+    std::vector<double> acc;
+    std::vector<std::string> labels;
+    for (int64_t it = 0; it < k; ++it) {
+        std::vector<std::string> nm;
+        std::vector<hipEvent_t> ev;
+        HIPCHK(h, bdkx::enqueue_step(h, &nm, &ev));
+        h->host_step++;
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (acc.empty()) {
+            acc.assign(ev.size() - 1, 0.0);
+            labels.assign(nm.begin() + 1, nm.end());
+        }
+        for (size_t i = 1; i < ev.size(); ++i) {
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, ev[i - 1], ev[i]);
+            acc[i - 1] += t;
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    if (n_out) *n_out = (int32_t)acc.size();
+    for (size_t i = 0; i < acc.size() && (int32_t)i < nk; ++i) ms[i] = k ? acc[i] / (double)k : 0.0;
+    if (names && names_cap > 0) {
+        std::string all;
+        for (size_t i = 0; i < labels.size(); ++i) all += labels[i] + (i + 1 < labels.size() ? "\n" : "");
+        std::strncpy(names, all.c_str(), (size_t)names_cap - 1);
+        names[names_cap - 1] = 0;
+    }
+    return check_nonfinite(h);
+}
+
+// ------------------------------------------------------------------ initial states
+struct bdk_mt {
+    std::mt19937_64 engine;
+};
+
+int bdk_mt_create(uint64_t seed, bdk_mt_t **out) {
+    if (!out) return BDK_ERR_ARG;
+    *out = new bdk_mt{std::mt19937_64(seed)};
+    return BDK_OK;
+}
+
+void bdk_mt_destroy(bdk_mt_t *g) { delete g; }
+
+// zufall_gleichverteilt_vonbis (zufall.cpp:35-39)
+double bdk_mt_uniform(bdk_mt_t *g, double lo, double hi) {
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    return lo + (hi - lo) * dist(g->engine);
+}
+
+int bdk_mt_init_positions(bdk_mt_t *g, const bd_params *p, int32_t *git1, double *rel1, int32_t *git2,
+                          double *rel2) {
+    if (!g || !p) return BDK_ERR_ARG;
+    Derived d = derive(*p);
+    for (int sp = 0; sp < 2; ++sp) {
+        const int n = sp == 0 ? p->n1 : p->n2;
+        int32_t *git = sp == 0 ? git1 : git2;
+        double *rel = sp == 0 ? rel1 : rel2;
+        for (int i = 0; i < n; ++i) {
+            double x = bdk_mt_uniform(g, 0.0, p->L);
+            double y = bdk_mt_uniform(g, 0.0, p->L);
+            int jc = (int)((int)x / d.B);  // dm:1655: (int) binds to x only
+            int kc = (int)((int)y / d.B);
+            git[2 * i] = jc;
+            git[2 * i + 1] = kc;
+            rel[2 * i] = x - jc * d.B;
+            rel[2 * i + 1] = y - kc * d.B;
+        }
+    }
+    return BDK_OK;
+}
+
+int bdk_mt_disc_noise(bdk_mt_t *g, int32_t n1, int32_t n2, double *out1, double *out2) {
+    if (!g) return BDK_ERR_ARG;
+    const double w5 = std::sqrt(5.0);
+    for (int sp = 0; sp < 2; ++sp) {
+        const int n = sp == 0 ? n1 : n2;
+        double *o = sp == 0 ? out1 : out2;
+        for (int i = 0; i < n; ++i) {
+            double x, y;
+            do {
+                x = w5 * bdk_mt_uniform(g, -1.0, 1.0);
+                y = w5 * bdk_mt_uniform(g, -1.0, 1.0);
+            } while (x * x + y * y > 5.0);
+            o[2 * i] = x;
+            o[2 * i + 1] = y;
+        }
+    }
+    return BDK_OK;
+}
+
+static void to_git_rel(double x, double y, const Derived &d, int32_t *git, double *rel) {
+    int gx = (int)std::floor(x / d.B), gy = (int)std::floor(y / d.B);
+    gx = std::min(std::max(gx, 0), d.nc - 1);
+    gy = std::min(std::max(gy, 0), d.nc - 1);
+    git[0] = gx;
+    git[1] = gy;
+    rel[0] = x - gx * d.B;
+    rel[1] = y - gy * d.B;
+}
+
+int bdk_init_uniform(const bd_params *p, uint64_t seed, double dmin, int32_t *git, double *rel) {
+    if (!p || !git || !rel) return fail(nullptr, BDK_ERR_ARG, "null argument");
+    const int N = p->n1 + p->n2;
+    const double L = p->L;
+    Derived d = derive(*p);
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> U(0.0, L);
+    const int ng = std::max(1, (int)(L / std::max(dmin, 1e-9)));
+    const double cw = L / ng;
+    std::vector<std::vector<int>> grid(dmin > 0 ? (size_t)ng * ng : 0);
+    std::vector<double> xs, ys;
+    xs.reserve(N);
+    ys.reserve(N);
+    const int64_t max_tries = 1000LL * N + 100000;
+    int64_t tries = 0;
+    while ((int)xs.size() < N) {
+        if (++tries > max_tries) return fail(nullptr, BDK_ERR_ARG, "random insertion jammed (density too high)");
+        double x = U(eng), y = U(eng);
+        if (x >= L) x = 0.0;
+        if (y >= L) y = 0.0;
+        if (dmin > 0) {
+            int cx = std::min((int)(x / cw), ng - 1), cy = std::min((int)(y / cw), ng - 1);
+            bool ok = true;
+            for (int a = -1; a <= 1 && ok; ++a)
+                for (int b = -1; b <= 1 && ok; ++b) {
+                    int u = ((cx + a) % ng + ng) % ng, v = ((cy + b) % ng + ng) % ng;
+                    for (int q : grid[(size_t)u * ng + v]) {
+                        double dx = x - xs[q], dy = y - ys[q];
+                        dx -= L * std::round(dx / L);
+                        dy -= L * std::round(dy / L);
+                        if (dx * dx + dy * dy < dmin * dmin) {
+                            ok = false;
+                            break;
+                        }
+                    }
+                }
+            if (!ok) continue;
+            grid[(size_t)cx * ng + cy].push_back((int)xs.size());
+        }
+        xs.push_back(x);
+        ys.push_back(y);
+    }
+    for (int i = 0; i < N; ++i) to_git_rel(xs[i], ys[i], d, git + 2 * i, rel + 2 * i);
+    return BDK_OK;
+}
+
+int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double background, double dmin, int32_t *git,
+                  double *rel, double *clipped_out) {
+    if (!p || !git || !rel || !(dmin > 0) || !(width > 0)) return fail(nullptr, BDK_ERR_ARG, "bad argument");
+    const int N = p->n1 + p->n2;
+    const double L = p->L, c = 0.5 * L;
+    Derived d = derive(*p);
+    // hexagonal lattice of spacing dmin with whole rows and columns that fit
+    // the periodic box (the row spacing is stretched slightly to close it)
+    const int nxl = std::max(1, (int)std::floor(L / dmin));
+    int nyl = std::max(2, (int)std::floor(L / (dmin * std::sqrt(3.0) / 2.0)));
+    if (nyl % 2) --nyl;
+    const double ax = L / nxl, ay = L / nyl;
+    const double rho_lat = 1.0 / (ax * ay);
+    // target density profile: (1-bg) N Gaussian(width) + bg N / L^2
+    const double amp = (1.0 - background) * N / (2.0 * M_PI * width * width);
+    const double flat = background * N / (L * L);
+    std::vector<double> sx, sy, w;
+    sx.reserve((size_t)nxl * nyl);
+    double wsum = 0.0, clip_lost = 0.0;
+    for (int j = 0; j < nyl; ++j)
+        for (int i = 0; i < nxl; ++i) {
+            double x = (i + 0.5 * (j & 1) + 0.25) * ax, y = (j + 0.5) * ay;
+            double dx = x - c, dy = y - c;
+            double target = amp * std::exp(-(dx * dx + dy * dy) / (2.0 * width * width)) + flat;
+            double prob = target / rho_lat;
+            if (prob > 1.0) {
+                clip_lost += (prob - 1.0);
+                prob = 1.0;
+            }
+            sx.push_back(x);
+            sy.push_back(y);
+            w.push_back(prob);
+            wsum += prob;
+        }
+    if ((double)sx.size() < N) return fail(nullptr, BDK_ERR_ARG, "lattice has fewer sites than N");
+    // renormalise after the clip: scale the profile by alpha so that the
+    // clipped keep-probabilities min(1, alpha w) sum to N (bisection)
+    {
+        std::vector<double> raw(w.size());
+        for (size_t s = 0; s < w.size(); ++s) {
+            double x = sx[s] - c, y = sy[s] - c;
+            raw[s] = (amp * std::exp(-(x * x + y * y) / (2.0 * width * width)) + flat) / rho_lat;
+        }
+        auto kept = [&](double a) {
+            double t = 0.0;
+            for (double r : raw) t += std::min(1.0, a * r);
+            return t;
+        };
+        double lo = 0.0, hi = 1.0;
+        while (kept(hi) < N) hi *= 2.0;
+        for (int it = 0; it < 100; ++it) {
+            double mid = 0.5 * (lo + hi);
+            (kept(mid) < N ? lo : hi) = mid;
+        }
+        for (size_t s = 0; s < w.size(); ++s) w[s] = std::min(1.0, hi * raw[s]);
+    }
+    // seeded thinning: keep each site independently with probability w
+    // (expected count N), then correct the count to exactly N by dropping
+    // random uncertain sites or adding sites accepted with probability w
+    std::mt19937_64 eng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::vector<char> keep(sx.size(), 0);
+    std::vector<int> chosen;
+    chosen.reserve((size_t)N + 1024);
+    for (size_t s = 0; s < sx.size(); ++s)
+        if (U(eng) < w[s]) {
+            keep[s] = 1;
+            chosen.push_back((int)s);
+        }
+    std::uniform_int_distribution<size_t> pick(0, sx.size() - 1);
+    while ((int)chosen.size() < N) {
+        size_t s = pick(eng);
+        if (!keep[s] && U(eng) < w[s]) {
+            keep[s] = 1;
+            chosen.push_back((int)s);
+        }
+    }
+    while ((int)chosen.size() > N) {
+        std::uniform_int_distribution<size_t> pc(0, chosen.size() - 1);
+        size_t q = pc(eng);
+        if (w[chosen[q]] >= 1.0) continue;
+        keep[chosen[q]] = 0;
+        chosen[q] = chosen.back();
+        chosen.pop_back();
+    }
+    std::sort(chosen.begin(), chosen.end());
+    // species labels: shuffle so both species share the profile
+    std::shuffle(chosen.begin(), chosen.end(), eng);
+    for (int i = 0; i < N; ++i) to_git_rel(sx[chosen[i]], sy[chosen[i]], d, git + 2 * i, rel + 2 * i);
+    if (clipped_out) *clipped_out = clip_lost / (wsum + clip_lost);
+    return BDK_OK;
+}
+
+int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt) {
+    if (!h || !path1) return fail(h, BDK_ERR_ARG, "null argument");
+    const int n1 = h->p.n1, n2 = h->p.n2;
+    std::vector<int32_t> g1(2 * (size_t)n1), g2(2 * (size_t)n2);
+    std::vector<double> r1(2 * (size_t)n1), r2(2 * (size_t)n2);
+    int rc = bdk_get_positions(h, 1, n1, g1.data(), r1.data());
+    if (rc) return rc;
+    if (n2) {
+        rc = bdk_get_positions(h, 2, n2, g2.data(), r2.data());
+        if (rc) return rc;
+    }
+    const double B = h->g.B;
+    for (int sp = 0; sp < (n2 ? 2 : 1); ++sp) {
+        const char *path = sp == 0 ? path1 : path2;
+        if (!path) continue;
+        FILE *f = std::fopen(path, nr == 0 ? "w" : "a");
+        if (!f) return fail(h, BDK_ERR_ARG, std::string("cannot open ") + path);
+        if (nr == 0) std::fprintf(f, "# Format: t TAB x TAB y \n\n");  // au:60
+        const int n = sp == 0 ? n1 : n2;
+        const std::vector<int32_t> &g = sp == 0 ? g1 : g2;
+        const std::vector<double> &r = sp == 0 ? r1 : r2;
+        for (int i = 0; i < n; ++i) {  // au:633-637
+            double x = r[2 * (size_t)i] + B * g[2 * (size_t)i];
+            double y = r[2 * (size_t)i + 1] + B * g[2 * (size_t)i + 1];
+            std::fprintf(f, "%g \t %g \t %g \n", nr * obs_dt, x, y);
+        }
+        std::fprintf(f, "\n");
+        std::fclose(f);
+    }
+    return BDK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Internal declarations shared by the HIP kernels (bdk_kernels.hip) and the
+// C-ABI layer (bdk_api.cpp).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bdk.h"
+
+namespace bdkx {
+
+// Geometry and physics constants passed by value to every kernel.
+// Derived values follow parameter.h:170-191 and gr:28.
+struct Geo {
+    int n1, n2, N;
+    int nc;          // nachList_Zellen
+    int ncell;       // nc*nc
+    int Z, logZ;
+    int scheme;
+    int idgas, nowca, restrict_radial, ngp_fix, noise_mode, green_mode;
+    int adaptive;    // max_travel finite: dt from the device min-reduction
+    double B;        // nachList_Breite
+    double L;
+    double h;        // densGrid_Breite
+    double drho;     // 1/h^2
+    double cut2;     // zweihoch1_6*zweihoch1_6
+    double s22, s12; // sigma ratios
+    double e22f, e12f; // eps22_11*sigma11_22, eps12_11*sigma11_12 (dm:851, dm:1031)
+    double maxpair;
+    double f2_f1;
+    double T, dt_max, mrd, noise_var;
+    double lam2inv;  // 1/(lambda*lambda)
+    double kap;
+    int zero_mode_g; // G(0,0) = 0 when lambda > 0.2 L (dm:1389)
+    unsigned long long seed;
+};
+
+// Device-resident scalars of one simulation (graph replays read them).
+struct DevScalars {
+    unsigned long long step;      // step counter (noise counter)
+    unsigned long long dt_bits;   // adaptive dt: running min, double bits
+    unsigned long long exc_bits[2]; // max excursion of rel outside [0,B], double bits, by step parity
+    unsigned long long dt_last;   // dt of the last update, double bits
+    int nonfinite;                // a non-finite position appeared
+    int fallback;                 // tiles that ran the unstaged WCA/deposit path
+    int pad[4];
+};
+
+struct DevBufs {
+    // particles, storage order ("slot"); species from pid (< n1: species 1)
+    int *cell = nullptr;      // packed cell gx*nc+gy
+    double2 *rel = nullptr;   // offset inside the cell
+    int *pid = nullptr;       // slot -> particle id
+    void *fwca = nullptr;     // [N] real2
+    void *fkap = nullptr;     // [N] real2
+    double2 *noise_up = nullptr; // [N] by particle id (upload mode)
+    double2 *noise_out = nullptr; // [N] by particle id (bdk_get_noise)
+    int *rank = nullptr;      // [N] rank of the slot inside its cell
+    // cell list
+    int *count = nullptr;     // [ncell]
+    int *start = nullptr;     // [ncell+1]
+    int *list = nullptr;      // [N] slots grouped by cell
+    int *bsum = nullptr;      // scan block sums
+    // mesh
+    void *rho = nullptr;      // Z*Z real
+    void *ht = nullptr;       // (Z/2)*Z complex, transposed half spectrum
+    void *gt = nullptr;       // (Z/2)*Z complex pairs (Fx, Fy), transposed
+    void *fmesh = nullptr;    // Z*Z real2 (Fx, Fy)
+    // tables: tw[k] = exp(-2 pi i k/Z), sq[j] = kap sin(q_j h)/h, lap[j] = 4 sin^2(q_j h/2)/h^2, qv[j]
+    void *tw = nullptr;
+    double *sq = nullptr, *lap = nullptr, *qv = nullptr;  // FP64 1D tables
+    float *sqf = nullptr, *lapf = nullptr, *qvf = nullptr;  // FP32 copies
+    DevScalars *sc = nullptr;
+};
+
+struct Launch;  // kernel sequence bookkeeping for profiling
+
+}  // namespace bdkx
+
+struct bdk {
+    bd_params p;
+    bdkx::Geo g;
+    bdkx::DevBufs d;
+    hipStream_t stream = nullptr;
+    int dev = 0;
+    bool pos_set[2] = {false, false};
+    bool dev_valid = false;       // device particle state uploaded
+    std::vector<int32_t> hgit[2];
+    std::vector<double> hrel[2];
+    int scan_blocks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+    int64_t host_step = 0;
+};
+
+namespace bdkx {
+
+// Launchers implemented in bdk_kernels.hip.  All enqueue on h->stream.
+// what: bit 0 = WCA, bit 1 = deposit
+hipError_t launch_force(bdk_t *h, int what);
+hipError_t launch_fft(bdk_t *h);                  // rho -> fmesh (r2c, multiply, c2r)
+hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)
+hipError_t launch_update(bdk_t *h, int fused);    // noise, Euler step, refresh, counting
+hipError_t launch_bin(bdk_t *h);                  // counting after an upload/refresh
+hipError_t launch_scan_fill(bdk_t *h);            // exclusive scan + fill of the cell list
+hipError_t launch_refresh(bdk_t *h);              // refresh only (no motion)
+hipError_t launch_noise_out(bdk_t *h);            // noise of the next step, by pid
+hipError_t set_kernel_attributes();
+hipError_t build_tables(bdk_t *h);                // spectral tables in device memory
+
+// one full step as a sequence of launches; names (if non-null) get the
+// kernel labels in launch order
+hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev);
+
+}  // namespace bdkx

@@ -1,0 +1,53 @@
+// Gathers the constants of a reference-format parameter.h (parameter.h:14-99)
+// into a RunConfig.  Compile this file with -I<dir of your parameter.h>, as
+// the reference compiles every translation unit against its parameter.h.
+// Environment overrides: BDK_SEED (MT seed; the reference uses time(NULL)),
+// BDK_NOISE=mt|disc (host MT stream uploaded per step, or device Philox
+// disc noise), BDK_PRECISION=fp64|fp32.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+
+#include "parameter.h"
+#include "run_zustand.hpp"
+
+RunConfig config_from_parameter_h() {
+    RunConfig c;
+    bdk_params_default(&c.p);
+    c.p.n1 = N1;
+    c.p.n2 = N2;
+    c.p.L = L;
+    c.p.scheme = densGrid_Schema;
+    c.p.Z = FFT_samples;
+    c.p.lambda = lambda_kapillar;
+    c.p.kap = kapillar_vorfaktor;
+    c.p.T = T;
+    c.p.dt_max = dt_max;
+    c.p.max_travel = max_reisedistanz;
+    c.p.sigma11_22 = sigma11_22;
+    c.p.sigma11_12 = sigma11_12;
+    c.p.eps22_11 = eps22_11;
+    c.p.eps12_11 = eps12_11;
+    c.p.f2_f1 = f2_f1;
+    c.p.max_pair_wca = maxPairWCA;
+    c.p.idgas_only = idgas_only;
+    c.p.no_wca = noWCA;
+    c.p.restrict_radial = restrictRadial;
+    c.runs = runs;
+    c.obs_dt = obs_dt;
+    c.obs_anzahl = obs_anzahl;
+    c.startpos_methode = startpos_methode;
+    c.max_write_interval = max_write_interval;
+    c.auswerten_animation = auswerten_animation;
+    c.krafttestmode = krafttestmode;
+    const char *s = std::getenv("BDK_SEED");
+    c.seed = s ? std::strtoull(s, nullptr, 10) : (uint64_t)std::time(nullptr);
+    const char *nz = std::getenv("BDK_NOISE");
+    c.noise_source = (nz && std::strcmp(nz, "mt") == 0) ? 1 : 0;
+    c.p.noise_mode = c.noise_source ? BDK_NOISE_UPLOAD : BDK_NOISE_DISC;
+    c.p.seed = c.seed;
+    const char *pr = std::getenv("BDK_PRECISION");
+    c.p.precision = (pr && std::strcmp(pr, "fp64") == 0) ? BDK_FP64 : BDK_FP32;
+    return c;
+}

@@ -1,0 +1,236 @@
+// RunZustand / RunDynamik over the C-ABI (see run_zustand.hpp).
+#include "run_zustand.hpp"
+
+#include <cmath>
+#include <cstdlib>
+#include <iostream>
+#include <stdexcept>
+
+namespace {
+
+void check(bdk_t *h, int rc, const char *what) {
+    if (rc != BDK_OK) {
+        std::cerr << "bdk: " << what << " failed (" << rc << "): " << bdk_last_error(h) << std::endl;
+        std::exit(1);
+    }
+}
+
+void gather_pos(int n, int **git, double **rel, std::vector<int32_t> &g, std::vector<double> &r) {
+    g.resize(2 * (size_t)n);
+    r.resize(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        g[2 * i] = git[i][0];
+        g[2 * i + 1] = git[i][1];
+        r[2 * i] = rel[i][0];
+        r[2 * i + 1] = rel[i][1];
+    }
+}
+
+}  // namespace
+
+RunZustand::RunDynamik::~RunDynamik() {
+    if (h) bdk_destroy(h);
+    if (mt) bdk_mt_destroy(mt);
+}
+
+void RunZustand::RunDynamik::push_positions() {
+    std::vector<int32_t> g;
+    std::vector<double> r;
+    gather_pos(cfg->p.n1, r1_git, r1_rel, g, r);
+    check(h, bdk_set_positions(h, 1, cfg->p.n1, g.data(), r.data()), "set_positions(1)");
+    gather_pos(cfg->p.n2, r2_git, r2_rel, g, r);
+    check(h, bdk_set_positions(h, 2, cfg->p.n2, g.data(), r.data()), "set_positions(2)");
+}
+
+void RunZustand::RunDynamik::sync_positions() {
+    for (int sp = 1; sp <= 2; ++sp) {
+        const int n = sp == 1 ? cfg->p.n1 : cfg->p.n2;
+        std::vector<int32_t> g(2 * (size_t)n);
+        std::vector<double> r(2 * (size_t)n);
+        check(h, bdk_get_positions(h, sp, n, g.data(), r.data()), "get_positions");
+        int **git = sp == 1 ? r1_git : r2_git;
+        double **rel = sp == 1 ? r1_rel : r2_rel;
+        for (int i = 0; i < n; ++i) {
+            git[i][0] = g[2 * i];
+            git[i][1] = g[2 * i + 1];
+            rel[i][0] = r[2 * i];
+            rel[i][1] = r[2 * i + 1];
+        }
+    }
+}
+
+void RunZustand::RunDynamik::init() {
+    if (!h) check(nullptr, bdk_create(&cfg->p, &h), "create");
+    push_positions();
+}
+
+void RunZustand::RunDynamik::berechne_WCA11() {
+    F1_WCA.assign(2 * (size_t)cfg->p.n1, 0.0);
+    F2_WCA.assign(2 * (size_t)cfg->p.n2, 0.0);
+    check(h, bdk_compute_wca(h, F1_WCA.data(), F2_WCA.data()), "compute_wca");
+}
+
+void RunZustand::RunDynamik::berechne_kapkraefte() {
+    F1kap.assign(2 * (size_t)cfg->p.n1, 0.0);
+    F2kap.assign(2 * (size_t)cfg->p.n2, 0.0);
+    check(h, bdk_compute_capillary(h, F1kap.data(), F2kap.data()), "compute_capillary");
+}
+
+void RunZustand::RunDynamik::berechne_zufallskraefte() {
+    if (cfg->p.noise_mode != BDK_NOISE_UPLOAD) return;  // device noise is drawn inside the step
+    std::vector<double> a(2 * (size_t)cfg->p.n1), b(2 * (size_t)cfg->p.n2);
+    bdk_mt_disc_noise(mt, cfg->p.n1, cfg->p.n2, a.data(), b.data());
+    check(h, bdk_set_noise(h, a.data(), b.data()), "set_noise");
+}
+
+double RunZustand::RunDynamik::optimaler_zeitschritt() {
+    double dt = 0.0;
+    check(h, bdk_optimal_dt(h, &dt), "optimal_dt");
+    return dt;
+}
+
+void RunZustand::RunDynamik::refresh_erwNachbar() { check(h, bdk_refresh(h), "refresh"); }
+
+double RunZustand::RunDynamik::zeitschritt(double tmax) {
+    if (tmax <= 0.0) {  // dm:80-83
+        std::cout << "Fehler: Negative Zeitschrittweite " << tmax << " gefordert, breche ab!" << std::endl;
+        return 1.0e10;
+    }
+    if (cfg->p.noise_mode == BDK_NOISE_UPLOAD && !cfg->p.no_wca) berechne_zufallskraefte();
+    double dt = 0.0;
+    check(h, bdk_step(h, tmax, &dt), "step");
+    return dt;
+}
+
+RunZustand::~RunZustand() { obs.close(); }
+
+void RunZustand::init_zufallspos() {
+    bd_params p = cfg_.p;
+    std::vector<int32_t> g1(2 * (size_t)p.n1), g2(2 * (size_t)p.n2);
+    std::vector<double> r1(2 * (size_t)p.n1), r2(2 * (size_t)p.n2);
+    bdk_mt_init_positions(dyn.mt, &p, g1.data(), r1.data(), g2.data(), r2.data());
+    for (int i = 0; i < p.n1; ++i) {
+        r1_git[i][0] = g1[2 * i];
+        r1_git[i][1] = g1[2 * i + 1];
+        r1_rel[i][0] = r1[2 * i];
+        r1_rel[i][1] = r1[2 * i + 1];
+    }
+    for (int i = 0; i < p.n2; ++i) {
+        r2_git[i][0] = g2[2 * i];
+        r2_git[i][1] = g2[2 * i + 1];
+        r2_rel[i][0] = r2[2 * i];
+        r2_rel[i][1] = r2[2 * i + 1];
+    }
+}
+
+void RunZustand::init(int nr_) {
+    nr = nr_;
+    obs_nr = 0;
+    const int n1 = cfg_.p.n1, n2 = cfg_.p.n2;
+    git_store1_.assign(2 * (size_t)n1, 0);
+    git_store2_.assign(2 * (size_t)n2, 0);
+    rel_store1_.assign(2 * (size_t)n1, 0.0);
+    rel_store2_.assign(2 * (size_t)n2, 0.0);
+    git_ptr1_.resize(n1);
+    git_ptr2_.resize(n2);
+    rel_ptr1_.resize(n1);
+    rel_ptr2_.resize(n2);
+    for (int i = 0; i < n1; ++i) {
+        git_ptr1_[i] = &git_store1_[2 * (size_t)i];
+        rel_ptr1_[i] = &rel_store1_[2 * (size_t)i];
+    }
+    for (int i = 0; i < n2; ++i) {
+        git_ptr2_[i] = &git_store2_[2 * (size_t)i];
+        rel_ptr2_[i] = &rel_store2_[2 * (size_t)i];
+    }
+    r1_git = git_ptr1_.data();
+    r2_git = git_ptr2_.data();
+    r1_rel = rel_ptr1_.data();
+    r2_rel = rel_ptr2_.data();
+    dyn.cfg = &cfg_;
+    if (!dyn.mt) bdk_mt_create(cfg_.seed + (uint64_t)nr, &dyn.mt);
+    if (cfg_.startpos_methode != 1) {
+        std::cout << "Fehler: startpos_methode=" << cfg_.startpos_methode
+                  << ", this build implements 1 (random start, dm:1647-1678)" << std::endl;
+        std::exit(1);
+    }
+    init_zufallspos();
+    dyn.r1_git = r1_git;  // am:305-308
+    dyn.r2_git = r2_git;
+    dyn.r1_rel = r1_rel;
+    dyn.r2_rel = r2_rel;
+    obs.r1_git = r1_git;
+    obs.r2_git = r2_git;
+    obs.r1_rel = r1_rel;
+    obs.r2_rel = r2_rel;
+    dyn.init();
+    obs.obs_init(nr, cfg_);
+}
+
+void RunZustand::zeitschritte_bis_obs(time_t last_write) {
+    (void)last_write;
+    if (std::isinf(cfg_.p.max_travel) && cfg_.p.noise_mode != BDK_NOISE_UPLOAD) {
+        // fixed dt: the long-double loop of dm:444-455 has a closed step count
+        int64_t k = 0;
+        check(dyn.h, bdk_run_interval(dyn.h, cfg_.obs_dt, &k), "run_interval");
+        schritte_seit_obs += (int)k;
+        long double tt = 0.0L;
+        for (int64_t i = 0; i < k; ++i) tt += (long double)cfg_.p.dt_max;
+        t = tt;
+        return;
+    }
+    while (t < cfg_.obs_dt) {
+        t += (long double)dyn.zeitschritt((double)((long double)cfg_.obs_dt - t));
+        schritte_seit_obs++;
+    }
+}
+
+void RunZustand::obs_point(int nr_obspoint) {
+    dyn.sync_positions();
+    if (cfg_.auswerten_animation) obs.schreibe_pos(nr_obspoint);
+    schritte_seit_obs = 0;
+    t = 0.0L;
+    obs_nr++;
+}
+
+void RunZustand::RunObs::obs_init(int run, const RunConfig &c) {
+    cfg = &c;
+    double d[6];
+    bdk_params_derived(&c.p, d);
+    B = d[2];
+    if (!c.auswerten_animation) return;
+    path1 = "pos1_" + std::to_string(run + 1) + ".txt";  // au:58-59
+    path2 = c.p.n2 ? "pos2_" + std::to_string(run + 1) + ".txt" : "";
+    FILE *f = std::fopen(path1.c_str(), "w");
+    if (f) {
+        std::fprintf(f, "# Format: t TAB x TAB y \n\n");
+        std::fclose(f);
+    }
+    if (!path2.empty()) {
+        f = std::fopen(path2.c_str(), "w");
+        if (f) {
+            std::fprintf(f, "# Format: t TAB x TAB y \n\n");
+            std::fclose(f);
+        }
+    }
+}
+
+void RunZustand::RunObs::schreibe_pos(int nr) {
+    for (int sp = 1; sp <= 2; ++sp) {
+        const int n = sp == 1 ? cfg->p.n1 : cfg->p.n2;
+        if (sp == 2 && n == 0) return;
+        FILE *f = std::fopen((sp == 1 ? path1 : path2).c_str(), "a");
+        if (!f) return;
+        int **git = sp == 1 ? r1_git : r2_git;
+        double **rel = sp == 1 ? r1_rel : r2_rel;
+        for (int i = 0; i < n; ++i) {
+            double x = rel[i][0] + B * git[i][0];
+            double y = rel[i][1] + B * git[i][1];
+            std::fprintf(f, "%g \t %g \t %g \n", nr * cfg->obs_dt, x, y);
+        }
+        std::fprintf(f, "\n");
+        std::fclose(f);
+    }
+}
+
+void RunZustand::RunObs::close() {}

@@ -1,0 +1,114 @@
+// C++ host shim with the interface of RunZustand / RunZustand::RunDynamik
+// (signaturen2.h:29-434), running the timestep on the GPU through the C-ABI
+// of include/bdk.h.  Drivers written against the reference (main.cpp,
+// main_profile.cpp) keep their calls: init(run), obs_point(nr),
+// zeitschritte_bis_obs(...), dyn.zeitschritt(tmax), dyn.berechne_*(),
+// dyn.getFKap(), dyn.getF().
+//
+// Positions keep the reference's layout, int** r*_git and double** r*_rel
+// of shape [N][2] (s2:380-383).  They are the truth at init() and are
+// refreshed from the device at every obs_point() and by sync_positions().
+#pragma once
+
+#include <ctime>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "bdk.h"
+
+// Run configuration: the constants of a reference-format parameter.h,
+// gathered by params_from_header.cpp (compiled against that header).
+struct RunConfig {
+    bd_params p;
+    int runs;
+    double obs_dt;
+    int obs_anzahl;
+    int startpos_methode;
+    int max_write_interval;
+    bool auswerten_animation;
+    bool krafttestmode;
+    uint64_t seed;      // MT seed (the reference seeds with time(NULL), zufall.cpp:18)
+    int noise_source;   // 0: device Philox disc noise; 1: host MT stream uploaded per step
+};
+
+RunConfig config_from_parameter_h();
+
+class RunZustand {
+  public:
+    class RunDynamik {
+      public:
+        RunDynamik() = default;
+        ~RunDynamik();
+        RunDynamik(const RunDynamik &) = delete;
+        RunDynamik &operator=(const RunDynamik &) = delete;
+
+        // dm:74-163
+        double zeitschritt(double tmax);
+        // am:361-364: lists from the current positions, spectral tables
+        void init();
+        void berechne_kapkraefte();
+        // The device computes the 11, 22 and 12 WCA forces in one pass;
+        // berechne_WCA11 runs it, berechne_WCA22 and addiere_WCA12 keep the
+        // call sequence of dm:86-88 and are no-ops.
+        void berechne_WCA11();
+        void berechne_WCA22() {}
+        void addiere_WCA12() {}
+        void refresh_erwNachbar();
+        void berechne_zufallskraefte();
+        double optimaler_zeitschritt();
+        double getFKap() { return F1kap.size() > 3 ? F1kap[3] : 0.0; }                 // s2:182-185
+        double getF() { return getFKap() + (F1_WCA.size() > 3 ? F1_WCA[3] : 0.0); }   // s2:186-188
+
+        // pull device positions into the aliased host arrays
+        void sync_positions();
+        // push the aliased host arrays to the device (after an external change)
+        void push_positions();
+
+        int **r1_git = nullptr, **r2_git = nullptr;
+        double **r1_rel = nullptr, **r2_rel = nullptr;
+
+        bdk_t *h = nullptr;
+        const RunConfig *cfg = nullptr;
+        bdk_mt_t *mt = nullptr;  // MT stream for the upload noise source
+        std::vector<double> F1kap, F2kap, F1_WCA, F2_WCA;
+    };
+
+    RunDynamik dyn;
+
+    class RunObs {
+      public:
+        int **r1_git = nullptr, **r2_git = nullptr;
+        double **r1_rel = nullptr, **r2_rel = nullptr;
+        void obs_init(int run, const RunConfig &cfg);  // au:52-75
+        void schreibe_pos(int nr);                     // au:631-648
+        void close();
+        std::string path1, path2;
+        const RunConfig *cfg = nullptr;
+        double B = 0.0;
+    };
+    RunObs obs;
+
+    explicit RunZustand(const RunConfig &cfg) : cfg_(cfg) {}
+    ~RunZustand();
+
+    int nr = 0;
+    long double t = 0.0L;
+    int obs_nr = 0;
+    int schritte_seit_obs = 0;
+
+    int **r1_git = nullptr, **r2_git = nullptr;
+    double **r1_rel = nullptr, **r2_rel = nullptr;
+
+    void init(int nr);                            // am:256-320
+    void zeitschritte_bis_obs(time_t last_write);  // dm:444-455
+    void obs_point(int nr_obspoint);              // au:78-93
+
+  private:
+    const RunConfig &cfg_;
+    std::vector<int> git_store1_, git_store2_;
+    std::vector<double> rel_store1_, rel_store2_;
+    std::vector<int *> git_ptr1_, git_ptr2_;
+    std::vector<double *> rel_ptr1_, rel_ptr2_;
+    void init_zufallspos();  // dm:1647-1678
+};

@@ -1,0 +1,168 @@
+"""Benchmark of the MI355X BD timestep: particle-steps/s at 2M particles with a
+2048^2 TSC mesh (L = 1414 sigma, N1 = N2 = 10^6, lambda = 50, q2/q1 = 0.5,
+kT = 1, dt = 1e-4, FP32 forces with FP64 positions), synthetic clipped
+Gaussian-blob start (hexagonal-lattice thinning, exclusion 0.9 sigma).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  The timed region is K graph-replayed steps with the
+particle state resident in HBM, bracketed by stream synchronisation.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bd-simulations_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+WORKLOAD = dict(name="2M particles, TSC 2048^2, L=1414, lambda=50, blob start",
+                n1=1_000_000, n2=1_000_000, L=1414.0, Z=2048, lam=50.0, kap=1.0, f2_f1=0.5, T=1.0,
+                dt=1e-4, scheme=2, blob_width=1414.0 / 6.0, background=0.1, dmin=0.9, seed=1)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel, N, Z, rb=4):
+    """Compulsory HBM bytes of one launch (FP32 forces/mesh, FP64 positions):
+    P = 16 B position state, f = 8 B force pair, g = 4 B mesh real."""
+    P, f, g = 16, 8, rb
+    table = {
+        # read positions, write WCA force and the deposited mesh
+        "k_force": P * N + f * N + g * Z * Z,
+        # read real mesh, write the half spectrum (Z x Z/2 complex)
+        "k_r2c_rows": g * Z * Z + g * Z * Z,
+        # read half spectrum, write the Fx and Fy half spectra
+        "k_col": g * Z * Z + 2 * g * Z * Z,
+        # read both half spectra, write the real (Fx, Fy) mesh
+        "k_c2r_rows": 2 * g * Z * Z + 2 * g * Z * Z,
+        # read positions + WCA force + (Fx, Fy) mesh, write positions
+        "k_update": (P + f) * N + 2 * g * Z * Z + P * N,
+    }
+    return table.get(kernel)
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), scaled by the counters' reading for a copy kernel of known
+    size; committed under profiles/ as JSON."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        k = d["kernels"].get(kernel)
+        if not k:
+            return None
+        return (k["fetch_kib"] + k["write_kib"]) * 1024.0 * d.get("copy_scale", 1.0)
+    except Exception:
+        return None
+
+
+def cpu_baseline(p, git, rel, budget_s=20.0):
+    """The FP64 CPU restatement (oracle/, one thread, like the reference's
+    single-threaded build) timed on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.build()
+    op = O.make_params(n1=p.n1, n2=p.n2, L=p.L, Z=p.Z, scheme=p.scheme, lam=p.lam, kap=p.kap, T=p.T,
+                       dt_max=p.dt_max, mrd=float("inf"), f2_f1=p.f2_f1, noise_mode=1, seed=p.seed)
+    o = O.Oracle(op)
+    o.set_all(git, rel)
+    log("cpu baseline: warm-up step")
+    t0 = time.perf_counter()
+    o.step(1.0)
+    t1 = time.perf_counter()
+    per = t1 - t0
+    n = max(1, min(20, int(budget_s / max(per, 1e-3))))
+    log("cpu baseline: %d timed steps (%.2f s per step)" % (n, per))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        o.step(1.0)
+    dt = time.perf_counter() - t0
+    o.close()
+    N = p.n1 + p.n2
+    return {"value": N * n / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "FP64 C restatement, 1 thread, %d steps after 1 warm-up of the same 2M-particle "
+                      "TSC 2048^2 workload (%.1f s); host %s" % (n, dt, platform.processor() or platform.machine())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs one simulation on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import bdk
+
+    w = WORKLOAD
+    p = bdk.make_params(n1=w["n1"], n2=w["n2"], L=w["L"], Z=w["Z"], scheme=w["scheme"], lam=w["lam"],
+                        kap=w["kap"], f2_f1=w["f2_f1"], T=w["T"], dt_max=w["dt"], precision=bdk.FP32,
+                        noise_mode=bdk.NOISE_DISC, seed=w["seed"])
+    log("bench: initial state (blob, %d particles)" % (w["n1"] + w["n2"]))
+    git, rel, clipped = bdk.init_blob(p, w["seed"], w["blob_width"], w["background"], w["dmin"])
+    sim = bdk.RunDynamik(p)
+    sim.set_positions(git, rel)
+    N = w["n1"] + w["n2"]
+    log("bench: warm-up %d steps" % args.warmup)
+    sim.run_steps(args.warmup)
+    sim.sync()
+    log("bench: timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    sim.run_steps(args.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = N * args.steps / elapsed
+
+    log("bench: per-kernel timing (%d steps, HIP events)" % args.profile_steps)
+    prof = sim.profile(args.profile_steps)
+    kernels = {k: ms for k, ms in prof}
+    dom = max(kernels, key=kernels.get)
+    ab = algorithmic_bytes(dom, N, w["Z"])
+    achieved = ab / (kernels[dom] * 1e-3) / 1e9 if ab else None
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": (achieved / PEAK_HBM_GBS) if achieved else None, "traffic": load_traffic(dom),
+            "algorithmic_bytes": ab, "avg_ms": kernels[dom]}
+    step_bytes = sum(algorithmic_bytes(k, N, w["Z"]) or 0 for k in kernels)
+    out = {
+        "metric": "particle-steps/s at 2M particles, 2048^2 TSC mesh",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 forces / f64 positions", "data": "synthetic (seeded clipped Gaussian blob, %.1f%% clipped)"
+                                                      % (100 * clipped),
+        "config": {"workload": w["name"], "n_particles": N, "mesh": "%dx%d" % (w["Z"], w["Z"]),
+                   "scheme": "TSC", "L": w["L"], "lambda": w["lam"], "dt": w["dt"], "kT": w["T"],
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "kernels_ms": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(p, git, rel)
+    else:
+        out["cpu_baseline"] = None
+    sim.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * bdk.h -- C-ABI of the MI355X Brownian-dynamics timestep (libbdk_hip.so).
+ *
+ * Drop-in boundary for RunZustand::RunDynamik of Gully85/BD-Simulations
+ * (signaturen2.h:33-252).  Plain pointers and sizes only.  Every call returns
+ * an int status (BDK_OK = 0); the message of the last failure on a handle is
+ * bdk_last_error(h) (bdk_last_error(NULL) for bdk_create failures).
+ *
+ * Threading: one handle per host thread.  Each handle owns its HIP stream
+ * and its device buffers.  Host arrays belong to the caller.
+ *
+ * Layout at the boundary (the reference's, signaturen2.h:197-233):
+ *   positions  git [n][2] int32 (cell index x, y) and rel [n][2] double
+ *              (offset inside the cell), x = git*B + rel, per species;
+ *   forces     [n][2] double per species;
+ *   meshes     Z*Z double, row-major [x][y] (iw(i,j) = i*Z + j, am:242-250).
+ */
+#ifndef BDK_H
+#define BDK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bdk_status {
+    BDK_OK = 0,
+    BDK_ERR_ARG = 1,       /* invalid argument / parameter */
+    BDK_ERR_HIP = 2,       /* HIP runtime failure */
+    BDK_ERR_STATE = 3,     /* call out of order (e.g. positions not set) */
+    BDK_ERR_NOMEM = 4,     /* device footprint does not fit */
+    BDK_ERR_NONFINITE = 5  /* non-finite particle position detected */
+};
+
+enum bdk_scheme { BDK_NGP = 0, BDK_CIC = 1, BDK_TSC = 2 };     /* densGrid_Schema, ph:22 */
+enum bdk_precision { BDK_FP64 = 0, BDK_FP32 = 1 };
+enum bdk_noise {
+    BDK_NOISE_DISC = 0,    /* uniform on the disc of radius sqrt 5 (dm:1317-1339), Philox */
+    BDK_NOISE_GAUSS = 1,   /* Gaussian of variance noise_var, Philox */
+    BDK_NOISE_UPLOAD = 2   /* caller-supplied per step (bdk_set_noise), e.g. the MT stream */
+};
+enum bdk_green {
+    BDK_GREEN_DISCRETE = 0,   /* reference symbols: discrete Laplacian + central difference (dm:585-593, 1384-1394) */
+    BDK_GREEN_CONTINUOUS = 1  /* i k / (k^2 + lambda^-2) */
+};
+enum bdk_mesh { BDK_MESH_RHO = 0, BDK_MESH_FX = 1, BDK_MESH_FY = 2 };
+
+/* parameter.h:14-99 user constants, plus the build's own switches. */
+typedef struct bd_params {
+    int32_t n1, n2;          /* N1, N2 (ph:15-16) */
+    int32_t scheme;          /* densGrid_Schema (ph:22) */
+    int32_t Z;               /* FFT_samples, power of two, 8..4096 (ph:25) */
+    double L;                /* box side (ph:19) */
+    double lambda;           /* lambda_kapillar (ph:29) */
+    double kap;              /* kapillar_vorfaktor (ph:32) */
+    double T;                /* temperature (ph:37) */
+    double dt_max;           /* dt_max (ph:41) */
+    double max_travel;       /* max_reisedistanz (ph:44); +inf gives dt = dt_max every step */
+    double sigma11_22, sigma11_12, eps22_11, eps12_11; /* ph:47-59 */
+    double f2_f1;            /* species-2 charge ratio (ph:65) */
+    double max_pair_wca;     /* maxPairWCA, cap on |F|/r (ph:81) */
+    int32_t idgas_only;      /* ph:68 */
+    int32_t no_wca;          /* noWCA (ph:70): switches the noise off (dm:90-101) */
+    int32_t restrict_radial; /* restrictRadial (ph:76) */
+    int32_t ngp_fix;         /* 0: reference NGP species-2 y force (gr:123, ~0); 1: use Fy */
+    int32_t precision;       /* BDK_FP64 or BDK_FP32 (forces, mesh, FFT; positions stay FP64) */
+    int32_t noise_mode;      /* bdk_noise */
+    int32_t green_mode;      /* bdk_green */
+    int32_t device;          /* HIP device ordinal */
+    double noise_var;        /* variance per component for BDK_NOISE_GAUSS */
+    uint64_t seed;           /* Philox key */
+} bd_params;
+
+typedef struct bdk bdk_t;
+
+/* Defaults: parameter.h's switches off, sigma/eps ratios 1, maxPairWCA 1e5,
+   fixed dt (max_travel = +inf), FP32, disc noise, discrete symbols. */
+int bdk_params_default(bd_params *p);
+
+/* Derived constants of parameter.h:170-191: out[0]=2^(1/6), [1]=nachList_Zellen,
+   [2]=nachList_Breite, [3]=densGrid_Breite, [4]=dq, [5]=drho (gr:28). */
+int bdk_params_derived(const bd_params *p, double *out6);
+
+/* RunZustand::RunDynamik constructor + init() (am:361-364, dm:459-613):
+   validates p, allocates device state, builds the spectral tables. */
+int bdk_create(const bd_params *p, bdk_t **out);
+void bdk_destroy(bdk_t *h);
+const char *bdk_last_error(const bdk_t *h);
+
+/* Positions of one species (the aliases set at am:305-308).  Setting both
+   species (re)builds the cell lists, as WCA_init does (dm:459-526). */
+int bdk_set_positions(bdk_t *h, int species, int32_t n, const int32_t *git, const double *rel);
+int bdk_get_positions(bdk_t *h, int species, int32_t n, int32_t *git, double *rel);
+
+/* berechne_WCA11 + berechne_WCA22 + addiere_WCA12 (dm:617-702, 790-879,
+   973-1059): WCA forces at the current positions; f1 [n1][2], f2 [n2][2]. */
+int bdk_compute_wca(bdk_t *h, double *f1, double *f2);
+
+/* berechne_kapkraefte (dm:1180-1233): capillary forces (deposit, FFT,
+   Green's function and gradient, inverse FFT, gather). */
+int bdk_compute_capillary(bdk_t *h, double *f1, double *f2);
+
+/* Noise for the next step in BDK_NOISE_UPLOAD mode, [n1][2] and [n2][2]
+   (the arrays berechne_zufallskraefte fills, dm:1317-1339). */
+int bdk_set_noise(bdk_t *h, const double *n1, const double *n2);
+
+/* Noise that the next step will use (Philox modes: a pure function of
+   seed, particle and step counter). */
+int bdk_get_noise(bdk_t *h, double *n1, double *n2);
+
+/* optimaler_zeitschritt (dm:1420-1491) on the forces and noise of the
+   current state, without stepping. */
+int bdk_optimal_dt(bdk_t *h, double *dt);
+
+/* zeitschritt(tmax) (dm:74-163): one step; *dt_out = the step taken.
+   tmax <= 0 reproduces dm:80-83: no step, *dt_out = 1e10, BDK_ERR_ARG. */
+int bdk_step(bdk_t *h, double tmax, double *dt_out);
+
+/* k steps (hipGraph replay when dt is fixed); *t_out = sum of the dt's,
+   accumulated in long double as zeitschritte_bis_obs does (dm:444-455). */
+int bdk_run_steps(bdk_t *h, int64_t k, double *t_out);
+
+/* zeitschritte_bis_obs (dm:444-455): steps until the long-double clock,
+   starting at 0 (obs_point, au:90), reaches obs_dt. */
+int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out);
+
+/* Number of steps zeitschritte_bis_obs takes for a constant dt. */
+int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out);
+
+/* refresh_erwNachbar (dm:216-325) on the current positions. */
+int bdk_refresh(bdk_t *h);
+
+int bdk_sync(bdk_t *h);
+int64_t bdk_step_counter(const bdk_t *h);
+
+/* Cell membership (the per-cell content behind erwNachbarn1/2, s2:215-216):
+   start [nc*nc+1], ids [n1+n2] = particle ids (species 1: 0..n1-1, species 2:
+   n1..n1+n2-1) of cell c in ids[start[c]..start[c+1]), ascending. */
+int bdk_get_cells(bdk_t *h, int32_t *start, int32_t *ids);
+
+/* Mesh arrays of the last capillary evaluation, Z*Z double, [x][y]:
+   BDK_MESH_RHO = rhox[..][0] (gr:31-547), FX/FY = Fx/Fy[..][0] (dm:1221-1222). */
+int bdk_get_mesh(bdk_t *h, int which, double *out);
+
+/* Device memory this parameter set needs, bytes (admission by footprint). */
+int bdk_footprint(const bd_params *p, uint64_t *bytes);
+
+/* Per-kernel timing of k steps with HIP events on the handle's stream.
+   names: buffer for up to nk '\n'-separated kernel names; ms [nk] average
+   milliseconds per launch; *n_out = number of kernels in one step. */
+int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, int32_t names_cap,
+                      int32_t *n_out);
+
+/* ---- initial states (RunZustand::init, am:256-320) ---- */
+
+/* Host mt19937_64 stream of zufall.cpp:18-39 (uniform_real_distribution over
+   a seeded engine; the reference seeds with time(NULL)). */
+typedef struct bdk_mt bdk_mt_t;
+int bdk_mt_create(uint64_t seed, bdk_mt_t **out);
+void bdk_mt_destroy(bdk_mt_t *g);
+double bdk_mt_uniform(bdk_mt_t *g, double lo, double hi);
+/* init_zufallspos (dm:1647-1678), incl. its ((int)x)/B cell index. */
+int bdk_mt_init_positions(bdk_mt_t *g, const bd_params *p, int32_t *git1, double *rel1,
+                          int32_t *git2, double *rel2);
+/* berechne_zufallskraefte (dm:1317-1339): disc noise by rejection, S1 then S2. */
+int bdk_mt_disc_noise(bdk_mt_t *g, int32_t n1, int32_t n2, double *out1, double *out2);
+
+/* Uniform random sequential insertion with exclusion distance dmin
+   (periodic), seeded; all particles, species 1 first. */
+int bdk_init_uniform(const bd_params *p, uint64_t seed, double dmin, int32_t *git, double *rel);
+/* Gaussian blob (std dev `width`, centred) plus a uniform background fraction
+   `background`, by seeded thinning of a hexagonal lattice of spacing dmin:
+   exclusion dmin holds exactly; target densities above the lattice density
+   2/(sqrt3 dmin^2) are clipped to it.  *clipped_out = fraction of the target
+   profile's particles lost to the clip (before renormalisation). */
+int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double background, double dmin,
+                  int32_t *git, double *rel, double *clipped_out);
+
+/* pos1_<run+1>.txt / pos2_<run+1>.txt writer (au:57-69, au:631-648):
+   header on nr == 0 with truncate, then one block per checkpoint. */
+int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
