@@ -81,6 +81,7 @@ def lib():
             "bdk_get_mesh": (C.c_int, [vp, C.c_int, vp]),
             "bdk_footprint": (C.c_int, [vp, vp]),
             "bdk_profile_steps": (C.c_int, [vp, i64, i32, vp, C.c_char_p, i32, vp]),
+            "bdk_profile_kernel": (C.c_int, [vp, C.c_int, i32, vp]),
             "bdk_mt_create": (C.c_int, [u64, C.POINTER(vp)]),
             "bdk_mt_destroy": (None, [vp]),
             "bdk_mt_uniform": (dbl, [vp, dbl, dbl]),
@@ -103,7 +104,8 @@ def exported_symbols():
             "bdk_set_positions", "bdk_get_positions", "bdk_compute_wca", "bdk_compute_capillary",
             "bdk_set_noise", "bdk_get_noise", "bdk_optimal_dt", "bdk_step", "bdk_run_steps",
             "bdk_run_interval", "bdk_steps_for_interval", "bdk_refresh", "bdk_sync", "bdk_step_counter",
-            "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_mt_create",
+            "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_profile_kernel",
+            "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos"]
 
@@ -340,6 +342,16 @@ class RunDynamik:
         self._chk(self.L.bdk_profile_steps(self.h, k, 64, _p(ms), names, 4096, C.byref(n)))
         labels = names.value.decode().split("\n")
         return list(zip(labels, ms[: n.value].tolist()))
+
+    KERNELS = {"k_force": 0, "k_force_wca": 1, "k_force_dep": 2, "k_r2c_rows": 3, "k_col": 4,
+               "k_c2r_rows": 5, "k_gather": 6, "step": 7}
+
+    def profile_kernel(self, name, reps=20):
+        """ms per launch of one kernel, reps back-to-back launches between
+        two HIP events on the simulation's stream."""
+        ms = C.c_double(0.0)
+        self._chk(self.L.bdk_profile_kernel(self.h, self.KERNELS[name], reps, C.byref(ms)))
+        return ms.value
 
     def write_pos(self, path1, path2, nr, obs_dt):
         self._chk(self.L.bdk_write_pos(self.h, path1.encode(), path2.encode() if path2 else None, nr, obs_dt))

@@ -785,6 +785,45 @@ int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, 
     return check_nonfinite(h);
 }
 
+int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (reps < 1 || kernel < 0 || kernel > 7 || !ms_per_launch) return fail(h, BDK_ERR_ARG, "bad profile request");
+    if (kernel == BDK_K_STEP && !h->g.adaptive && h->p.noise_mode != BDK_NOISE_UPLOAD) {
+        rc = ensure_graph(h);
+        if (rc) return rc;
+    }
+    auto launch = [&]() -> hipError_t {
+        switch (kernel) {
+        case BDK_K_FORCE: return bdkx::launch_force(h, 1 | 2);
+        case BDK_K_FORCE_WCA: return bdkx::launch_force(h, 1);
+        case BDK_K_FORCE_DEP: return bdkx::launch_force(h, 2);
+        case BDK_K_R2C: return bdkx::launch_fft_pass(h, 0);
+        case BDK_K_COL: return bdkx::launch_fft_pass(h, 1);
+        case BDK_K_C2R: return bdkx::launch_fft_pass(h, 2);
+        case BDK_K_GATHER: return bdkx::launch_gather(h, 0);
+        default:
+            if (h->gexec) return hipGraphLaunch(h->gexec, h->stream);
+            return bdkx::enqueue_step(h, nullptr, nullptr);
+        }
+    };
+    HIPCHK(h, launch());  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    for (int i = 0; i < reps; ++i) HIPCHK(h, launch());
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel == BDK_K_STEP) h->host_step += reps + 1;
+    *ms_per_launch = ms / reps;
+    return check_nonfinite(h);
+}
+
 // ------------------------------------------------------------------ initial states
 struct bdk_mt {
     std::mt19937_64 engine;

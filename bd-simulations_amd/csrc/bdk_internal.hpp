@@ -103,6 +103,7 @@ namespace bdkx {
 // what: bit 0 = WCA, bit 1 = deposit
 hipError_t launch_force(bdk_t *h, int what);
 hipError_t launch_fft(bdk_t *h);                  // rho -> fmesh (r2c, multiply, c2r)
+hipError_t launch_fft_pass(bdk_t *h, int pass);   // 0 r2c rows, 1 columns, 2 c2r rows
 hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)
 hipError_t launch_update(bdk_t *h, int fused);    // noise, Euler step, refresh, counting
 hipError_t launch_bin(bdk_t *h);                  // counting after an upload/refresh

@@ -360,6 +360,7 @@ __device__ void force_tile_direct(const Geo &g, const DevBufs &d, int cx0, int c
                         int dk = wrapc(k - kp + Z / 2, Z) - Z / 2, dl = wrapc(l - lp + Z / 2, Z) - Z / 2;
                         if (dk < -1 || dk > 1 || dl < -1 || dl > 1) continue;
                         if (SCHEME == 0) {
+                            if (dk != 0 || dl != 0) continue;  // NGP: the particle's own mesh cell only
                             if (pj < g.n1) acc += (R)g.drho;
                             else acc += (R)(g.drho * g.f2_f1);
                         } else {
@@ -526,6 +527,7 @@ __global__ __launch_bounds__(FT_THREADS) void k_force(Geo g, DevBufs d, int what
                         if (dk < -1 || dk > 1 || dl < -1 || dl > 1) continue;
                         const bool s2 = S.pid[u] >= g.n1;
                         if (SCHEME == 0) {
+                            if (dk != 0 || dl != 0) continue;  // NGP: the particle's own mesh cell only
                             acc += s2 ? drho2 : drho;
                         } else {
                             R c = S.wx[u][dk + 1] * S.wy[u][dl + 1] * drho;
@@ -1153,19 +1155,30 @@ hipError_t launch_force(bdk_t *h, int what) {
     return hipGetLastError();
 }
 
-template <typename R> static void launch_fft_t(bdk_t *h) {
+template <typename R> static void launch_fft_t(bdk_t *h, int only = -1) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     const int Z = h->g.Z, logZ = h->g.logZ;
     FftCfg<R> c = fft_cfg<R>(Z);
     const R *sq = std::is_same<R, float>::value ? (const R *)h->d.sqf : (const R *)h->d.sq;
     const R *lap = std::is_same<R, float>::value ? (const R *)h->d.lapf : (const R *)h->d.lap;
-    hipLaunchKernelGGL((k_r2c_rows<R>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream, (const R *)h->d.rho,
-                       (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1);
-    hipLaunchKernelGGL((k_col<R>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream, (const C *)h->d.ht,
-                       (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC);
-    hipLaunchKernelGGL((k_c2r_rows<R>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream, (const C4 *)h->d.gt,
-                       (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2);
+    if (only < 0 || only == 0)
+        hipLaunchKernelGGL((k_r2c_rows<R>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream,
+                           (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1);
+    if (only < 0 || only == 1)
+        hipLaunchKernelGGL((k_col<R>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream,
+                           (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC);
+    if (only < 0 || only == 2)
+        hipLaunchKernelGGL((k_c2r_rows<R>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream,
+                           (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2);
+}
+
+hipError_t launch_fft_pass(bdk_t *h, int pass) {
+    if (h->p.precision == BDK_FP32)
+        launch_fft_t<float>(h, pass);
+    else
+        launch_fft_t<double>(h, pass);
+    return hipGetLastError();
 }
 
 hipError_t launch_fft(bdk_t *h) {

@@ -152,6 +152,23 @@ int bdk_footprint(const bd_params *p, uint64_t *bytes);
 int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, int32_t names_cap,
                       int32_t *n_out);
 
+/* Average duration (ms) of one launch of a single kernel of the step,
+   timed as `reps` back-to-back launches between two HIP events on the
+   handle's stream.  kernel: BDK_K_* below.  The kernels timed this way are
+   idempotent (they read the state and overwrite their outputs), except
+   BDK_K_STEP, which advances the simulation by `reps` full steps. */
+enum bdk_kernel_id {
+    BDK_K_FORCE = 0,      /* WCA + deposit */
+    BDK_K_FORCE_WCA = 1,  /* WCA part only */
+    BDK_K_FORCE_DEP = 2,  /* deposit part only */
+    BDK_K_R2C = 3,        /* row r2c FFT pass */
+    BDK_K_COL = 4,        /* column FFT + Green's function + inverse column FFTs */
+    BDK_K_C2R = 5,        /* row c2r FFT pass */
+    BDK_K_GATHER = 6,     /* capillary gather (no update) */
+    BDK_K_STEP = 7        /* one full step (graph replay when dt is fixed) */
+};
+int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch);
+
 /* ---- initial states (RunZustand::init, am:256-320) ---- */
 
 /* Host mt19937_64 stream of zufall.cpp:18-39 (uniform_real_distribution over

