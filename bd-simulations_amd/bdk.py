@@ -343,8 +343,8 @@ class RunDynamik:
         labels = names.value.decode().split("\n")
         return list(zip(labels, ms[: n.value].tolist()))
 
-    KERNELS = {"k_force": 0, "k_force_wca": 1, "k_force_dep": 2, "k_r2c_rows": 3, "k_col": 4,
-               "k_c2r_rows": 5, "k_gather": 6, "step": 7}
+    KERNELS = {"k_wca": 0, "k_deposit": 1, "k_r2c_rows": 2, "k_col": 3, "k_c2r_rows": 4, "k_gather": 5,
+               "step": 6, "k_update": 7}
 
     def profile_kernel(self, name, reps=20):
         """ms per launch of one kernel, reps back-to-back launches between

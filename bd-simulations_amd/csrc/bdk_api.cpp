@@ -86,6 +86,12 @@ std::string validate(const bd_params &p) {
     if (p.noise_mode < 0 || p.noise_mode > 2) return "noise_mode must be DISC, GAUSS or UPLOAD";
     if (p.green_mode < 0 || p.green_mode > 1) return "green_mode must be DISCRETE or CONTINUOUS";
     if (p.noise_mode == BDK_NOISE_GAUSS && !(p.noise_var >= 0.0)) return "noise_var must be >= 0";
+    {
+        // the deposit owns at most (floor(B/h)+2)^2 mesh points per list cell tile in LDS
+        const double side = std::floor(d.B / d.h) + 2.0;
+        if (side * side * (p.precision == BDK_FP64 ? 16.0 : 8.0) > 120.0 * 1024.0)
+            return "mesh spacing L/Z is too fine relative to the list cell (B/h > 84)";
+    }
     return "";
 }
 
@@ -533,7 +539,7 @@ int bdk_compute_wca(bdk_t *h, double *f1, double *f2) {
     if (h->g.idgas) {
         HIPCHK(h, hipMemsetAsync(h->d.fwca, 0, (size_t)h->g.N * 2 * real_size(h->p), h->stream));
     } else {
-        HIPCHK(h, bdkx::launch_force(h, 1));
+        HIPCHK(h, bdkx::launch_wca(h));
     }
     return download_forces(h, h->d.fwca, f1, f2);
 }
@@ -544,7 +550,7 @@ int bdk_compute_capillary(bdk_t *h, double *f1, double *f2) {
     if (h->g.idgas) {
         HIPCHK(h, hipMemsetAsync(h->d.fkap, 0, (size_t)h->g.N * 2 * real_size(h->p), h->stream));
     } else {
-        HIPCHK(h, bdkx::launch_force(h, 2));
+        HIPCHK(h, bdkx::launch_deposit(h, 0));
         HIPCHK(h, bdkx::launch_fft(h));
         HIPCHK(h, bdkx::launch_gather(h, 0));
     }
@@ -586,9 +592,10 @@ int bdk_optimal_dt(bdk_t *h, double *dt) {
     if (h->g.idgas) {
         HIPCHK(h, hipMemsetAsync(h->d.fwca, 0, (size_t)h->g.N * 2 * real_size(h->p), h->stream));
         HIPCHK(h, hipMemsetAsync(h->d.fmesh, 0, (size_t)h->g.Z * h->g.Z * 2 * real_size(h->p), h->stream));
-        HIPCHK(h, bdkx::launch_force(h, 4));
+        HIPCHK(h, bdkx::launch_rearm(h));
     } else {
-        HIPCHK(h, bdkx::launch_force(h, 1 | 2 | 4));
+        HIPCHK(h, bdkx::launch_wca(h));
+        HIPCHK(h, bdkx::launch_deposit(h, 1));
         HIPCHK(h, bdkx::launch_fft(h));
     }
     HIPCHK(h, bdkx::launch_gather(h, 1));
@@ -712,7 +719,7 @@ int bdk_get_cells(bdk_t *h, int32_t *start, int32_t *ids) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (start) std::memcpy(start, st.data(), (size_t)(ncell + 1) * 4);
     if (ids) {
-        for (int q = 0; q < N; ++q) ids[q] = pid[list[q]];
+        for (int q = 0; q < N; ++q) ids[q] = pid[list[q] & 0x7fffffff];
         for (int c = 0; c < ncell; ++c) std::sort(ids + st[c], ids + st[c + 1]);
     }
     return BDK_OK;
@@ -789,19 +796,40 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
     int rc = ensure(h);
     if (rc) return rc;
     if (reps < 1 || kernel < 0 || kernel > 7 || !ms_per_launch) return fail(h, BDK_ERR_ARG, "bad profile request");
+    // k_update changes positions, cells, ranks, counts and scalars: snapshot them
+    struct Snap {
+        void *p;
+        void *src;
+        size_t n;
+    };
+    std::vector<Snap> snaps;
+    if (kernel == BDK_K_UPDATE) {
+        const size_t N = (size_t)h->g.N, nc2 = (size_t)h->g.ncell;
+        Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.rank, N * 4},
+                       {nullptr, h->d.count, nc2 * 4}, {nullptr, h->d.sc, sizeof(DevScalars)}};
+        for (Snap sn : list) {
+            HIPCHK(h, hipMalloc(&sn.p, sn.n));
+            HIPCHK(h, hipMemcpyAsync(sn.p, sn.src, sn.n, hipMemcpyDeviceToDevice, h->stream));
+            snaps.push_back(sn);
+        }
+        // forces of the current state for the update to consume
+        HIPCHK(h, bdkx::launch_wca(h));
+        HIPCHK(h, bdkx::launch_deposit(h, 1));
+        HIPCHK(h, bdkx::launch_fft(h));
+    }
     if (kernel == BDK_K_STEP && !h->g.adaptive && h->p.noise_mode != BDK_NOISE_UPLOAD) {
         rc = ensure_graph(h);
         if (rc) return rc;
     }
     auto launch = [&]() -> hipError_t {
         switch (kernel) {
-        case BDK_K_FORCE: return bdkx::launch_force(h, 1 | 2);
-        case BDK_K_FORCE_WCA: return bdkx::launch_force(h, 1);
-        case BDK_K_FORCE_DEP: return bdkx::launch_force(h, 2);
+        case BDK_K_WCA: return bdkx::launch_wca(h);
+        case BDK_K_DEPOSIT: return bdkx::launch_deposit(h, 0);
         case BDK_K_R2C: return bdkx::launch_fft_pass(h, 0);
         case BDK_K_COL: return bdkx::launch_fft_pass(h, 1);
         case BDK_K_C2R: return bdkx::launch_fft_pass(h, 2);
         case BDK_K_GATHER: return bdkx::launch_gather(h, 0);
+        case BDK_K_UPDATE: return bdkx::launch_update(h, h->g.adaptive ? 0 : 1);
         default:
             if (h->gexec) return hipGraphLaunch(h->gexec, h->stream);
             return bdkx::enqueue_step(h, nullptr, nullptr);
@@ -820,6 +848,11 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (kernel == BDK_K_STEP) h->host_step += reps + 1;
+    for (Snap &sn : snaps) {
+        HIPCHK(h, hipMemcpyAsync(sn.src, sn.p, sn.n, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        (void)hipFree(sn.p);
+    }
     *ms_per_launch = ms / reps;
     return check_nonfinite(h);
 }

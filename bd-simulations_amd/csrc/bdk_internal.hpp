@@ -100,8 +100,10 @@ struct bdk {
 namespace bdkx {
 
 // Launchers implemented in bdk_kernels.hip.  All enqueue on h->stream.
-// what: bit 0 = WCA, bit 1 = deposit
-hipError_t launch_force(bdk_t *h, int what);
+hipError_t launch_wca(bdk_t *h);                  // WCA forces -> fwca
+hipError_t launch_deposit(bdk_t *h, int reset);   // charges -> rho (reset: re-arm per-step scalars)
+hipError_t launch_rearm(bdk_t *h);                // per-step scalars when no deposit runs
+size_t deposit_lds_bytes(const bdk_t *h);          // LDS of one deposit block
 hipError_t launch_fft(bdk_t *h);                  // rho -> fmesh (r2c, multiply, c2r)
 hipError_t launch_fft_pass(bdk_t *h, int pass);   // 0 r2c rows, 1 columns, 2 c2r rows
 hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)

@@ -1,8 +1,9 @@
 // HIP kernels of the MI355X Brownian-dynamics timestep (gfx950).
 //
 // One step (dynamik_methoden.cpp:74-163) is this launch sequence:
-//   k_force     WCA forces (dm:617-1059) and the charge deposit (gr:31-547),
-//               per tile of 16x16 list cells, particles staged in LDS;
+//   k_wca       WCA forces (dm:617-1059), one thread per particle;
+//   k_deposit   charge deposit (gr:31-547) per tile of list cells into the
+//               tile's own mesh points, integer LDS accumulation;
 //   k_r2c_rows  row FFTs of the real mesh, transposed half-spectrum out;
 //   k_col       column FFT, Green's function x gradient (dm:1210-1216), two
 //               inverse column FFTs (Fx, Fy);
@@ -10,7 +11,8 @@
 //   k_update    gather (gr:81-668), noise (dm:1317-1339), Euler step
 //               (dm:116-157), refresh (dm:216-325), cell counting;
 //   k_scan_*    exclusive scan of the cell counts;
-//   k_fill      cell list = slots grouped by cell.
+//   k_fill      cell list = slots grouped by cell, species in bit 31;
+//   k_sort_cells  each cell's entries in slot order.
 // With a finite max_travel, k_gather (forces + dt min-reduction, dm:1420-1491)
 // runs before a non-fused k_update.
 //
@@ -129,6 +131,41 @@ __device__ __forceinline__ void weights1d(double x, double h, int &k, double w[3
     }
 }
 
+// Weights in the precision of the forces: FP64 follows the reference's
+// operations exactly; FP32 keeps the mesh index exact (x/h in FP64) and
+// evaluates the offset t = d/h and the textbook weights (CIC: 1-|t|, -t, t;
+// TSC: 3/4-t^2, (1/2-t)^2/2, (1/2+t)^2/2) in FP32.
+template <typename R, int SCHEME>
+__device__ __forceinline__ void weights_r(double x, double h, int &k, R w[3]) {
+    if (std::is_same<R, double>::value) {
+        double wd[3];
+        weights1d<SCHEME>(x, h, k, wd);
+        w[0] = (R)wd[0];
+        w[1] = (R)wd[1];
+        w[2] = (R)wd[2];
+        return;
+    }
+    const int kk = (int)(x / h);
+    k = kk;
+    if (SCHEME == 0) {
+        w[0] = 0;
+        w[1] = 1;
+        w[2] = 0;
+        return;
+    }
+    const float t = (float)(x / h - (kk + 0.5));
+    if (SCHEME == 1) {
+        w[1] = 1.0f - fabsf(t);
+        w[0] = t < 0.0f ? -t : 0.0f;
+        w[2] = t < 0.0f ? 0.0f : t;
+    } else {
+        const float a = 0.5f - t, b = 0.5f + t;
+        w[1] = 0.75f - t * t;
+        w[0] = 0.5f * a * a;
+        w[2] = 0.5f * b * b;
+    }
+}
+
 // One WCA pair, the arithmetic of dm:640-698 (type 0, pair 11), dm:815-868
 // (type 1, pair 22) and dm:987-1040 (type 2, pair 12): force on particle a
 // at (xa,ya) from b at (xb,yb).  Minimum image as abstand2 (am:100-166) and
@@ -222,35 +259,117 @@ __device__ int block_exscan4(int *a, int n, int *wsum) {
     return total;
 }
 
-// ------------------------------------------------------------------ k_force
-// WCA forces and charge deposit for one tile of FT_TC x FT_TC list cells.
-// The tile's cells plus a halo of hx cells are staged in LDS, particles of
-// each cell in ascending particle id, so every sum has a fixed order and
-// the result is deterministic (no floating-point atomics).  Mesh points are
-// owned by the tile whose cells contain their centres; each owned point
-// sums the contributions of the staged particles whose 3x3 stencil covers
-// it (gather form of the deposit).
-constexpr int FT_TC = 16;
-constexpr int FT_HX = 2;
-constexpr int FT_S = FT_TC + 2 * FT_HX;
-constexpr int FT_NCS = FT_S * FT_S;
-constexpr int FT_MAXP = 1024;
-constexpr int FT_THREADS = 256;
+// ------------------------------------------------------------------ WCA
+// One thread per particle (slot).  Neighbours come from the cell list of
+// the 3x3 cells around the particle's own cell (the extended-list semantics
+// of dm:459-526): entries are slot | species << 31, each cell's entries
+// sorted by slot, so every sum has a fixed order.  Same-species and
+// cross-species contributions are summed separately and then added, the
+// grouping of dm:617-702 + dm:973-1059.
+constexpr unsigned LIST_SLOT = 0x7fffffffu;
 
-template <typename R> struct ForceLDS {
-    int cstart[FT_NCS + 1];
-    int cwrap[FT_NCS];
-    int wsum[16];
-    int info[8];
-    double x[FT_MAXP], y[FT_MAXP];
-    int slot[FT_MAXP];
-    int pid[FT_MAXP];
-    int kp[FT_MAXP], lp[FT_MAXP];
-    R wx[FT_MAXP][3], wy[FT_MAXP][3];
-    unsigned short pcell[FT_MAXP];
-};
+template <typename R>
+__global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d, int fast) {
+    using R2 = typename V2<R>::T;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.N) return;
+    const int nc = g.nc;
+    const int c = d.cell[s];
+    const int cx = c / nc, cy = c - cx * nc;
+    const double2 ri = d.rel[s];
+    const bool s2i = d.pid[s] >= g.n1;
+    const unsigned *list = (const unsigned *)d.list;
+    R fsx = 0, fsy = 0, fcx = 0, fcy = 0;
+    if (fast) {
+        // FP32 forces: displacement from the cell offset and the in-cell
+        // offsets (minimum image by construction for nc >= 6)
+        const float B = (float)g.B, cut2 = (float)g.cut2, cap = (float)g.maxpair;
+        const float s22 = (float)g.s22, s12 = (float)g.s12, e22 = (float)g.e22f, e12 = (float)g.e12f;
+        for (int da = -1; da <= 1; ++da)
+            for (int db = -1; db <= 1; ++db) {
+                const int cc = wrapc(cx + da, nc) * nc + wrapc(cy + db, nc);
+                const int qe = d.start[cc + 1];
+                const float ox = -da * B, oy = -db * B;
+                for (int q = d.start[cc]; q < qe; ++q) {
+                    const unsigned ent = list[q];
+                    const int sj = (int)(ent & LIST_SLOT);
+                    if (sj == s) continue;
+                    const bool s2j = (ent >> 31) != 0;
+                    const double2 rj = d.rel[sj];
+                    const float dx = (float)(ri.x - rj.x) + ox, dy = (float)(ri.y - rj.y) + oy;
+                    const float r2 = dx * dx + dy * dy;
+                    float sig = 1.0f, pref = 1.0f;
+                    if (s2i && s2j) {
+                        sig = s22;
+                        pref = e22;
+                    } else if (s2i != s2j) {
+                        sig = s12;
+                        pref = e12;
+                    }
+                    const float r2s = r2 * sig * sig;
+                    if (r2s > cut2) continue;
+                    const float inv = __builtin_amdgcn_rcpf(r2s);
+                    const float i2 = inv * inv, i4 = i2 * i2;
+                    const float a_8 = i4, a_14 = i4 * i2 * inv;
+                    float absF = pref * 24.0f * (2.0f * a_14 - a_8);
+                    if (absF > cap) absF = cap;
+                    const float fx = absF * dx * sig, fy = absF * dy * sig;
+                    if (s2i == s2j) {
+                        fsx += fx;
+                        fsy += fy;
+                    } else {
+                        fcx += fx;
+                        fcy += fy;
+                    }
+                }
+            }
+    } else {
+        // the reference's FP64 arithmetic on absolute positions (am:100-166,
+        // dm:640-698); the S2 side of a 12 pair is the negated S1 side
+        const double xi = cx * g.B + ri.x, yi = cy * g.B + ri.y;
+        for (int da = -1; da <= 1; ++da)
+            for (int db = -1; db <= 1; ++db) {
+                const int ux = wrapc(cx + da, nc), uy = wrapc(cy + db, nc);
+                const int cc = ux * nc + uy;
+                const int qe = d.start[cc + 1];
+                for (int q = d.start[cc]; q < qe; ++q) {
+                    const unsigned ent = list[q];
+                    const int sj = (int)(ent & LIST_SLOT);
+                    if (sj == s) continue;
+                    const bool s2j = (ent >> 31) != 0;
+                    const double2 rj = d.rel[sj];
+                    const double xj = ux * g.B + rj.x, yj = uy * g.B + rj.y;
+                    R fx, fy;
+                    if (!s2i && !s2j) {
+                        if (wca_pair<R>(xi, yi, xj, yj, 0, g, fx, fy)) { fsx += fx; fsy += fy; }
+                    } else if (s2i && s2j) {
+                        if (wca_pair<R>(xi, yi, xj, yj, 1, g, fx, fy)) { fsx += fx; fsy += fy; }
+                    } else if (!s2i) {
+                        if (wca_pair<R>(xi, yi, xj, yj, 2, g, fx, fy)) { fcx += fx; fcy += fy; }
+                    } else {
+                        if (wca_pair<R>(xj, yj, xi, yi, 2, g, fx, fy)) { fcx -= fx; fcy -= fy; }
+                    }
+                }
+            }
+    }
+    R2 o;
+    o.x = fsx + fcx;
+    o.y = fsy + fcy;
+    ((R2 *)d.fwca)[s] = o;
+}
 
-// first mesh index whose centre lies in list cell >= c: owner(k) = floor((k+1/2)h/B)
+// ------------------------------------------------------------------ deposit
+// Charge assignment (gr:31-547) per tile of tc x tc list cells.  A mesh
+// point belongs to the tile whose cells contain its centre,
+// owner(k) = floor((k+1/2)h/B).  Every particle of the tile's cells and of
+// a halo of hx cells adds its (up to 9) weighted charges to the tile's own
+// mesh points in LDS.  The additions are integer (fixed point, scale 2^S),
+// so the sums do not depend on the order of the atomics: the deposit is
+// deterministic without floating-point atomics.  FP64 mode splits every
+// charge into a coarse and a fine integer part (two accumulators), which
+// keeps the sum exact to ~2^-(S+40).
+constexpr int DP_THREADS = 256;
+
 __device__ int first_owned(int c, const Geo &g) {
     if (c <= 0) return 0;
     if (c >= g.nc) return g.Z;
@@ -263,284 +382,180 @@ __device__ int first_owned(int c, const Geo &g) {
     return k;
 }
 
-// ascending-pid walk over the particles of global cell c (fallback path)
-__device__ __forceinline__ int next_in_cell(const int *list, const int *pid, int beg, int end, int last_pid,
-                                            int &slot_out) {
-    int best = 0x7fffffff, bs = -1;
-    for (int q = beg; q < end; ++q) {
-        int s = list[q];
-        int p = pid[s];
-        if (p > last_pid && p < best) {
-            best = p;
-            bs = s;
-        }
-    }
-    slot_out = bs;
-    return best;
-}
+struct DepArgs {
+    int tc;        // list cells per tile side
+    int ncs_cap;   // staged-cell capacity of the flattened path
+    int win_cap;   // owned mesh points per tile (capacity)
+    double scale_hi, scale_lo;      // 2^S1, 2^S2
+    double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
+    int reset;     // re-arm the per-step scalars (block 0)
+};
 
 template <typename R, int SCHEME>
-__device__ void force_tile_direct(const Geo &g, const DevBufs &d, int cx0, int cy0, int nx, int ny, int klo,
-                                  int khi, int llo, int lhi, double e, int what) {
-    using R2 = typename V2<R>::T;
-    const int nc = g.nc, tid = threadIdx.x, nt = blockDim.x;
-    R2 *fw = (R2 *)d.fwca;
-    R *rho = (R *)d.rho;
-    if (what & 1) {
-        for (int ci = tid; ci < nx * ny; ci += nt) {
-            int cx = cx0 + ci / ny, cy = cy0 + ci % ny;
-            int c = cx * nc + cy;
-            for (int q = d.start[c]; q < d.start[c + 1]; ++q) {
-                int si = d.list[q];
-                int pi = d.pid[si];
-                double2 ri = d.rel[si];
-                double xi = cx * g.B + ri.x, yi = cy * g.B + ri.y;
-                bool s2i = pi >= g.n1;
-                R fxs = 0, fys = 0;
-                for (int pass = 0; pass < 2; ++pass)
-                    for (int da = -1; da <= 1; ++da)
-                        for (int db = -1; db <= 1; ++db) {
-                            int ux = wrapc(cx + da, nc), uy = wrapc(cy + db, nc);
-                            int cc = ux * nc + uy;
-                            int beg = d.start[cc], end = d.start[cc + 1];
-                            int last = -1, sj;
-                            while (true) {
-                                int pj = next_in_cell(d.list, d.pid, beg, end, last, sj);
-                                if (sj < 0) break;
-                                last = pj;
-                                if (sj == si) continue;
-                                bool s2j = pj >= g.n1;
-                                if ((pass == 0) != (s2i == s2j)) continue;
-                                double2 rj = d.rel[sj];
-                                double xj = ux * g.B + rj.x, yj = uy * g.B + rj.y;
-                                R fx, fy;
-                                if (!s2i && !s2j) {
-                                    if (wca_pair<R>(xi, yi, xj, yj, 0, g, fx, fy)) { fxs += fx; fys += fy; }
-                                } else if (s2i && s2j) {
-                                    if (wca_pair<R>(xi, yi, xj, yj, 1, g, fx, fy)) { fxs += fx; fys += fy; }
-                                } else if (!s2i) {
-                                    if (wca_pair<R>(xi, yi, xj, yj, 2, g, fx, fy)) { fxs += fx; fys += fy; }
-                                } else {
-                                    if (wca_pair<R>(xj, yj, xi, yi, 2, g, fx, fy)) { fxs -= fx; fys -= fy; }
-                                }
-                            }
-                        }
-                R2 o;
-                o.x = fxs;
-                o.y = fys;
-                fw[si] = o;
+__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, unsigned ent,
+                                                 int c, int klo, int khi, int llo, int lhi,
+                                                 unsigned long long *acc_hi, unsigned long long *acc_lo) {
+    const int s = (int)(ent & LIST_SLOT);
+    const bool s2 = (ent >> 31) != 0;
+    const int gx = c / g.nc, gy = c - gx * g.nc;
+    const double2 rr = d.rel[s];
+    const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
+    int kp, lp;
+    R wx[3], wy[3];
+    weights_r<R, SCHEME>(x, g.h, kp, wx);
+    weights_r<R, SCHEME>(y, g.h, lp, wy);
+    const int Z = g.Z, nl = lhi - llo;
+#pragma unroll
+    for (int ia = 0; ia < 3; ++ia) {
+        if (SCHEME == 0 && ia != 1) continue;
+        if (wx[ia] == 0.0) continue;
+        const int k = wrapc(kp + ia - 1, Z);
+        if (k < klo || k >= khi) continue;
+#pragma unroll
+        for (int ib = 0; ib < 3; ++ib) {
+            if (SCHEME == 0 && ib != 1) continue;
+            if (wy[ib] == 0.0) continue;
+            const int l = wrapc(lp + ib - 1, Z);
+            if (l < llo || l >= lhi) continue;
+            const int w = (k - klo) * nl + (l - llo);
+            if (std::is_same<R, float>::value) {
+                float cf;
+                if (SCHEME == 0)
+                    cf = s2 ? (float)(g.drho * g.f2_f1) : (float)g.drho;
+                else {
+                    cf = (float)wx[ia] * (float)wy[ib] * (float)g.drho;
+                    if (s2) cf = cf * (float)g.f2_f1;
+                }
+                const long long X = __double2ll_rn((double)cf * a.scale_hi);
+                atomicAdd(&acc_hi[w], (unsigned long long)X);
+            } else {
+                double cd;
+                if (SCHEME == 0)
+                    cd = s2 ? g.drho * g.f2_f1 : g.drho;
+                else {
+                    cd = wx[ia] * wy[ib] * g.drho;
+                    if (s2) cd = cd * g.f2_f1;
+                }
+                const long long Xh = __double2ll_rn(cd * a.scale_hi);
+                const double rem = cd - (double)Xh * a.iscale_hi;
+                const long long Xl = __double2ll_rn(rem * a.scale_lo);
+                atomicAdd(&acc_hi[w], (unsigned long long)Xh);
+                atomicAdd(&acc_lo[w], (unsigned long long)Xl);
             }
         }
     }
-    if (what & 2) {
-        const int Z = g.Z, nlx = lhi - llo;
-        for (int pi = tid; pi < (khi - klo) * nlx; pi += nt) {
-            int k = klo + pi / nlx, l = llo + pi % nlx;
-            int culo = (int)ceil(((k - 1) * g.h - e) / g.B) - 1, cuhi = (int)ceil(((k + 2) * g.h + e) / g.B) - 1;
-            int cvlo = (int)ceil(((l - 1) * g.h - e) / g.B) - 1, cvhi = (int)ceil(((l + 2) * g.h + e) / g.B) - 1;
-            if (cuhi - culo + 1 > nc) cuhi = culo + nc - 1;
-            if (cvhi - cvlo + 1 > nc) cvhi = cvlo + nc - 1;
-            R acc = 0;
-            for (int cu = culo; cu <= cuhi; ++cu)
-                for (int cv = cvlo; cv <= cvhi; ++cv) {
-                    int ux = wrapc(cu, nc), uy = wrapc(cv, nc);
-                    int cc = ux * nc + uy;
-                    int beg = d.start[cc], end = d.start[cc + 1];
-                    int last = -1, sj;
-                    while (true) {
-                        int pj = next_in_cell(d.list, d.pid, beg, end, last, sj);
-                        if (sj < 0) break;
-                        last = pj;
-                        double2 rj = d.rel[sj];
-                        double x = ux * g.B + rj.x, y = uy * g.B + rj.y;
-                        int kp, lp;
-                        double wx[3], wy[3];
-                        weights1d<SCHEME>(x, g.h, kp, wx);
-                        weights1d<SCHEME>(y, g.h, lp, wy);
-                        int dk = wrapc(k - kp + Z / 2, Z) - Z / 2, dl = wrapc(l - lp + Z / 2, Z) - Z / 2;
-                        if (dk < -1 || dk > 1 || dl < -1 || dl > 1) continue;
-                        if (SCHEME == 0) {
-                            if (dk != 0 || dl != 0) continue;  // NGP: the particle's own mesh cell only
-                            if (pj < g.n1) acc += (R)g.drho;
-                            else acc += (R)(g.drho * g.f2_f1);
-                        } else {
-                            R c = (R)wx[dk + 1] * (R)wy[dl + 1] * (R)g.drho;
-                            if (pj >= g.n1) c = c * (R)g.f2_f1;
-                            acc += c;
-                        }
-                    }
-                }
-            rho[(size_t)k * Z + l] = acc;
-        }
-    }
 }
 
 template <typename R, int SCHEME>
-__global__ __launch_bounds__(FT_THREADS) void k_force(Geo g, DevBufs d, int what) {
-    using R2 = typename V2<R>::T;
-    __shared__ ForceLDS<R> S;
+__global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArgs a) {
+    extern __shared__ unsigned char smem_raw[];
+    unsigned long long *acc_hi = (unsigned long long *)smem_raw;
+    unsigned long long *acc_lo = acc_hi + a.win_cap;
+    int *cstart = (int *)(acc_lo + (std::is_same<R, double>::value ? a.win_cap : 0));
+    int *cwrap = cstart + a.ncs_cap + 1;
+    __shared__ int info[4];
+    __shared__ int wsum[16];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nc = g.nc, Z = g.Z;
-    const int cx0 = blockIdx.x * FT_TC, cy0 = blockIdx.y * FT_TC;
-    const int nx = min(FT_TC, nc - cx0), ny = min(FT_TC, nc - cy0);
+    const int nc = g.nc, tc = a.tc;
+    const int cx0 = blockIdx.x * tc, cy0 = blockIdx.y * tc;
+    const int nx = min(tc, nc - cx0), ny = min(tc, nc - cy0);
     const unsigned long long step = d.sc->step;
-    // Excursion bound of this step's positions (written by the previous
-    // update); the other parity slot is cleared for the coming update, and
-    // the dt minimum is re-armed.  Both are read only by later kernels.
     const double e = dbits(d.sc->exc_bits[step & 1]) + 1e-9 * g.L;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0 && (what & 4)) {
+    if (a.reset && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+        // per-step scalars for the kernels that follow this one
         d.sc->dt_bits = bitsd(g.dt_max);
         d.sc->exc_bits[(step + 1) & 1] = 0ull;
     }
+    if (tid == 0) {
+        info[0] = first_owned(cx0, g);
+        info[1] = first_owned(cx0 + nx, g);
+        info[2] = first_owned(cy0, g);
+        info[3] = first_owned(cy0 + ny, g);
+    }
+    __syncthreads();
+    const int klo = info[0], khi = info[1], llo = info[2], lhi = info[3];
+    const int nwin = (khi - klo) * (lhi - llo);
+    for (int w = tid; w < nwin; w += nt) {
+        acc_hi[w] = 0ull;
+        if (std::is_same<R, double>::value) acc_lo[w] = 0ull;
+    }
     int hx = (int)floor((1.5 * g.h + e) / g.B) + 1;
     if (hx < 1) hx = 1;
-    if (tid == 0) {
-        S.info[0] = first_owned(cx0, g);
-        S.info[1] = first_owned(cx0 + nx, g);
-        S.info[2] = first_owned(cy0, g);
-        S.info[3] = first_owned(cy0 + ny, g);
-    }
-    const int SX = nx + 2 * hx, SY = ny + 2 * hx;
-    bool direct = hx > FT_HX;
-    if (!direct) {
-        for (int idx = tid; idx < SX * SY; idx += nt) {
-            int a = idx / SY, b = idx - a * SY;
-            int c = wrapc(cx0 - hx + a, nc) * nc + wrapc(cy0 - hx + b, nc);
-            S.cwrap[idx] = c;
-            S.cstart[idx] = d.start[c + 1] - d.start[c];
+    // staged cells: the tile plus hx halo cells, each distinct cell once (a
+    // halo that would wrap onto the tile itself stages the whole axis)
+    const bool allx = nx + 2 * hx >= nc, ally = ny + 2 * hx >= nc;
+    const int SX = allx ? nc : nx + 2 * hx, SY = ally ? nc : ny + 2 * hx, ncs = SX * SY;
+    const int sx0 = allx ? 0 : cx0 - hx, sy0 = ally ? 0 : cy0 - hx;
+    const unsigned *list = (const unsigned *)d.list;
+    if (ncs <= a.ncs_cap) {
+        for (int idx = tid; idx < ncs; idx += nt) {
+            const int ia = idx / SY, ib = idx - ia * SY;
+            const int c = wrapc(sx0 + ia, nc) * nc + wrapc(sy0 + ib, nc);
+            cwrap[idx] = c;
+            cstart[idx] = d.start[c + 1] - d.start[c];
         }
         __syncthreads();
-        int total = block_exscan4(S.cstart, SX * SY, S.wsum);
-        if (tid == 0) S.cstart[SX * SY] = total;
-        direct = total > FT_MAXP;
-    }
-    __syncthreads();
-    const int klo = S.info[0], khi = S.info[1], llo = S.info[2], lhi = S.info[3];
-    if (direct) {
-        if (tid == 0) atomicAdd(&d.sc->fallback, 1);
-        force_tile_direct<R, SCHEME>(g, d, cx0, cy0, nx, ny, klo, khi, llo, lhi, e, what);
-        return;
-    }
-    // stage: one thread per staged cell, particles sorted by pid
-    for (int idx = tid; idx < SX * SY; idx += nt) {
-        const int c = S.cwrap[idx];
-        const int gbeg = d.start[c], n = d.start[c + 1] - gbeg;
-        const int base = S.cstart[idx];
-        for (int q = 0; q < n; ++q) {
-            int s = d.list[gbeg + q];
-            int p = d.pid[s];
-            int r = q - 1;
-            while (r >= 0 && S.pid[base + r] > p) {
-                S.pid[base + r + 1] = S.pid[base + r];
-                S.slot[base + r + 1] = S.slot[base + r];
-                --r;
-            }
-            S.pid[base + r + 1] = p;
-            S.slot[base + r + 1] = s;
-        }
-        const int gx = c / nc, gy = c - gx * nc;
-        for (int q = 0; q < n; ++q) {
-            const int t = base + q;
-            double2 rr = d.rel[S.slot[t]];
-            double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
-            S.x[t] = x;
-            S.y[t] = y;
-            S.pcell[t] = (unsigned short)idx;
-            if (what & 2) {
-                int kp, lp;
-                double wx[3], wy[3];
-                weights1d<SCHEME>(x, g.h, kp, wx);
-                weights1d<SCHEME>(y, g.h, lp, wy);
-                S.kp[t] = wrapc(kp, Z);
-                S.lp[t] = wrapc(lp, Z);
-#pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    S.wx[t][u] = (R)wx[u];
-                    S.wy[t][u] = (R)wy[u];
+        int total = 0;
+        if (ncs <= 4 * nt) {
+            total = block_exscan4(cstart, ncs, wsum);
+        } else {
+            // serial prefix in chunks by one wave (large staged regions)
+            if (tid < 64) {
+                int run = 0;
+                for (int base = 0; base < ncs; base += 64) {
+                    int i = base + tid;
+                    int v = i < ncs ? cstart[i] : 0, x = v;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        int y = __shfl_up(x, o, 64);
+                        if (tid >= o) x += y;
+                    }
+                    if (i < ncs) cstart[i] = run + x - v;
+                    run += __shfl(x, 63, 64);
                 }
+                if (tid == 0) wsum[0] = run;
             }
+            __syncthreads();
+            total = wsum[0];
         }
-    }
-    __syncthreads();
-    const int total = S.cstart[SX * SY];
-    if (what & 1) {
-        R2 *fw = (R2 *)d.fwca;
+        if (tid == 0) cstart[ncs] = total;
+        __syncthreads();
         for (int t = tid; t < total; t += nt) {
-            const int idx = S.pcell[t];
-            const int a = idx / SY, b = idx - a * SY;
-            if (a < hx || a >= hx + nx || b < hx || b >= hx + ny) continue;
-            const double xi = S.x[t], yi = S.y[t];
-            const bool s2i = S.pid[t] >= g.n1;
-            R fxs = 0, fys = 0;
-            for (int pass = 0; pass < 2; ++pass)
-                for (int da = -1; da <= 1; ++da)
-                    for (int db = -1; db <= 1; ++db) {
-                        const int cj = (a + da) * SY + (b + db);
-                        const int ue = S.cstart[cj + 1];
-                        for (int u = S.cstart[cj]; u < ue; ++u) {
-                            if (u == t) continue;
-                            const bool s2j = S.pid[u] >= g.n1;
-                            if ((pass == 0) != (s2i == s2j)) continue;
-                            R fx, fy;
-                            if (!s2i && !s2j) {
-                                if (wca_pair<R>(xi, yi, S.x[u], S.y[u], 0, g, fx, fy)) { fxs += fx; fys += fy; }
-                            } else if (s2i && s2j) {
-                                if (wca_pair<R>(xi, yi, S.x[u], S.y[u], 1, g, fx, fy)) { fxs += fx; fys += fy; }
-                            } else if (!s2i) {
-                                if (wca_pair<R>(xi, yi, S.x[u], S.y[u], 2, g, fx, fy)) { fxs += fx; fys += fy; }
-                            } else {
-                                if (wca_pair<R>(S.x[u], S.y[u], xi, yi, 2, g, fx, fy)) { fxs -= fx; fys -= fy; }
-                            }
-                        }
-                    }
-            R2 o;
-            o.x = fxs;
-            o.y = fys;
-            fw[S.slot[t]] = o;
+            int lo = 0, hi = ncs - 1;  // last staged cell with cstart <= t
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (cstart[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int c = cwrap[lo];
+            const unsigned ent = list[d.start[c] + (t - cstart[lo])];
+            deposit_particle<R, SCHEME>(g, d, a, ent, c, klo, khi, llo, lhi, acc_hi, acc_lo);
+        }
+    } else {
+        __syncthreads();  // accumulators cleared before the first atomic
+        for (int idx = tid; idx < ncs; idx += nt) {
+            const int ia = idx / SY, ib = idx - ia * SY;
+            const int c = wrapc(sx0 + ia, nc) * nc + wrapc(sy0 + ib, nc);
+            for (int q = d.start[c]; q < d.start[c + 1]; ++q)
+                deposit_particle<R, SCHEME>(g, d, a, list[q], c, klo, khi, llo, lhi, acc_hi, acc_lo);
         }
     }
-    if (what & 2) {
-        R *rho = (R *)d.rho;
-        const int nlx = lhi - llo;
-        const int base_u = cx0 - hx, base_v = cy0 - hx;
-        const R drho = (R)g.drho, q2 = (R)g.f2_f1;
-        const R drho2 = (R)(g.drho * g.f2_f1);
-        for (int pi = tid; pi < (khi - klo) * nlx; pi += nt) {
-            const int k = klo + pi / nlx, l = llo + pi - (pi / nlx) * nlx;
-            int culo = (int)ceil(((k - 1) * g.h - e) / g.B) - 1, cuhi = (int)ceil(((k + 2) * g.h + e) / g.B) - 1;
-            int cvlo = (int)ceil(((l - 1) * g.h - e) / g.B) - 1, cvhi = (int)ceil(((l + 2) * g.h + e) / g.B) - 1;
-            if (cuhi - culo + 1 > nc) cuhi = culo + nc - 1;
-            if (cvhi - cvlo + 1 > nc) cvhi = cvlo + nc - 1;
-            int alo = max(culo - base_u, 0), ahi = min(cuhi - base_u, SX - 1);
-            int blo = max(cvlo - base_v, 0), bhi = min(cvhi - base_v, SY - 1);
-            R acc = 0;
-            for (int a = alo; a <= ahi; ++a)
-                for (int b = blo; b <= bhi; ++b) {
-                    const int cj = a * SY + b;
-                    const int ue = S.cstart[cj + 1];
-                    for (int u = S.cstart[cj]; u < ue; ++u) {
-                        int dk = k - S.kp[u], dl = l - S.lp[u];
-                        if (dk > Z / 2) dk -= Z;
-                        if (dk < -Z / 2) dk += Z;
-                        if (dl > Z / 2) dl -= Z;
-                        if (dl < -Z / 2) dl += Z;
-                        if (dk < -1 || dk > 1 || dl < -1 || dl > 1) continue;
-                        const bool s2 = S.pid[u] >= g.n1;
-                        if (SCHEME == 0) {
-                            if (dk != 0 || dl != 0) continue;  // NGP: the particle's own mesh cell only
-                            acc += s2 ? drho2 : drho;
-                        } else {
-                            R c = S.wx[u][dk + 1] * S.wy[u][dl + 1] * drho;
-                            if (s2) c = c * q2;
-                            acc += c;
-                        }
-                    }
-                }
-            rho[(size_t)k * Z + l] = acc;
-        }
+    __syncthreads();
+    R *rho = (R *)d.rho;
+    const int nl = lhi - llo;
+    for (int w = tid; w < nwin; w += nt) {
+        const int k = klo + w / nl, l = llo + w - (w / nl) * nl;
+        double v = (double)(long long)acc_hi[w] * a.iscale_hi;
+        if (std::is_same<R, double>::value) v += (double)(long long)acc_lo[w] * a.iscale_lo;
+        rho[(size_t)k * g.Z + l] = (R)v;
     }
 }
 
+// per-step scalar re-arm when no deposit runs (ideal-gas runs)
+__global__ void k_rearm(Geo g, DevScalars *sc) {
+    const unsigned long long step = sc->step;
+    sc->dt_bits = bitsd(g.dt_max);
+    sc->exc_bits[(step + 1) & 1] = 0ull;
+}
 // ------------------------------------------------------------------ FFT in LDS
 // One Stockham pass of radix RAD over nrows rows of length n = 2^logn held
 // in LDS (row stride `stride`), in place: every thread loads its butterflies'
@@ -604,7 +619,7 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
 }
 
 template <typename R, int EPT>
-__device__ void lds_fft(typename V2<R>::T *buf, int logn, int nrows, int stride,
+__device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int logn, int nrows, int stride,
                         const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     int logNs = 0;
     while (logn - logNs >= 2) {
@@ -784,9 +799,9 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
     using R2 = typename V2<R>::T;
     const int Z = g.Z;
     int k, l;
-    double wxd[3], wyd[3];
-    weights1d<SCHEME>(x, g.h, k, wxd);
-    weights1d<SCHEME>(y, g.h, l, wyd);
+    R wx[3], wy[3];
+    weights_r<R, SCHEME>(x, g.h, k, wx);
+    weights_r<R, SCHEME>(y, g.h, l, wy);
     k = wrapc(k, Z);
     l = wrapc(l, Z);
     if (SCHEME == 0) {
@@ -804,12 +819,6 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
     }
     const int kx[3] = {(k - 1 + Z) & (Z - 1), k, (k + 1) & (Z - 1)};
     const int ly[3] = {(l - 1 + Z) & (Z - 1), l, (l + 1) & (Z - 1)};
-    R wx[3], wy[3];
-#pragma unroll
-    for (int u = 0; u < 3; ++u) {
-        wx[u] = (R)wxd[u];
-        wy[u] = (R)wyd[u];
-    }
     R ax = 0, ay = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1098,7 +1107,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
 __global__ void k_fill(Geo g, DevBufs d) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
-    d.list[d.start[d.cell[s]] + d.rank[s]] = s;
+    const unsigned sp2 = d.pid[s] >= g.n1 ? 1u : 0u;
+    ((unsigned *)d.list)[d.start[d.cell[s]] + d.rank[s]] = (unsigned)s | (sp2 << 31);
+}
+
+// each cell's list entries in ascending slot order (fixed summation order)
+__global__ void k_sort_cells(Geo g, DevBufs d) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.ncell) return;
+    const int b = d.start[c], n = d.start[c + 1] - b;
+    if (n < 2) return;
+    unsigned *l = (unsigned *)d.list + b;
+    for (int i = 1; i < n; ++i) {
+        const unsigned v = l[i];
+        int j = i - 1;
+        while (j >= 0 && (l[j] & LIST_SLOT) > (v & LIST_SLOT)) {
+            l[j + 1] = l[j];
+            --j;
+        }
+        l[j + 1] = v;
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1137,21 +1165,85 @@ hipError_t set_kernel_attributes() {
     BDK_ATTR(k_c2r_rows<float>)
     BDK_ATTR(k_c2r_rows<double>)
 #undef BDK_ATTR
+    // the deposit also has a little static LDS: ask for 128 KB of dynamic LDS
+#define BDK_ATTR_D(K)                                                                          \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+    BDK_ATTR_D((k_deposit<float, 0>))
+    BDK_ATTR_D((k_deposit<float, 1>))
+    BDK_ATTR_D((k_deposit<float, 2>))
+    BDK_ATTR_D((k_deposit<double, 0>))
+    BDK_ATTR_D((k_deposit<double, 1>))
+    BDK_ATTR_D((k_deposit<double, 2>))
+#undef BDK_ATTR_D
     return e;
 }
 
-template <typename R, int SCHEME> static void launch_force_t(bdk_t *h, int what) {
-    dim3 grid(nblk(h->g.nc, FT_TC), nblk(h->g.nc, FT_TC));
-    hipLaunchKernelGGL((k_force<R, SCHEME>), grid, dim3(FT_THREADS), 0, h->stream, h->g, h->d, what);
+hipError_t launch_wca(bdk_t *h) {
+    // FP32 forces use the cell-relative displacement when the box holds at
+    // least 6 cells per side (the neighbour offset is then the minimum image)
+    const int fast = (h->p.precision == BDK_FP32 && h->g.nc >= 6) ? 1 : 0;
+    if (h->p.precision == BDK_FP32)
+        hipLaunchKernelGGL(k_wca<float>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, fast);
+    else
+        hipLaunchKernelGGL(k_wca<double>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
+    return hipGetLastError();
 }
 
-hipError_t launch_force(bdk_t *h, int what) {
+// Deposit tiling: tile side tc (list cells) such that a tile owns at most
+// about 38 x 38 mesh points; fixed-point scales chosen so that no sum of
+// N charges can overflow 63 bits.
+struct DepPlan {
+    DepArgs a;
+    dim3 grid;
+    size_t lds;
+};
+
+static DepPlan dep_plan(const bdk_t *h, int reset) {
+    const Geo &g = h->g;
+    const bool f64 = h->p.precision == BDK_FP64;
+    DepPlan pl;
+    int tc = (int)std::floor(38.0 * g.h / g.B);
+    if (tc < 1) tc = 1;
+    if (tc > g.nc) tc = g.nc;
+    const int side = (int)std::floor(tc * g.B / g.h) + 2;
+    pl.a.tc = tc;
+    pl.a.win_cap = side * side;
+    pl.a.ncs_cap = (tc + 6) * (tc + 6);
+    const double cmax = g.drho * std::max(1.0, std::fabs(g.f2_f1));
+    const double nlog = std::log2((double)g.N * cmax + 1.0);
+    int s1 = (int)std::floor(62.0 - nlog);
+    if (!f64) s1 = std::min(s1, 40);
+    const int gap = std::min(40, 61 - (int)std::ceil(std::log2((double)g.N + 1.0)));
+    const int s2 = s1 + gap;
+    pl.a.scale_hi = std::ldexp(1.0, s1);
+    pl.a.iscale_hi = std::ldexp(1.0, -s1);
+    pl.a.scale_lo = std::ldexp(1.0, s2);
+    pl.a.iscale_lo = std::ldexp(1.0, -s2);
+    pl.a.reset = reset;
+    pl.grid = dim3(nblk(g.nc, tc), nblk(g.nc, tc));
+    pl.lds = (size_t)pl.a.win_cap * (f64 ? 16 : 8) + (size_t)(2 * pl.a.ncs_cap + 1) * 4;
+    return pl;
+}
+
+size_t deposit_lds_bytes(const bdk_t *h) { return dep_plan(h, 0).lds; }
+
+template <typename R, int SCHEME> static void launch_deposit_t(bdk_t *h, const DepPlan &pl) {
+    hipLaunchKernelGGL((k_deposit<R, SCHEME>), pl.grid, dim3(DP_THREADS), pl.lds, h->stream, h->g, h->d, pl.a);
+}
+
+hipError_t launch_deposit(bdk_t *h, int reset) {
+    const DepPlan pl = dep_plan(h, reset);
     const bool f32 = h->p.precision == BDK_FP32;
     switch (h->g.scheme) {
-    case 0: f32 ? launch_force_t<float, 0>(h, what) : launch_force_t<double, 0>(h, what); break;
-    case 1: f32 ? launch_force_t<float, 1>(h, what) : launch_force_t<double, 1>(h, what); break;
-    default: f32 ? launch_force_t<float, 2>(h, what) : launch_force_t<double, 2>(h, what); break;
+    case 0: f32 ? launch_deposit_t<float, 0>(h, pl) : launch_deposit_t<double, 0>(h, pl); break;
+    case 1: f32 ? launch_deposit_t<float, 1>(h, pl) : launch_deposit_t<double, 1>(h, pl); break;
+    default: f32 ? launch_deposit_t<float, 2>(h, pl) : launch_deposit_t<double, 2>(h, pl); break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_rearm(bdk_t *h) {
+    hipLaunchKernelGGL(k_rearm, dim3(1), dim3(1), 0, h->stream, h->g, h->d.sc);
     return hipGetLastError();
 }
 
@@ -1232,6 +1324,7 @@ static void launch_scan_fill_inner(bdk_t *h, int inc_step) {
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bsum, h->d.start,
                        ncell);
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
 }
 
 hipError_t launch_scan_fill(bdk_t *h) {
@@ -1261,61 +1354,43 @@ hipError_t launch_noise_out(bdk_t *h) {
 // One step.  With ev != nullptr an event is recorded after every launch
 // (profiling); names receives the labels.
 hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev) {
+    hipError_t e = hipSuccess;
     auto mark = [&](const char *nm) {
         if (names) names->push_back(nm);
         if (ev) {
-            hipEvent_t e;
-            hipEventCreate(&e);
-            hipEventRecord(e, h->stream);
-            ev->push_back(e);
+            hipEvent_t x;
+            if (hipEventCreate(&x) == hipSuccess) {
+                if (hipEventRecord(x, h->stream) != hipSuccess && e == hipSuccess) e = hipErrorUnknown;
+                ev->push_back(x);
+            }
         }
+    };
+    auto chk = [&](hipError_t r) {
+        if (e == hipSuccess && r != hipSuccess) e = r;
     };
     if (ev) mark("begin");
     const bool kap = !h->g.idgas;
-    const bool adaptive = h->g.adaptive;
-    // what: 1 WCA, 2 deposit, 4 reset of the dt min
-    int what = (h->g.idgas ? 0 : 1 | 2) | 4;
-    hipError_t e = launch_force(h, what);
-    mark("k_force");
     if (kap) {
-        if (h->p.precision == BDK_FP32) {
-            using C = float2;
-            using C4 = float4;
-            const int Z = h->g.Z, logZ = h->g.logZ;
-            FftCfg<float> c = fft_cfg<float>(Z);
-            hipLaunchKernelGGL((k_r2c_rows<float>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream,
-                               (const float *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1);
-            mark("k_r2c_rows");
-            hipLaunchKernelGGL((k_col<float>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream,
-                               (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, (const float *)h->d.sqf,
-                               (const float *)h->d.lapf, h->g, c.RC, c.sC);
-            mark("k_col");
-            hipLaunchKernelGGL((k_c2r_rows<float>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream,
-                               (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2);
-            mark("k_c2r_rows");
-        } else {
-            using C = double2;
-            using C4 = double4;
-            const int Z = h->g.Z, logZ = h->g.logZ;
-            FftCfg<double> c = fft_cfg<double>(Z);
-            hipLaunchKernelGGL((k_r2c_rows<double>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream,
-                               (const double *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1);
-            mark("k_r2c_rows");
-            hipLaunchKernelGGL((k_col<double>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream,
-                               (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, (const double *)h->d.sq,
-                               (const double *)h->d.lap, h->g, c.RC, c.sC);
-            mark("k_col");
-            hipLaunchKernelGGL((k_c2r_rows<double>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream,
-                               (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2);
-            mark("k_c2r_rows");
-        }
-    }
-    if (adaptive) {
-        launch_gather(h, 1);
-        mark("k_gather");
-        launch_update(h, 0);
+        chk(launch_wca(h));
+        mark("k_wca");
+        chk(launch_deposit(h, 1));
+        mark("k_deposit");
+        chk(launch_fft_pass(h, 0));
+        mark("k_r2c_rows");
+        chk(launch_fft_pass(h, 1));
+        mark("k_col");
+        chk(launch_fft_pass(h, 2));
+        mark("k_c2r_rows");
     } else {
-        launch_update(h, kap ? 1 : 0);
+        chk(launch_rearm(h));
+        mark("k_rearm");
+    }
+    if (h->g.adaptive) {
+        chk(launch_gather(h, 1));
+        mark("k_gather");
+        chk(launch_update(h, 0));
+    } else {
+        chk(launch_update(h, kap ? 1 : 0));
     }
     mark("k_update");
     const int ncell = h->g.ncell;
@@ -1329,7 +1404,9 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     mark("k_scan_down");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
     mark("k_fill");
-    if (e == hipSuccess) e = hipGetLastError();
+    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    mark("k_sort_cells");
+    chk(hipGetLastError());
     return e;
 }
 

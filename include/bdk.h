@@ -158,14 +158,15 @@ int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, 
    idempotent (they read the state and overwrite their outputs), except
    BDK_K_STEP, which advances the simulation by `reps` full steps. */
 enum bdk_kernel_id {
-    BDK_K_FORCE = 0,      /* WCA + deposit */
-    BDK_K_FORCE_WCA = 1,  /* WCA part only */
-    BDK_K_FORCE_DEP = 2,  /* deposit part only */
-    BDK_K_R2C = 3,        /* row r2c FFT pass */
-    BDK_K_COL = 4,        /* column FFT + Green's function + inverse column FFTs */
-    BDK_K_C2R = 5,        /* row c2r FFT pass */
-    BDK_K_GATHER = 6,     /* capillary gather (no update) */
-    BDK_K_STEP = 7        /* one full step (graph replay when dt is fixed) */
+    BDK_K_WCA = 0,        /* WCA forces */
+    BDK_K_DEPOSIT = 1,    /* charge deposit */
+    BDK_K_R2C = 2,        /* row r2c FFT pass */
+    BDK_K_COL = 3,        /* column FFT + Green's function + inverse column FFTs */
+    BDK_K_C2R = 4,        /* row c2r FFT pass */
+    BDK_K_GATHER = 5,     /* capillary gather (no update) */
+    BDK_K_STEP = 6,       /* one full step (graph replay when dt is fixed) */
+    BDK_K_UPDATE = 7      /* gather + noise + Euler step + refresh + counting; the state it
+                             changes is saved before and restored after the timing */
 };
 int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch);
 

@@ -80,6 +80,15 @@ def test_wca_parity(prec, tol, ratios):
     assert rel_rms(fg, fo) < tol, rel_rms(fg, fo)
 
 
+@pytest.mark.parametrize("prec,tol", [(bdk.FP64, TOL64), (bdk.FP32, TOL32)])
+@pytest.mark.parametrize("L", [5.0, 8.0, 12.0])
+def test_wca_parity_small_boxes(prec, tol, L):
+    """3 to 10 list cells per side: wrapped neighbourhoods and images."""
+    n = int(0.3 * L * L)
+    g, o, git, rel = pair(n1=n, n2=n, L=L, Z=8, seed=7, dmin=0.85, precision=prec)
+    assert rel_rms(g.berechne_WCA(), o.wca()) < tol
+
+
 def test_wca_parity_p7_initial_state():
     """Start positions of init_zufallspos with rel up to 2B (dm:1655)."""
     bp, op = params(n1=1000, n2=1000, L=64.0, Z=128, noise_mode=bdk.NOISE_UPLOAD)
@@ -97,7 +106,7 @@ def test_wca_parity_p7_initial_state():
 
 @pytest.mark.parametrize("scheme", [bdk.NGP, bdk.CIC, bdk.TSC])
 @pytest.mark.parametrize("prec,tol", [(bdk.FP64, TOL64), (bdk.FP32, TOL32)])
-@pytest.mark.parametrize("Z,L", [(64, 40.0), (128, 64.0), (256, 90.0)])
+@pytest.mark.parametrize("Z,L", [(8, 8.0), (16, 12.0), (64, 40.0), (128, 64.0), (256, 90.0)])
 def test_capillary_parity(scheme, prec, tol, Z, L):
     say("case: kap scheme=%d prec=%d Z=%d" % (scheme, prec, Z))
     n = int(0.25 * L * L)
