@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -107,7 +108,7 @@ Footprint footprint_of(const bd_params &p) {
     const size_t ncell = (size_t)d.nc * d.nc, Z = (size_t)p.Z;
     Footprint f;
     // cell, rel, pid, fwca, fkap, noise_up, noise_out, rank, list
-    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4);
+    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4);
     f.cells = ncell * 8 + 4 + 4096 * 4;
     // rho, ht, gt, fmesh, tables
     f.mesh = Z * Z * R + (Z / 2) * Z * 2 * R + (Z / 2) * Z * 4 * R + Z * Z * 2 * R + Z * (2 * R + 3 * 8 + 3 * 4);
@@ -117,7 +118,7 @@ Footprint footprint_of(const bd_params &p) {
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.start, h->d.list, h->d.bsum, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.bsum, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -242,14 +243,14 @@ int ensure(bdk_t *h) {
     return upload(h);
 }
 
-// slot-ordered device array of real2 -> per-species [n][2] doubles
+// list-ordered device array of real2 -> per-species [n][2] doubles
 template <typename R>
 int download_r2(bdk_t *h, const void *src, double *f1, double *f2) {
     const int N = h->g.N, n1 = h->p.n1;
     std::vector<R> buf(2 * (size_t)N);
     std::vector<int> pid(N);
     HIPCHK(h, hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(R), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.lpid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     for (int s = 0; s < N; ++s) {
         int p = pid[s];
@@ -265,24 +266,40 @@ int download_forces(bdk_t *h, const void *src, double *f1, double *f2) {
     return h->p.precision == BDK_FP32 ? download_r2<float>(h, src, f1, f2) : download_r2<double>(h, src, f1, f2);
 }
 
+// steps per replay of the multi-step graph
+constexpr int kGraphSteps = 8;
+
 void drop_graph(bdk_t *h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->gexec_multi) (void)hipGraphExecDestroy(h->gexec_multi);
+    if (h->graph_multi) (void)hipGraphDestroy(h->graph_multi);
     h->gexec = nullptr;
     h->graph = nullptr;
+    h->gexec_multi = nullptr;
+    h->graph_multi = nullptr;
 }
 
-int ensure_graph(bdk_t *h) {
-    if (h->gexec) return BDK_OK;
+int capture_steps(bdk_t *h, int nsteps, hipGraph_t *graph, hipGraphExec_t *exec) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = bdkx::enqueue_step(h, nullptr, nullptr);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = bdkx::enqueue_step(h, nullptr, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess) return hipfail(h, e, "enqueue_step (capture)");
     if (e2 != hipSuccess) return hipfail(h, e2, "hipStreamEndCapture");
-    h->graph = g;
-    HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    *graph = g;
+    HIPCHK(h, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return BDK_OK;
+}
+
+// fixed-dt steps are replayed from hipGraphs: one of a single step and one
+// of kGraphSteps steps (fewer launches per simulated time)
+int ensure_graph(bdk_t *h) {
+    if (h->gexec && h->gexec_multi) return BDK_OK;
+    int rc = capture_steps(h, 1, &h->graph, &h->gexec);
+    if (rc) return rc;
+    return capture_steps(h, kGraphSteps, &h->graph_multi, &h->gexec_multi);
 }
 
 double dt_from_bits(unsigned long long b) {
@@ -401,6 +418,13 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     g.lam2inv = 1.0 / (p->lambda * p->lambda);
     g.kap = p->kap;
     g.zero_mode_g = p->lambda > 0.2 * p->L;
+    g.ncinv = 1.0f / (float)g.nc;
+    g.hinv = 1.0 / g.h;
+    {
+        // BDK_XCD_REMAP=0 keeps the hardware workgroup order (for comparison)
+        const char *e = getenv("BDK_XCD_REMAP");
+        g.xcd = (e && e[0] == '0') ? 0 : 1;
+    }
     g.seed = p->seed;
 
     const size_t N = g.N, R = real_size(*p), ncell = g.ncell, Z = g.Z;
@@ -423,6 +447,9 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.count, ncell));
     ALLOC(dalloc(d.start, ncell + 1));
     ALLOC(dalloc(d.list, N));
+    ALLOC(dalloc(d.lcell, N));
+    ALLOC(dalloc(d.lrel, N));
+    ALLOC(dalloc(d.lpid, N));
     ALLOC(dalloc(d.bsum, 4096));
     ALLOC(dalloc_v(d.rho, Z * Z * R));
     ALLOC(dalloc_v(d.ht, (Z / 2) * Z * 2 * R));
@@ -638,10 +665,10 @@ int bdk_run_steps(bdk_t *h, int64_t k, double *t_out) {
     if (!h->g.adaptive && h->p.noise_mode != BDK_NOISE_UPLOAD) {
         rc = ensure_graph(h);
         if (rc) return rc;
-        for (int64_t i = 0; i < k; ++i) {
-            HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
-            t += (long double)h->p.dt_max;
-        }
+        int64_t i = 0;
+        for (; i + kGraphSteps <= k; i += kGraphSteps) HIPCHK(h, hipGraphLaunch(h->gexec_multi, h->stream));
+        for (; i < k; ++i) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+        for (int64_t j = 0; j < k; ++j) t += (long double)h->p.dt_max;
         h->host_step += k;
     } else {
         for (int64_t i = 0; i < k; ++i) {
@@ -805,7 +832,8 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
     std::vector<Snap> snaps;
     if (kernel == BDK_K_UPDATE) {
         const size_t N = (size_t)h->g.N, nc2 = (size_t)h->g.ncell;
-        Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.rank, N * 4},
+        Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.pid, N * 4},
+                       {nullptr, h->d.rank, N * 4},
                        {nullptr, h->d.count, nc2 * 4}, {nullptr, h->d.sc, sizeof(DevScalars)}};
         for (Snap sn : list) {
             HIPCHK(h, hipMalloc(&sn.p, sn.n));

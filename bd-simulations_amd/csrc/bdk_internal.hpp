@@ -35,6 +35,9 @@ struct Geo {
     double lam2inv;  // 1/(lambda*lambda)
     double kap;
     int zero_mode_g; // G(0,0) = 0 when lambda > 0.2 L (dm:1389)
+    int xcd;         // renumber workgroups per XCD (launch order only)
+    float ncinv;     // 1/nc (fast exact division of cell indices)
+    double hinv;     // 1/h
     unsigned long long seed;
 };
 
@@ -51,18 +54,22 @@ struct DevScalars {
 
 struct DevBufs {
     // particles, storage order ("slot"); species from pid (< n1: species 1)
-    int *cell = nullptr;      // packed cell gx*nc+gy
+    int *cell = nullptr;      // packed cell gx*nc+gy (k_update writes slot q = list position q)
     double2 *rel = nullptr;   // offset inside the cell
     int *pid = nullptr;       // slot -> particle id
-    void *fwca = nullptr;     // [N] real2
-    void *fkap = nullptr;     // [N] real2
+    void *fwca = nullptr;     // [N] real2, list order
+    void *fkap = nullptr;     // [N] real2, list order
     double2 *noise_up = nullptr; // [N] by particle id (upload mode)
     double2 *noise_out = nullptr; // [N] by particle id (bdk_get_noise)
     int *rank = nullptr;      // [N] rank of the slot inside its cell
     // cell list
     int *count = nullptr;     // [ncell]
     int *start = nullptr;     // [ncell+1]
-    int *list = nullptr;      // [N] slots grouped by cell
+    int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
+    // the state in list order (k_list_state): cell, rel and particle id of list entry q
+    int *lcell = nullptr;
+    double2 *lrel = nullptr;
+    int *lpid = nullptr;
     int *bsum = nullptr;      // scan block sums
     // mesh
     void *rho = nullptr;      // Z*Z real
@@ -91,8 +98,10 @@ struct bdk {
     std::vector<int32_t> hgit[2];
     std::vector<double> hrel[2];
     int scan_blocks = 0;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;           // one step
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph_multi = nullptr;     // kGraphSteps steps
+    hipGraphExec_t gexec_multi = nullptr;
     std::string err;
     int64_t host_step = 0;
 };

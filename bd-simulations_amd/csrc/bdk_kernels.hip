@@ -12,7 +12,13 @@
 //               (dm:116-157), refresh (dm:216-325), cell counting;
 //   k_scan_*    exclusive scan of the cell counts;
 //   k_fill      cell list = slots grouped by cell, species in bit 31;
-//   k_sort_cells  each cell's entries in slot order.
+//   k_sort_cells  each cell's entries in slot order;
+//   k_list_state  the particle state gathered into list order (lcell,
+//               lrel, lpid).
+// Every force, gather and update kernel runs over list positions q and
+// reads the list-ordered state; k_update writes the new state to slot q.
+// The slots are therefore re-sorted by cell every step: consecutive threads
+// handle particles of neighbouring cells for the whole run.
 // With a finite max_travel, k_gather (forces + dt min-reduction, dm:1420-1491)
 // runs before a non-fused k_update.
 //
@@ -26,6 +32,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <type_traits>
 
 #include "bdk_internal.hpp"
@@ -89,16 +97,39 @@ __device__ __forceinline__ double2 philox_noise(unsigned long long seed, uint32_
     double u2 = (double)(((((unsigned long long)c[3]) << 32) | c[2]) >> 11) * inv53;
     double r = gauss ? sqrt(-2.0 * var * log(1.0 - u1)) : sqrt(5.0 * u1);
     double a = 2.0 * M_PI * u2;
+    double sa, ca;
+    sincos(a, &sa, &ca);
     double2 o;
-    o.x = r * cos(a);
-    o.y = r * sin(a);
+    o.x = r * ca;
+    o.y = r * sa;
     return o;
+}
+
+// ------------------------------------------------------------------ launch order
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumbered so that every XCD runs one contiguous range of the logical
+// grid, neighbouring tiles (whose inputs overlap) share an L2.
+__device__ __forceinline__ int xcd_block(int on) {
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    if (!on) return b;
+    const int nb = gridDim.x * gridDim.y;
+    const int x = b & 7, i = b >> 3, base = nb >> 3, rem = nb & 7;
+    return x < rem ? x * (base + 1) + i : rem * (base + 1) + (x - rem) * base + i;
 }
 
 // ------------------------------------------------------------------ geometry
 __device__ __forceinline__ int wrapc(int u, int n) {
     int v = u % n;
     return v < 0 ? v + n : v;
+}
+// u in [-n, 2n)
+__device__ __forceinline__ int wrap1(int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); }
+// c / nc for 0 <= c < 2^24 (float reciprocal, corrected to the exact quotient)
+__device__ __forceinline__ int div_nc(int c, const Geo &g) {
+    int q = (int)((float)c * g.ncinv);
+    const int r = c - q * g.nc;
+    q += (r >= g.nc) - (r < 0);
+    return q;
 }
 
 // mesh index and weights along one axis (NGP gr:44-54, CIC gr:154-190,
@@ -132,28 +163,32 @@ __device__ __forceinline__ void weights1d(double x, double h, int &k, double w[3
 }
 
 // Weights in the precision of the forces: FP64 follows the reference's
-// operations exactly; FP32 keeps the mesh index exact (x/h in FP64) and
-// evaluates the offset t = d/h and the textbook weights (CIC: 1-|t|, -t, t;
+// operations exactly; FP32 takes the index from x/h in FP64 and evaluates
+// the offset t = x/h - k - 1/2 and the textbook weights (CIC: 1-|t|, -t, t;
 // TSC: 3/4-t^2, (1/2-t)^2/2, (1/2+t)^2/2) in FP32.
 template <typename R, int SCHEME>
-__device__ __forceinline__ void weights_r(double x, double h, int &k, R w[3]) {
+__device__ __forceinline__ void weights_r(double x, const Geo &g, int &k, R w[3]) {
     if (std::is_same<R, double>::value) {
         double wd[3];
-        weights1d<SCHEME>(x, h, k, wd);
+        weights1d<SCHEME>(x, g.h, k, wd);
         w[0] = (R)wd[0];
         w[1] = (R)wd[1];
         w[2] = (R)wd[2];
         return;
     }
-    const int kk = (int)(x / h);
-    k = kk;
     if (SCHEME == 0) {
+        k = (int)(x / g.h);
         w[0] = 0;
         w[1] = 1;
         w[2] = 0;
         return;
     }
-    const float t = (float)(x / h - (kk + 0.5));
+    // CIC/TSC weights are continuous across a cell edge, so x * (1/h) may
+    // pick the neighbouring index at an exact edge without changing the sum
+    const double u = x * g.hinv;
+    const int kk = (int)u;
+    k = kk;
+    const float t = (float)(u - (kk + 0.5));
     if (SCHEME == 1) {
         w[1] = 1.0f - fabsf(t);
         w[0] = t < 0.0f ? -t : 0.0f;
@@ -260,7 +295,7 @@ __device__ int block_exscan4(int *a, int n, int *wsum) {
 }
 
 // ------------------------------------------------------------------ WCA
-// One thread per particle (slot).  Neighbours come from the cell list of
+// One thread per list position.  Neighbours come from the cell list of
 // the 3x3 cells around the particle's own cell (the extended-list semantics
 // of dm:459-526): entries are slot | species << 31, each cell's entries
 // sorted by slot, so every sum has a fixed order.  Same-species and
@@ -269,67 +304,25 @@ __device__ int block_exscan4(int *a, int n, int *wsum) {
 constexpr unsigned LIST_SLOT = 0x7fffffffu;
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d, int fast) {
+__global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
     using R2 = typename V2<R>::T;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= g.N) return;
+    const int qi = xcd_block(g.xcd) * blockDim.x + threadIdx.x;
+    if (qi >= g.N) return;
     const int nc = g.nc;
-    const int c = d.cell[s];
-    const int cx = c / nc, cy = c - cx * nc;
-    const double2 ri = d.rel[s];
-    const bool s2i = d.pid[s] >= g.n1;
     const unsigned *list = (const unsigned *)d.list;
+    const int s = (int)(list[qi] & LIST_SLOT);
+    const int c = d.lcell[qi];
+    const int cx = div_nc(c, g), cy = c - cx * nc;
+    const double2 ri = d.lrel[qi];
+    const bool s2i = d.lpid[qi] >= g.n1;
     R fsx = 0, fsy = 0, fcx = 0, fcy = 0;
-    if (fast) {
-        // FP32 forces: displacement from the cell offset and the in-cell
-        // offsets (minimum image by construction for nc >= 6)
-        const float B = (float)g.B, cut2 = (float)g.cut2, cap = (float)g.maxpair;
-        const float s22 = (float)g.s22, s12 = (float)g.s12, e22 = (float)g.e22f, e12 = (float)g.e12f;
-        for (int da = -1; da <= 1; ++da)
-            for (int db = -1; db <= 1; ++db) {
-                const int cc = wrapc(cx + da, nc) * nc + wrapc(cy + db, nc);
-                const int qe = d.start[cc + 1];
-                const float ox = -da * B, oy = -db * B;
-                for (int q = d.start[cc]; q < qe; ++q) {
-                    const unsigned ent = list[q];
-                    const int sj = (int)(ent & LIST_SLOT);
-                    if (sj == s) continue;
-                    const bool s2j = (ent >> 31) != 0;
-                    const double2 rj = d.rel[sj];
-                    const float dx = (float)(ri.x - rj.x) + ox, dy = (float)(ri.y - rj.y) + oy;
-                    const float r2 = dx * dx + dy * dy;
-                    float sig = 1.0f, pref = 1.0f;
-                    if (s2i && s2j) {
-                        sig = s22;
-                        pref = e22;
-                    } else if (s2i != s2j) {
-                        sig = s12;
-                        pref = e12;
-                    }
-                    const float r2s = r2 * sig * sig;
-                    if (r2s > cut2) continue;
-                    const float inv = __builtin_amdgcn_rcpf(r2s);
-                    const float i2 = inv * inv, i4 = i2 * i2;
-                    const float a_8 = i4, a_14 = i4 * i2 * inv;
-                    float absF = pref * 24.0f * (2.0f * a_14 - a_8);
-                    if (absF > cap) absF = cap;
-                    const float fx = absF * dx * sig, fy = absF * dy * sig;
-                    if (s2i == s2j) {
-                        fsx += fx;
-                        fsy += fy;
-                    } else {
-                        fcx += fx;
-                        fcy += fy;
-                    }
-                }
-            }
-    } else {
+    {
         // the reference's FP64 arithmetic on absolute positions (am:100-166,
         // dm:640-698); the S2 side of a 12 pair is the negated S1 side
         const double xi = cx * g.B + ri.x, yi = cy * g.B + ri.y;
         for (int da = -1; da <= 1; ++da)
             for (int db = -1; db <= 1; ++db) {
-                const int ux = wrapc(cx + da, nc), uy = wrapc(cy + db, nc);
+                const int ux = wrap1(cx + da, nc), uy = wrap1(cy + db, nc);
                 const int cc = ux * nc + uy;
                 const int qe = d.start[cc + 1];
                 for (int q = d.start[cc]; q < qe; ++q) {
@@ -337,7 +330,7 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d, int fast) {
                     const int sj = (int)(ent & LIST_SLOT);
                     if (sj == s) continue;
                     const bool s2j = (ent >> 31) != 0;
-                    const double2 rj = d.rel[sj];
+                    const double2 rj = d.lrel[q];
                     const double xj = ux * g.B + rj.x, yj = uy * g.B + rj.y;
                     R fx, fy;
                     if (!s2i && !s2j) {
@@ -355,7 +348,90 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d, int fast) {
     R2 o;
     o.x = fsx + fcx;
     o.y = fsy + fcy;
-    ((R2 *)d.fwca)[s] = o;
+    ((R2 *)d.fwca)[qi] = o;
+}
+
+// FP32 forces for boxes of at least 6 cells per side: the displacement is
+// the in-cell offset difference plus the cell offset (the minimum image by
+// construction).  The list entry and rel of a candidate are loaded together,
+// one candidate ahead of the arithmetic.  UNIT: all sigma and epsilon ratios
+// are 1, so the per-type factors drop out (same results, fewer operations).
+// Summation order and grouping are those of k_wca.
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
+    const int qi = xcd_block(g.xcd) * blockDim.x + threadIdx.x;
+    if (qi >= g.N) return;
+    const int nc = g.nc;
+    const unsigned *list = (const unsigned *)d.list;
+    const int c = d.lcell[qi];
+    const int cx = div_nc(c, g), cy = c - cx * nc;
+    const double2 ri = d.lrel[qi];
+    const bool s2i = d.lpid[qi] >= g.n1;
+    const float B = (float)g.B, cut2 = (float)g.cut2, cap = (float)g.maxpair;
+    // factors for a partner of species 1 (A) or species 2 (Bp)
+    const float sigA = s2i ? (float)g.s12 : 1.0f, prefA = s2i ? (float)g.e12f : 1.0f;
+    const float sigB = s2i ? (float)g.s22 : (float)g.s12, prefB = s2i ? (float)g.e22f : (float)g.e12f;
+    float fsx = 0, fsy = 0, fcx = 0, fcy = 0;
+    // candidates q0..qe-1; oy = B below m1, 0 below m2, -B from m2 on
+    auto range = [&](int q0, int qe, int m1, int m2, float ox) {
+        if (q0 >= qe) return;
+        unsigned ent = list[q0];
+        double2 rj = d.lrel[q0];
+        for (int q = q0; q < qe; ++q) {
+            const unsigned e = ent;
+            const double2 r = rj;
+            if (q + 1 < qe) {
+                ent = list[q + 1];
+                rj = d.lrel[q + 1];
+            }
+            const float oy = q < m1 ? B : (q < m2 ? 0.0f : -B);
+            const float dx = (float)(ri.x - r.x) + ox, dy = (float)(ri.y - r.y) + oy;
+            const float r2 = dx * dx + dy * dy;
+            const bool s2j = (e >> 31) != 0;
+            const float sig = UNIT ? 1.0f : (s2j ? sigB : sigA);
+            const float r2s = UNIT ? r2 : r2 * sig * sig;
+            if (r2s <= cut2 && q != qi) {
+                const float inv = __builtin_amdgcn_rcpf(r2s);
+                const float i2 = inv * inv, i4 = i2 * i2;
+                const float a_8 = i4, a_14 = i4 * i2 * inv;
+                float absF;
+                if (UNIT)
+                    absF = 24.0f * (2.0f * a_14 - a_8);
+                else
+                    absF = (s2j ? prefB : prefA) * 24.0f * (2.0f * a_14 - a_8);
+                if (absF > cap) absF = cap;
+                const float fx = UNIT ? absF * dx : absF * dx * sig;
+                const float fy = UNIT ? absF * dy : absF * dy * sig;
+                if (s2i == s2j) {
+                    fsx += fx;
+                    fsy += fy;
+                } else {
+                    fcx += fx;
+                    fcy += fy;
+                }
+            }
+        }
+    };
+    const bool inner = cy >= 1 && cy <= nc - 2;
+    for (int da = -1; da <= 1; ++da) {
+        const int row = wrap1(cx + da, nc) * nc;
+        const float ox = -da * B;
+        if (inner) {
+            // cells cy-1, cy, cy+1 of a row are adjacent in the list
+            const int *st = d.start + row + cy - 1;
+            range(st[0], st[3], st[1], st[2], ox);
+        } else {
+            for (int db = -1; db <= 1; ++db) {
+                const int cc = row + wrap1(cy + db, nc);
+                const int q0 = d.start[cc], qe = d.start[cc + 1];
+                range(q0, qe, db < 0 ? qe : q0, db > 0 ? q0 : qe, ox);
+            }
+        }
+    }
+    float2 o;
+    o.x = fsx + fcx;
+    o.y = fsy + fcy;
+    ((float2 *)d.fwca)[qi] = o;
 }
 
 // ------------------------------------------------------------------ deposit
@@ -384,7 +460,7 @@ __device__ int first_owned(int c, const Geo &g) {
 
 struct DepArgs {
     int tc;        // list cells per tile side
-    int ncs_cap;   // staged-cell capacity of the flattened path
+    int seg_cap;   // staged list segments (rows, two per row when y wraps)
     int win_cap;   // owned mesh points per tile (capacity)
     double scale_hi, scale_lo;      // 2^S1, 2^S2
     double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
@@ -392,30 +468,28 @@ struct DepArgs {
 };
 
 template <typename R, int SCHEME>
-__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, unsigned ent,
-                                                 int c, int klo, int khi, int llo, int lhi,
+__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int q,
+                                                 int gx, int gy, int klo, int khi, int llo, int lhi,
                                                  unsigned long long *acc_hi, unsigned long long *acc_lo) {
-    const int s = (int)(ent & LIST_SLOT);
-    const bool s2 = (ent >> 31) != 0;
-    const int gx = c / g.nc, gy = c - gx * g.nc;
-    const double2 rr = d.rel[s];
+    const bool s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
+    const double2 rr = d.lrel[q];
     const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
     int kp, lp;
     R wx[3], wy[3];
-    weights_r<R, SCHEME>(x, g.h, kp, wx);
-    weights_r<R, SCHEME>(y, g.h, lp, wy);
+    weights_r<R, SCHEME>(x, g, kp, wx);
+    weights_r<R, SCHEME>(y, g, lp, wy);
     const int Z = g.Z, nl = lhi - llo;
 #pragma unroll
     for (int ia = 0; ia < 3; ++ia) {
         if (SCHEME == 0 && ia != 1) continue;
         if (wx[ia] == 0.0) continue;
-        const int k = wrapc(kp + ia - 1, Z);
+        const int k = (kp + ia - 1) & (Z - 1);
         if (k < klo || k >= khi) continue;
 #pragma unroll
         for (int ib = 0; ib < 3; ++ib) {
             if (SCHEME == 0 && ib != 1) continue;
             if (wy[ib] == 0.0) continue;
-            const int l = wrapc(lp + ib - 1, Z);
+            const int l = (lp + ib - 1) & (Z - 1);
             if (l < llo || l >= lhi) continue;
             const int w = (k - klo) * nl + (l - llo);
             if (std::is_same<R, float>::value) {
@@ -451,17 +525,19 @@ __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArg
     extern __shared__ unsigned char smem_raw[];
     unsigned long long *acc_hi = (unsigned long long *)smem_raw;
     unsigned long long *acc_lo = acc_hi + a.win_cap;
-    int *cstart = (int *)(acc_lo + (std::is_same<R, double>::value ? a.win_cap : 0));
-    int *cwrap = cstart + a.ncs_cap + 1;
+    int *seg_q = (int *)(acc_lo + (std::is_same<R, double>::value ? a.win_cap : 0));
+    int *seg_pre = seg_q + a.seg_cap;       // [seg_cap + 1]
+    int *seg_row = seg_pre + a.seg_cap + 1;
     __shared__ int info[4];
     __shared__ int wsum[16];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nc = g.nc, tc = a.tc;
-    const int cx0 = blockIdx.x * tc, cy0 = blockIdx.y * tc;
+    const int lb = xcd_block(0);  // hardware order: measured faster than per-XCD ranges for this kernel
+    const int cx0 = (lb % gridDim.x) * tc, cy0 = (lb / gridDim.x) * tc;
     const int nx = min(tc, nc - cx0), ny = min(tc, nc - cy0);
     const unsigned long long step = d.sc->step;
     const double e = dbits(d.sc->exc_bits[step & 1]) + 1e-9 * g.L;
-    if (a.reset && blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+    if (a.reset && lb == 0 && tid == 0) {
         // per-step scalars for the kernels that follow this one
         d.sc->dt_bits = bitsd(g.dt_max);
         d.sc->exc_bits[(step + 1) & 1] = 0ull;
@@ -484,59 +560,56 @@ __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArg
     // staged cells: the tile plus hx halo cells, each distinct cell once (a
     // halo that would wrap onto the tile itself stages the whole axis)
     const bool allx = nx + 2 * hx >= nc, ally = ny + 2 * hx >= nc;
-    const int SX = allx ? nc : nx + 2 * hx, SY = ally ? nc : ny + 2 * hx, ncs = SX * SY;
+    const int SX = allx ? nc : nx + 2 * hx, SY = ally ? nc : ny + 2 * hx;
     const int sx0 = allx ? 0 : cx0 - hx, sy0 = ally ? 0 : cy0 - hx;
-    const unsigned *list = (const unsigned *)d.list;
-    if (ncs <= a.ncs_cap) {
-        for (int idx = tid; idx < ncs; idx += nt) {
-            const int ia = idx / SY, ib = idx - ia * SY;
-            const int c = wrapc(sx0 + ia, nc) * nc + wrapc(sy0 + ib, nc);
-            cwrap[idx] = c;
-            cstart[idx] = d.start[c + 1] - d.start[c];
+    // The list holds the particles in cell order, so the staged cells of one
+    // row (fixed x) are one contiguous list range, or two when the y range
+    // wraps: segments of consecutive list positions.
+    const int ywrap = (sy0 < 0 || sy0 + SY > nc) ? 1 : 0;
+    const int nsr = 1 + ywrap, nseg = SX * nsr;
+    __syncthreads();  // accumulators cleared before the first atomic
+    if (nseg <= a.seg_cap) {
+        for (int i = tid; i < nseg; i += nt) {
+            const int r = i / nsr, part = i - r * nsr;
+            const int gx = wrap1(sx0 + r, nc), row = gx * nc;
+            int ya, yb;
+            if (!ywrap) {
+                ya = sy0;
+                yb = sy0 + SY;
+            } else if (sy0 < 0) {
+                ya = part == 0 ? sy0 + nc : 0;
+                yb = part == 0 ? nc : sy0 + SY;
+            } else {
+                ya = part == 0 ? sy0 : 0;
+                yb = part == 0 ? nc : sy0 + SY - nc;
+            }
+            const int qa = d.start[row + ya];
+            seg_q[i] = qa;
+            seg_pre[i] = d.start[row + yb] - qa;
+            seg_row[i] = gx;
         }
         __syncthreads();
-        int total = 0;
-        if (ncs <= 4 * nt) {
-            total = block_exscan4(cstart, ncs, wsum);
-        } else {
-            // serial prefix in chunks by one wave (large staged regions)
-            if (tid < 64) {
-                int run = 0;
-                for (int base = 0; base < ncs; base += 64) {
-                    int i = base + tid;
-                    int v = i < ncs ? cstart[i] : 0, x = v;
-                    for (int o = 1; o < 64; o <<= 1) {
-                        int y = __shfl_up(x, o, 64);
-                        if (tid >= o) x += y;
-                    }
-                    if (i < ncs) cstart[i] = run + x - v;
-                    run += __shfl(x, 63, 64);
-                }
-                if (tid == 0) wsum[0] = run;
-            }
-            __syncthreads();
-            total = wsum[0];
-        }
-        if (tid == 0) cstart[ncs] = total;
+        const int total = block_exscan4(seg_pre, nseg, wsum);
+        if (tid == 0) seg_pre[nseg] = total;
         __syncthreads();
         for (int t = tid; t < total; t += nt) {
-            int lo = 0, hi = ncs - 1;  // last staged cell with cstart <= t
+            int lo = 0, hi = nseg - 1;  // last segment with seg_pre <= t
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (cstart[mid] <= t) lo = mid;
+                if (seg_pre[mid] <= t) lo = mid;
                 else hi = mid - 1;
             }
-            const int c = cwrap[lo];
-            const unsigned ent = list[d.start[c] + (t - cstart[lo])];
-            deposit_particle<R, SCHEME>(g, d, a, ent, c, klo, khi, llo, lhi, acc_hi, acc_lo);
+            const int q = seg_q[lo] + (t - seg_pre[lo]);
+            const int gx = seg_row[lo];
+            deposit_particle<R, SCHEME>(g, d, a, q, gx, d.lcell[q] - gx * nc, klo, khi, llo, lhi, acc_hi, acc_lo);
         }
     } else {
-        __syncthreads();  // accumulators cleared before the first atomic
-        for (int idx = tid; idx < ncs; idx += nt) {
+        for (int idx = tid; idx < SX * SY; idx += nt) {
             const int ia = idx / SY, ib = idx - ia * SY;
-            const int c = wrapc(sx0 + ia, nc) * nc + wrapc(sy0 + ib, nc);
+            const int gx = wrap1(sx0 + ia, nc), gy = wrap1(sy0 + ib, nc);
+            const int c = gx * nc + gy;
             for (int q = d.start[c]; q < d.start[c + 1]; ++q)
-                deposit_particle<R, SCHEME>(g, d, a, list[q], c, klo, khi, llo, lhi, acc_hi, acc_lo);
+                deposit_particle<R, SCHEME>(g, d, a, q, gx, gy, klo, khi, llo, lhi, acc_hi, acc_lo);
         }
     }
     __syncthreads();
@@ -630,27 +703,26 @@ __device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int logn, int nr
 }
 
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_EPT = 32;
 
 // Row r2c: rows k0..k0+RB-1 of the real mesh -> half spectrum X[m], m<Z/2,
 // with X[0] packing (DC, Nyquist) (both real), written transposed:
 // ht[m*Z + k].  Length-Z/2 complex FFT of (x[2n], x[2n+1]) + split.
-template <typename R>
+template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
                                                           const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
-                                                          int RB, int stride) {
+                                                          int RB, int stride, int remap) {
     using C = typename V2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = Z >> 1, logn = logZ - 1;
-    const int k0 = blockIdx.x * RB;
+    const int k0 = xcd_block(remap) * RB;
     for (int idx = tid; idx < RB * n; idx += nt) {
         const int row = idx >> logn, m = idx & (n - 1);
         s[row * stride + m] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
     }
     __syncthreads();
-    lds_fft<R, FFT_EPT>(s, logn, RB, stride, tw, logZ, false);
+    lds_fft<R, EPT>(s, logn, RB, stride, tw, logZ, false);
     const int half = n >> 1;
     for (int idx = tid; idx < RB * (half + 1); idx += nt) {
         const int row = idx / (half + 1), m = idx - row * (half + 1);
@@ -683,14 +755,15 @@ __device__ __forceinline__ void symbols(int j, int mm, const Geo &g, const R *sq
         G = (R)0.0;
     else
         G = (R)1.0 / ((lap[j] + lap[mm]) + (R)g.lam2inv);
-    const R Zr = (R)g.Z;
-    gx = sq[j] * G / Zr / Zr;
-    gy = sq[mm] * G / Zr / Zr;
+    // x / Z / Z == x * 2^(-2 log2 Z) exactly (Z is a power of two)
+    const R izz = (R)1.0 / ((R)g.Z * (R)g.Z);
+    gx = sq[j] * G * izz;
+    gy = sq[mm] * G * izz;
 }
 
 // Column pass: rows m0..m0+RC-1 of ht (length Z along x), forward FFT,
 // multiply by i*gx and i*gy, inverse FFT of both, out gt[m*Z+k] = (Fx, Fy).
-template <typename R>
+template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__restrict__ ht,
                                                      typename V4<R>::T *__restrict__ gt,
                                                      const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
@@ -708,7 +781,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__
         s[row * stride + k] = ht[(size_t)(m0 + row) * Z + k];
     }
     __syncthreads();
-    lds_fft<R, FFT_EPT>(s, logZ, RC, stride, tw, logZ, false);
+    lds_fft<R, EPT>(s, logZ, RC, stride, tw, logZ, false);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = idx >> logZ, j = idx & (Z - 1);
         const int m = m0 + row;
@@ -744,7 +817,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__
         }
     }
     __syncthreads();
-    lds_fft<R, FFT_EPT>(s, logZ, 2 * RC, stride, tw, logZ, true);
+    lds_fft<R, EPT>(s, logZ, 2 * RC, stride, tw, logZ, true);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = idx >> logZ, k = idx & (Z - 1);
         C a = s[row * stride + k], b = sB[row * stride + k];
@@ -760,18 +833,18 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__
 // Row c2r of the two real signals Fx, Fy: rows k0..k0+RB2-1, spectrum
 // C[m] = X[m] + i Y[m] completed by Hermitian symmetry, inverse FFT of
 // length Z, out fmesh[k*Z + l] = (Re, Im) = (Fx, Fy).
-template <typename R>
+template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
                                                           typename V2<R>::T *__restrict__ fm,
                                                           const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
-                                                          int RB2, int stride) {
+                                                          int RB2, int stride, int remap) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int Zh = Z >> 1;
-    const int k0 = blockIdx.x * RB2;
+    const int k0 = xcd_block(remap) * RB2;
     for (int idx = tid; idx < RB2 * Zh; idx += nt) {
         const int row = idx % RB2, m = idx / RB2;
         C4 v = gt[(size_t)m * Z + k0 + row];
@@ -785,7 +858,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::
         }
     }
     __syncthreads();
-    lds_fft<R, FFT_EPT>(s, logZ, RB2, stride, tw, logZ, true);
+    lds_fft<R, EPT>(s, logZ, RB2, stride, tw, logZ, true);
     for (int idx = tid; idx < RB2 * Z; idx += nt) {
         const int row = idx >> logZ, l = idx & (Z - 1);
         fm[(size_t)(k0 + row) * Z + l] = s[row * stride + l];
@@ -800,10 +873,10 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
     const int Z = g.Z;
     int k, l;
     R wx[3], wy[3];
-    weights_r<R, SCHEME>(x, g.h, k, wx);
-    weights_r<R, SCHEME>(y, g.h, l, wy);
-    k = wrapc(k, Z);
-    l = wrapc(l, Z);
+    weights_r<R, SCHEME>(x, g, k, wx);
+    weights_r<R, SCHEME>(y, g, l, wy);
+    k &= Z - 1;  // Z is a power of two
+    l &= Z - 1;
     if (SCHEME == 0) {
         R2 f = fm[(size_t)k * Z + l];
         if (!s2) {
@@ -866,13 +939,13 @@ __device__ __forceinline__ double2 particle_noise(const Geo &g, const DevBufs &d
 template <typename R, int SCHEME>
 __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
     using R2 = typename V2<R>::T;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(g.xcd) * blockDim.x + threadIdx.x;  // list position
     double cand = INFINITY;
     if (s < g.N) {
-        const int c = d.cell[s];
-        const int gx = c / g.nc, gy = c - gx * g.nc;
-        const double2 r = d.rel[s];
-        const int pid = d.pid[s];
+        const int c = d.lcell[s];
+        const int gx = div_nc(c, g), gy = c - gx * g.nc;
+        const double2 r = d.lrel[s];
+        const int pid = d.lpid[s];
         double fx = 0.0, fy = 0.0;
         if (!g.idgas) gather_kap<R, SCHEME>(gx * g.B + r.x, gy * g.B + r.y, pid >= g.n1, (const R2 *)d.fmesh, g, fx, fy);
         R2 o;
@@ -904,13 +977,13 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
 template <typename R, int SCHEME, bool FUSED>
 __global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
     using R2 = typename V2<R>::T;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(g.xcd) * blockDim.x + threadIdx.x;  // list position in, slot out
     if (s >= g.N) return;
     const unsigned long long step = d.sc->step;
-    const int c = d.cell[s];
-    int gx = c / g.nc, gy = c - gx * g.nc;
-    double2 r = d.rel[s];
-    const int pid = d.pid[s];
+    const int c = d.lcell[s];
+    int gx = div_nc(c, g), gy = c - gx * g.nc;
+    double2 r = d.lrel[s];
+    const int pid = d.lpid[s];
     const bool s2 = pid >= g.n1;
     double fwx = 0.0, fwy = 0.0, fkx = 0.0, fky = 0.0;
     if (!g.idgas) {
@@ -963,6 +1036,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
     }
     d.cell[s] = nc_new;
     d.rel[s] = rn;
+    d.pid[s] = pid;
     d.rank[s] = atomicAdd(&d.count[nc_new], 1);
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
     if (ex > 0.0) atomicMax(&d.sc->exc_bits[(step + 1) & 1], bitsd(ex));
@@ -1129,21 +1203,67 @@ __global__ void k_sort_cells(Geo g, DevBufs d) {
     }
 }
 
+// the particle state in list order, read by the force and update kernels
+__global__ void k_list_state(Geo g, DevBufs d) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.N) return;
+    const int sl = (int)(((const unsigned *)d.list)[q] & LIST_SLOT);
+    d.lcell[q] = d.cell[sl];
+    d.lrel[q] = d.rel[sl];
+    d.lpid[q] = d.pid[sl];
+}
+
 // ------------------------------------------------------------------ launchers
 static inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); }
 
-template <typename R> struct FftCfg {
-    int RB, RC, RB2, sR1, sC, sR2;
-    size_t l1, lc, l2;
+// Elements per thread (EPT) of the three FFT kernels; a block of
+// FFT_THREADS threads holds EPT * FFT_THREADS complex values.  The defaults
+// (16, 16, 32) were the fastest of {8, 16, 32} per kernel at Z = 2048 (FP32):
+// 4 rows per r2c block (512 blocks), one column per k_col block (1024),
+// 4 rows per c2r block (512), so the transposed reads of c2r move 64 B per
+// row group.  Each EPT is raised to the smallest value whose block still
+// holds one transform.  BDK_FFT_EPT="a,b,c" (8, 16 or 32) overrides them.
+struct FftCfg {
+    int e1, e2, e3;      // EPT of r2c rows, columns, c2r rows
+    int RB, RC, RB2;     // rows per block
+    int sR1, sC, sR2;    // LDS row strides (padded)
+    size_t l1, lc, l2;   // LDS bytes
 };
 
-template <typename R> static FftCfg<R> fft_cfg(int Z) {
+static const int *fft_ept_default() {
+    static int v[3] = {0, 0, 0};
+    if (!v[0]) {
+        int a = 16, b = 16, c = 32;
+        if (const char *e = getenv("BDK_FFT_EPT")) {
+            int x, y, z;
+            if (sscanf(e, "%d,%d,%d", &x, &y, &z) == 3) {
+                auto ok = [](int t) { return t == 8 || t == 16 || t == 32; };
+                if (ok(x) && ok(y) && ok(z)) a = x, b = y, c = z;
+            }
+        }
+        v[1] = b;
+        v[2] = c;
+        v[0] = a;
+    }
+    return v;
+}
+
+static int ept_for(int want, int need) {
+    int e = want;
+    while (e * FFT_THREADS < need && e < 32) e *= 2;
+    return e;
+}
+
+template <typename R> static FftCfg fft_cfg(int Z) {
     using C = typename V2<R>::T;
-    const int E = FFT_EPT * FFT_THREADS;  // complex elements per block
-    FftCfg<R> c;
-    c.RB = std::min(Z, E / (Z / 2));
-    c.RC = std::max(1, std::min(Z / 2, E / (2 * Z)));
-    c.RB2 = std::min(Z, E / Z);
+    const int *w = fft_ept_default();
+    FftCfg c;
+    c.e1 = ept_for(w[0], Z / 2);
+    c.e2 = ept_for(w[1], 2 * Z);
+    c.e3 = ept_for(w[2], Z);
+    c.RB = std::min(Z, c.e1 * FFT_THREADS / (Z / 2));
+    c.RC = std::max(1, std::min(Z / 2, c.e2 * FFT_THREADS / (2 * Z)));
+    c.RB2 = std::min(Z, c.e3 * FFT_THREADS / Z);
     c.sR1 = Z / 2 + 1;
     c.sC = Z + 1;
     c.sR2 = Z + 1;
@@ -1158,12 +1278,17 @@ hipError_t set_kernel_attributes() {
     hipError_t e = hipSuccess;
 #define BDK_ATTR(K)                                                                            \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-    BDK_ATTR(k_r2c_rows<float>)
-    BDK_ATTR(k_r2c_rows<double>)
-    BDK_ATTR(k_col<float>)
-    BDK_ATTR(k_col<double>)
-    BDK_ATTR(k_c2r_rows<float>)
-    BDK_ATTR(k_c2r_rows<double>)
+#define BDK_ATTR_E(E)                \
+    BDK_ATTR((k_r2c_rows<float, E>))  \
+    BDK_ATTR((k_r2c_rows<double, E>)) \
+    BDK_ATTR((k_col<float, E>))       \
+    BDK_ATTR((k_col<double, E>))      \
+    BDK_ATTR((k_c2r_rows<float, E>))  \
+    BDK_ATTR((k_c2r_rows<double, E>))
+    BDK_ATTR_E(8)
+    BDK_ATTR_E(16)
+    BDK_ATTR_E(32)
+#undef BDK_ATTR_E
 #undef BDK_ATTR
     // the deposit also has a little static LDS: ask for 128 KB of dynamic LDS
 #define BDK_ATTR_D(K)                                                                          \
@@ -1181,11 +1306,19 @@ hipError_t set_kernel_attributes() {
 hipError_t launch_wca(bdk_t *h) {
     // FP32 forces use the cell-relative displacement when the box holds at
     // least 6 cells per side (the neighbour offset is then the minimum image)
-    const int fast = (h->p.precision == BDK_FP32 && h->g.nc >= 6) ? 1 : 0;
-    if (h->p.precision == BDK_FP32)
-        hipLaunchKernelGGL(k_wca<float>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, fast);
-    else
-        hipLaunchKernelGGL(k_wca<double>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
+    const dim3 grid(nblk(h->g.N, 256)), blk(256);
+    if (h->p.precision == BDK_FP32 && h->g.nc >= 6) {
+        const bool unit = h->p.sigma11_22 == 1.0 && h->p.sigma11_12 == 1.0 && h->p.eps22_11 == 1.0 &&
+                          h->p.eps12_11 == 1.0;
+        if (unit)
+            hipLaunchKernelGGL(k_wca_f32<true>, grid, blk, 0, h->stream, h->g, h->d);
+        else
+            hipLaunchKernelGGL(k_wca_f32<false>, grid, blk, 0, h->stream, h->g, h->d);
+    } else if (h->p.precision == BDK_FP32) {
+        hipLaunchKernelGGL(k_wca<float>, grid, blk, 0, h->stream, h->g, h->d);
+    } else {
+        hipLaunchKernelGGL(k_wca<double>, grid, blk, 0, h->stream, h->g, h->d);
+    }
     return hipGetLastError();
 }
 
@@ -1208,7 +1341,7 @@ static DepPlan dep_plan(const bdk_t *h, int reset) {
     const int side = (int)std::floor(tc * g.B / g.h) + 2;
     pl.a.tc = tc;
     pl.a.win_cap = side * side;
-    pl.a.ncs_cap = (tc + 6) * (tc + 6);
+    pl.a.seg_cap = std::min(2 * (tc + 6), 4 * DP_THREADS);  // block_exscan4 limit
     const double cmax = g.drho * std::max(1.0, std::fabs(g.f2_f1));
     const double nlog = std::log2((double)g.N * cmax + 1.0);
     int s1 = (int)std::floor(62.0 - nlog);
@@ -1221,7 +1354,7 @@ static DepPlan dep_plan(const bdk_t *h, int reset) {
     pl.a.iscale_lo = std::ldexp(1.0, -s2);
     pl.a.reset = reset;
     pl.grid = dim3(nblk(g.nc, tc), nblk(g.nc, tc));
-    pl.lds = (size_t)pl.a.win_cap * (f64 ? 16 : 8) + (size_t)(2 * pl.a.ncs_cap + 1) * 4;
+    pl.lds = (size_t)pl.a.win_cap * (f64 ? 16 : 8) + (size_t)(3 * pl.a.seg_cap + 1) * 4;
     return pl;
 }
 
@@ -1247,22 +1380,35 @@ hipError_t launch_rearm(bdk_t *h) {
     return hipGetLastError();
 }
 
+#define BDK_EPT_SWITCH(E, CALL) \
+    switch (E) {                \
+    case 8: CALL(8); break;     \
+    case 16: CALL(16); break;   \
+    default: CALL(32); break;   \
+    }
+
 template <typename R> static void launch_fft_t(bdk_t *h, int only = -1) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     const int Z = h->g.Z, logZ = h->g.logZ;
-    FftCfg<R> c = fft_cfg<R>(Z);
+    const FftCfg c = fft_cfg<R>(Z);
     const R *sq = std::is_same<R, float>::value ? (const R *)h->d.sqf : (const R *)h->d.sq;
     const R *lap = std::is_same<R, float>::value ? (const R *)h->d.lapf : (const R *)h->d.lap;
-    if (only < 0 || only == 0)
-        hipLaunchKernelGGL((k_r2c_rows<R>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream,
-                           (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1);
-    if (only < 0 || only == 1)
-        hipLaunchKernelGGL((k_col<R>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream,
-                           (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC);
-    if (only < 0 || only == 2)
-        hipLaunchKernelGGL((k_c2r_rows<R>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream,
-                           (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2);
+#define R2C(E)                                                                                              \
+    hipLaunchKernelGGL((k_r2c_rows<R, E>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream,             \
+                       (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1, h->g.xcd)
+#define COL(E)                                                                                              \
+    hipLaunchKernelGGL((k_col<R, E>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream,            \
+                       (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC)
+#define C2R(E)                                                                                              \
+    hipLaunchKernelGGL((k_c2r_rows<R, E>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream,            \
+                       (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2, h->g.xcd)
+    if (only < 0 || only == 0) BDK_EPT_SWITCH(c.e1, R2C)
+    if (only < 0 || only == 1) BDK_EPT_SWITCH(c.e2, COL)
+    if (only < 0 || only == 2) BDK_EPT_SWITCH(c.e3, C2R)
+#undef R2C
+#undef COL
+#undef C2R
 }
 
 hipError_t launch_fft_pass(bdk_t *h, int pass) {
@@ -1325,6 +1471,7 @@ static void launch_scan_fill_inner(bdk_t *h, int inc_step) {
                        ncell);
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    hipLaunchKernelGGL(k_list_state, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
 }
 
 hipError_t launch_scan_fill(bdk_t *h) {
@@ -1406,6 +1553,8 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
     mark("k_sort_cells");
+    hipLaunchKernelGGL(k_list_state, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    mark("k_list_state");
     chk(hipGetLastError());
     return e;
 }

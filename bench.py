@@ -1,7 +1,8 @@
 """Benchmark of the MI355X BD timestep: particle-steps/s at 2M particles with a
 2048^2 TSC mesh (L = 1414 sigma, N1 = N2 = 10^6, lambda = 50, q2/q1 = 0.5,
 kT = 1, dt = 1e-4, FP32 forces with FP64 positions), synthetic clipped
-Gaussian-blob start (hexagonal-lattice thinning, exclusion 0.9 sigma).
+Gaussian-blob start (thinned hexagonal lattice of spacing 0.95 sigma, so every pair
+is farther apart than the 0.9 sigma exclusion).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
@@ -23,7 +24,7 @@ PEAK_HBM_GBS = 8000.0
 
 WORKLOAD = dict(name="2M particles, TSC 2048^2, L=1414, lambda=50, blob start",
                 n1=1_000_000, n2=1_000_000, L=1414.0, Z=2048, lam=50.0, kap=1.0, f2_f1=0.5, T=1.0,
-                dt=1e-4, scheme=2, blob_width=1414.0 / 6.0, background=0.1, dmin=0.9, seed=1)
+                dt=1e-4, scheme=2, blob_width=1414.0 / 6.0, background=0.1, dmin=0.95, seed=1)
 
 
 def log(*a):
@@ -60,9 +61,9 @@ def canonical_step_bytes(N, Z):
 
 
 def load_traffic(kernel):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE), scaled by the counters' reading for a copy kernel of known
-    size; committed under profiles/ as JSON."""
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by the same counter's reading
+    for a copy kernel of known size; committed under profiles/ as JSON."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
@@ -71,9 +72,39 @@ def load_traffic(kernel):
         k = d["kernels"].get(kernel)
         if not k:
             return None
-        return (k["fetch_kib"] + k["write_kib"]) * 1024.0 * d.get("copy_scale", 1.0)
-    except Exception:
+        return 1024.0 * (k["fetch_kib"] * d["fetch_scale"] + k["write_kib"] * d["write_scale"])
+    except (OSError, KeyError, ValueError, TypeError):
         return None
+
+
+PROBE_COPY_BYTES = 256 << 20
+
+
+def kernel_probe(sim, reps):
+    """For rocprofv3 --pmc passes: each step kernel `reps` times at the
+    benchmark state, then `reps` device-to-device copies of PROBE_COPY_BYTES
+    (the runtime's copy kernel reads and writes each byte once), whose
+    counter readings calibrate FETCH_SIZE and WRITE_SIZE."""
+    import ctypes as C
+
+    for k in ("k_wca", "k_deposit", "k_r2c_rows", "k_col", "k_c2r_rows", "k_gather", "k_update"):
+        sim.profile_kernel(k, reps)
+    sim.sync()
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(PROBE_COPY_BYTES)
+    for buf in (a, b):
+        if hip.hipMalloc(C.byref(buf), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, n) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, n, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("kernel probe: %d launches per kernel, %d copies of %d bytes" % (reps, reps, PROBE_COPY_BYTES))
 
 
 def cpu_baseline(p, git, rel, budget_s=20.0):
@@ -112,6 +143,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--kernel-probe", type=int, default=0, metavar="REPS",
+                    help="run each kernel REPS times plus a calibration copy and exit (for --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one simulation on one GPU; --gpus %d is not supported "
@@ -129,6 +162,11 @@ def main():
     sim = bdk.RunDynamik(p)
     sim.set_positions(git, rel)
     N = w["n1"] + w["n2"]
+    if args.kernel_probe:
+        sim.run_steps(args.warmup)
+        kernel_probe(sim, args.kernel_probe)
+        sim.close()
+        return
     log("bench: warm-up %d steps" % args.warmup)
     sim.run_steps(args.warmup)
     sim.sync()
