@@ -108,8 +108,8 @@ Footprint footprint_of(const bd_params &p) {
     const size_t ncell = (size_t)d.nc * d.nc, Z = (size_t)p.Z;
     Footprint f;
     // cell, rel, pid, fwca, fkap, noise_up, noise_out, rank, list
-    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4);
-    f.cells = ncell * 8 + 4 + 4096 * 4;
+    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4 + 8);
+    f.cells = ncell * 12 + 4 + 4096 * 4;
     // rho, ht, gt, fmesh, tables
     f.mesh = Z * Z * R + (Z / 2) * Z * 2 * R + (Z / 2) * Z * 4 * R + Z * Z * 2 * R + Z * (2 * R + 3 * 8 + 3 * 4);
     f.total = f.particles + f.cells + f.mesh + sizeof(DevScalars);
@@ -118,7 +118,7 @@ Footprint footprint_of(const bd_params &p) {
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.bsum, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bsum, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -419,6 +419,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     g.kap = p->kap;
     g.zero_mode_g = p->lambda > 0.2 * p->L;
     g.ncinv = 1.0f / (float)g.nc;
+    g.fast_wca = (p->precision == BDK_FP32 && g.nc >= 6) ? 1 : 0;
     g.hinv = 1.0 / g.h;
     {
         // BDK_XCD_REMAP=0 keeps the hardware workgroup order (for comparison)
@@ -445,11 +446,14 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.noise_out, N));
     ALLOC(dalloc(d.rank, N));
     ALLOC(dalloc(d.count, ncell));
+    ALLOC(dalloc(d.arr, ncell));
     ALLOC(dalloc(d.start, ncell + 1));
-    ALLOC(dalloc(d.list, N));
-    ALLOC(dalloc(d.lcell, N));
-    ALLOC(dalloc(d.lrel, N));
-    ALLOC(dalloc(d.lpid, N));
+    // list-ordered arrays carry one padding entry (read ahead by k_wca_f32)
+    ALLOC(dalloc(d.list, N + 1));
+    ALLOC(dalloc(d.lcell, N + 1));
+    ALLOC(dalloc(d.lrel, N + 1));
+    ALLOC(dalloc(d.lpid, N + 1));
+    ALLOC(dalloc(d.lrelf, N + 1));
     ALLOC(dalloc(d.bsum, 4096));
     ALLOC(dalloc_v(d.rho, Z * Z * R));
     ALLOC(dalloc_v(d.ht, (Z / 2) * Z * 2 * R));
@@ -475,6 +479,9 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     std::memset(&sc, 0, sizeof(sc));
     e = hipMemcpy(d.sc, &sc, sizeof(sc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(d.count, 0, ncell * 4);
+    if (e == hipSuccess) e = hipMemset(d.arr, 0, ncell * 4);
+    if (e == hipSuccess) e = hipMemset(d.list, 0, (N + 1) * 4);
+    if (e == hipSuccess) e = hipMemset(d.lrelf, 0, (N + 1) * 8);
     if (e == hipSuccess) e = hipMemset(d.fwca, 0, N * 2 * R);
     if (e == hipSuccess) e = hipMemset(d.fkap, 0, N * 2 * R);
     if (e == hipSuccess) e = hipMemset(d.noise_up, 0, N * 16);
@@ -834,7 +841,8 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
         const size_t N = (size_t)h->g.N, nc2 = (size_t)h->g.ncell;
         Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.pid, N * 4},
                        {nullptr, h->d.rank, N * 4},
-                       {nullptr, h->d.count, nc2 * 4}, {nullptr, h->d.sc, sizeof(DevScalars)}};
+                       {nullptr, h->d.count, nc2 * 4}, {nullptr, h->d.arr, nc2 * 4},
+                       {nullptr, h->d.sc, sizeof(DevScalars)}};
         for (Snap sn : list) {
             HIPCHK(h, hipMalloc(&sn.p, sn.n));
             HIPCHK(h, hipMemcpyAsync(sn.p, sn.src, sn.n, hipMemcpyDeviceToDevice, h->stream));

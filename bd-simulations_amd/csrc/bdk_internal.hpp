@@ -37,6 +37,7 @@ struct Geo {
     int zero_mode_g; // G(0,0) = 0 when lambda > 0.2 L (dm:1389)
     int xcd;         // renumber workgroups per XCD (launch order only)
     float ncinv;     // 1/nc (fast exact division of cell indices)
+    int fast_wca;    // FP32 forces and nc >= 6: k_wca_f32 (reads lrelf)
     double hinv;     // 1/h
     unsigned long long seed;
 };
@@ -63,13 +64,15 @@ struct DevBufs {
     double2 *noise_out = nullptr; // [N] by particle id (bdk_get_noise)
     int *rank = nullptr;      // [N] rank of the slot inside its cell
     // cell list
-    int *count = nullptr;     // [ncell]
+    int *count = nullptr;     // [ncell] particles per cell (kept current across steps)
+    int *arr = nullptr;       // [ncell] arrivals of the current step (zero between steps)
     int *start = nullptr;     // [ncell+1]
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
     // the state in list order (k_list_state): cell, rel and particle id of list entry q
     int *lcell = nullptr;
     double2 *lrel = nullptr;
     int *lpid = nullptr;
+    float2 *lrelf = nullptr;  // FP32 rel - B/2 of list entry q (FP32 WCA)
     int *bsum = nullptr;      // scan block sums
     // mesh
     void *rho = nullptr;      // Z*Z real

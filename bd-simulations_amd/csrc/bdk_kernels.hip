@@ -9,7 +9,9 @@
 //               inverse column FFTs (Fx, Fy);
 //   k_c2r_rows  inverse row FFT of (Fx + i Fy) -> real Fx, Fy mesh;
 //   k_update    gather (gr:81-668), noise (dm:1317-1339), Euler step
-//               (dm:116-157), refresh (dm:216-325), cell counting;
+//               (dm:116-157), refresh (dm:216-325); the cell counts are
+//               updated by the particles that change cell;
+//   k_rank      rank of each particle that stayed in its cell (no atomics);
 //   k_scan_*    exclusive scan of the cell counts;
 //   k_fill      cell list = slots grouped by cell, species in bit 31;
 //   k_sort_cells  each cell's entries in slot order;
@@ -352,20 +354,24 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
 }
 
 // FP32 forces for boxes of at least 6 cells per side: the displacement is
-// the in-cell offset difference plus the cell offset (the minimum image by
-// construction).  The list entry and rel of a candidate are loaded together,
-// one candidate ahead of the arithmetic.  UNIT: all sigma and epsilon ratios
-// are 1, so the per-type factors drop out (same results, fewer operations).
-// Summation order and grouping are those of k_wca.
+// the difference of the in-cell offsets (FP32 copies centred on the cell,
+// lrelf) plus the cell offset, the minimum image by construction.  The
+// list entry and offsets of a candidate are loaded one candidate ahead of
+// the arithmetic (the list arrays carry one padding entry).  UNIT: all sigma
+// and epsilon ratios are 1, so the per-type factors drop out and one sum
+// collects all partners; otherwise same-species and cross-species partners
+// are summed apart and added at the end (the grouping of dm:617-702 +
+// dm:973-1059).  Partners are visited in list order, cell by cell.
 template <bool UNIT>
 __global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
     const int qi = xcd_block(g.xcd) * blockDim.x + threadIdx.x;
     if (qi >= g.N) return;
     const int nc = g.nc;
     const unsigned *list = (const unsigned *)d.list;
+    const float2 *rf = d.lrelf;
     const int c = d.lcell[qi];
     const int cx = div_nc(c, g), cy = c - cx * nc;
-    const double2 ri = d.lrel[qi];
+    const float2 ri = rf[qi];
     const bool s2i = d.lpid[qi] >= g.n1;
     const float B = (float)g.B, cut2 = (float)g.cut2, cap = (float)g.maxpair;
     // factors for a partner of species 1 (A) or species 2 (Bp)
@@ -374,40 +380,40 @@ __global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
     float fsx = 0, fsy = 0, fcx = 0, fcy = 0;
     // candidates q0..qe-1; oy = B below m1, 0 below m2, -B from m2 on
     auto range = [&](int q0, int qe, int m1, int m2, float ox) {
-        if (q0 >= qe) return;
         unsigned ent = list[q0];
-        double2 rj = d.lrel[q0];
+        float2 rj = rf[q0];
         for (int q = q0; q < qe; ++q) {
             const unsigned e = ent;
-            const double2 r = rj;
-            if (q + 1 < qe) {
-                ent = list[q + 1];
-                rj = d.lrel[q + 1];
-            }
+            const float2 r = rj;
+            ent = list[q + 1];  // q + 1 <= N: padding entry
+            rj = rf[q + 1];
             const float oy = q < m1 ? B : (q < m2 ? 0.0f : -B);
-            const float dx = (float)(ri.x - r.x) + ox, dy = (float)(ri.y - r.y) + oy;
+            const float dx = (ri.x - r.x) + ox, dy = (ri.y - r.y) + oy;
             const float r2 = dx * dx + dy * dy;
-            const bool s2j = (e >> 31) != 0;
-            const float sig = UNIT ? 1.0f : (s2j ? sigB : sigA);
-            const float r2s = UNIT ? r2 : r2 * sig * sig;
-            if (r2s <= cut2 && q != qi) {
-                const float inv = __builtin_amdgcn_rcpf(r2s);
-                const float i2 = inv * inv, i4 = i2 * i2;
-                const float a_8 = i4, a_14 = i4 * i2 * inv;
-                float absF;
-                if (UNIT)
-                    absF = 24.0f * (2.0f * a_14 - a_8);
-                else
-                    absF = (s2j ? prefB : prefA) * 24.0f * (2.0f * a_14 - a_8);
-                if (absF > cap) absF = cap;
-                const float fx = UNIT ? absF * dx : absF * dx * sig;
-                const float fy = UNIT ? absF * dy : absF * dy * sig;
-                if (s2i == s2j) {
-                    fsx += fx;
-                    fsy += fy;
-                } else {
-                    fcx += fx;
-                    fcy += fy;
+            if (UNIT) {
+                if (r2 <= cut2 && q != qi) {
+                    const float inv = __builtin_amdgcn_rcpf(r2);
+                    const float i2 = inv * inv, i4 = i2 * i2;
+                    const float absF = fminf(24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
+                    fsx += absF * dx;
+                    fsy += absF * dy;
+                }
+            } else {
+                const bool s2j = (e >> 31) != 0;
+                const float sig = s2j ? sigB : sigA;
+                const float r2s = r2 * sig * sig;
+                if (r2s <= cut2 && q != qi) {
+                    const float inv = __builtin_amdgcn_rcpf(r2s);
+                    const float i2 = inv * inv, i4 = i2 * i2;
+                    const float absF = fminf((s2j ? prefB : prefA) * 24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
+                    const float fx = absF * dx * sig, fy = absF * dy * sig;
+                    if (s2i == s2j) {
+                        fsx += fx;
+                        fsy += fy;
+                    } else {
+                        fcx += fx;
+                        fcy += fy;
+                    }
                 }
             }
         }
@@ -1037,7 +1043,14 @@ __global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
     d.cell[s] = nc_new;
     d.rel[s] = rn;
     d.pid[s] = pid;
-    d.rank[s] = atomicAdd(&d.count[nc_new], 1);
+    if (nc_new != c) {
+        // a cell change (a few percent of the particles per step): the
+        // counts follow, and the rank among the new cell's arrivals comes
+        // from arr; particles that stay are ranked by k_rank without atomics
+        atomicSub(&d.count[c], 1);
+        atomicAdd(&d.count[nc_new], 1);
+        d.rank[s] = atomicAdd(&d.arr[nc_new], 1);
+    }
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
     if (ex > 0.0) atomicMax(&d.sc->exc_bits[(step + 1) & 1], bitsd(ex));
     if (s == 0) d.sc->dt_last = bitsd(dt);
@@ -1171,24 +1184,47 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
     __syncthreads();
     for (int q = threadIdx.x; q < SCAN_BLOCK; q += SCAN_THREADS) {
         int i = base + q;
-        if (i < ncell) {
-            start[i] = a[q];
-            count[i] = 0;
-        }
+        if (i < ncell) start[i] = a[q];
     }
 }
 
-__global__ void k_fill(Geo g, DevBufs d) {
+// Rank of a particle that stayed in its cell during the step: its position
+// among the cell's previous entries, minus the entries before it that left.
+// Runs before the scan overwrites the previous starts.
+__global__ void k_rank(Geo g, DevBufs d) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.N) return;
+    const int c = d.lcell[q];
+    if (d.cell[q] != c) return;  // moved: ranked among the arrivals
+    const int b = d.start[c];
+    int r = q - b;
+    for (int p = b; p < q; ++p) r -= d.cell[p] != d.lcell[p];
+    d.rank[q] = r;
+}
+
+// List entries from the ranks.  After a step (moved_ranks) a cell holds its
+// stayers first, then its arrivals; after a full binning every rank is the
+// position in the cell.
+__global__ void k_fill(Geo g, DevBufs d, int moved_ranks) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
     const unsigned sp2 = d.pid[s] >= g.n1 ? 1u : 0u;
-    ((unsigned *)d.list)[d.start[d.cell[s]] + d.rank[s]] = (unsigned)s | (sp2 << 31);
+    const int c = d.cell[s];
+    int pos = d.start[c] + d.rank[s];
+    if (moved_ranks && c != d.lcell[s]) pos += d.start[c + 1] - d.start[c] - d.arr[c];
+    ((unsigned *)d.list)[pos] = (unsigned)s | (sp2 << 31);
+}
+
+__global__ void k_zero_int(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // each cell's list entries in ascending slot order (fixed summation order)
 __global__ void k_sort_cells(Geo g, DevBufs d) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.ncell) return;
+    d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
     const int b = d.start[c], n = d.start[c + 1] - b;
     if (n < 2) return;
     unsigned *l = (unsigned *)d.list + b;
@@ -1208,9 +1244,11 @@ __global__ void k_list_state(Geo g, DevBufs d) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= g.N) return;
     const int sl = (int)(((const unsigned *)d.list)[q] & LIST_SLOT);
+    const double2 r = d.rel[sl];
     d.lcell[q] = d.cell[sl];
-    d.lrel[q] = d.rel[sl];
+    d.lrel[q] = r;
     d.lpid[q] = d.pid[sl];
+    if (g.fast_wca) d.lrelf[q] = make_float2((float)(r.x - 0.5 * g.B), (float)(r.y - 0.5 * g.B));
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1307,7 +1345,7 @@ hipError_t launch_wca(bdk_t *h) {
     // FP32 forces use the cell-relative displacement when the box holds at
     // least 6 cells per side (the neighbour offset is then the minimum image)
     const dim3 grid(nblk(h->g.N, 256)), blk(256);
-    if (h->p.precision == BDK_FP32 && h->g.nc >= 6) {
+    if (h->g.fast_wca) {
         const bool unit = h->p.sigma11_22 == 1.0 && h->p.sigma11_12 == 1.0 && h->p.eps22_11 == 1.0 &&
                           h->p.eps12_11 == 1.0;
         if (unit)
@@ -1469,7 +1507,7 @@ static void launch_scan_fill_inner(bdk_t *h, int inc_step) {
                        inc_step);
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bsum, h->d.start,
                        ncell);
-    hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
     hipLaunchKernelGGL(k_list_state, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
 }
@@ -1488,6 +1526,7 @@ hipError_t launch_bin(bdk_t *h) {
 
 hipError_t launch_refresh(bdk_t *h) {
     hipLaunchKernelGGL(k_reset_exc, dim3(1), dim3(1), 0, h->stream, h->d.sc);
+    hipLaunchKernelGGL(k_zero_int, dim3(nblk(h->g.ncell, 256)), dim3(256), 0, h->stream, h->d.count, h->g.ncell);
     hipLaunchKernelGGL(k_refresh, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
     launch_scan_fill_inner(h, 0);
     return hipGetLastError();
@@ -1540,6 +1579,8 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         chk(launch_update(h, kap ? 1 : 0));
     }
     mark("k_update");
+    hipLaunchKernelGGL(k_rank, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    mark("k_rank");
     const int ncell = h->g.ncell;
     const int nb = (int)nblk(ncell, SCAN_BLOCK);
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, ncell, h->d.bsum);
@@ -1549,7 +1590,7 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bsum, h->d.start,
                        ncell);
     mark("k_scan_down");
-    hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
     mark("k_sort_cells");
