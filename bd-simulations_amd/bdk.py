@@ -90,6 +90,20 @@ def lib():
             "bdk_init_uniform": (C.c_int, [vp, u64, dbl, vp, vp]),
             "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_batch_create": (C.c_int, [vp, i32, C.POINTER(vp)]),
+            "bdk_batch_destroy": (None, [vp]),
+            "bdk_batch_last_error": (C.c_char_p, [vp]),
+            "bdk_batch_run_steps": (C.c_int, [vp, i64]),
+            "bdk_batch_sync": (C.c_int, [vp]),
+            "bdk_batch_tune": (C.c_int, [vp, i32, vp]),
+            "bdk_batch_profile": (C.c_int, [vp, i32, i32, vp]),
+            "bdk_serve_create": (C.c_int, [i32, u64, i32, i64, C.POINTER(vp)]),
+            "bdk_serve_destroy": (None, [vp]),
+            "bdk_serve_last_error": (C.c_char_p, [vp]),
+            "bdk_serve_submit": (C.c_int, [vp, vp, vp, vp, i64, vp]),
+            "bdk_serve_step": (C.c_int, [vp, vp, vp, vp]),
+            "bdk_serve_result": (C.c_int, [vp, i64, vp, vp, vp, vp]),
+            "bdk_serve_stats": (C.c_int, [vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -107,7 +121,11 @@ def exported_symbols():
             "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_profile_kernel",
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
-            "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos"]
+            "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos",
+            "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
+            "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
+            "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
+            "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
 
 
 def _p(a):
@@ -355,6 +373,109 @@ class RunDynamik:
 
     def write_pos(self, path1, path2, nr, obs_dt):
         self._chk(self.L.bdk_write_pos(self.h, path1.encode(), path2.encode() if path2 else None, nr, obs_dt))
+
+
+class Batch:
+    """Several RunDynamik simulations advanced with one launch per stage
+    (bdk_batch_*).  The batch keeps references to the members; their state
+    stays readable through each member after sync()."""
+
+    STAGES = {"k_wca": 0, "k_deposit": 1, "k_r2c_rows": 2, "k_col": 3, "k_c2r_rows": 4, "step": 7}
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        self.L = lib()
+        arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        h = C.c_void_p()
+        rc = self.L.bdk_batch_create(arr, len(self.sims), C.byref(h))
+        if rc:
+            raise BdkError(rc, self.L.bdk_batch_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc:
+            raise BdkError(rc, self.L.bdk_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bdk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run_steps(self, k):
+        self._chk(self.L.bdk_batch_run_steps(self.h, k))
+
+    def sync(self):
+        self._chk(self.L.bdk_batch_sync(self.h))
+
+    def tune(self, reps=10):
+        out = (C.c_int32 * 3)()
+        self._chk(self.L.bdk_batch_tune(self.h, reps, C.cast(out, C.c_void_p)))
+        return list(out)
+
+    def profile(self, stage, reps=20):
+        ms = C.c_double(0.0)
+        self._chk(self.L.bdk_batch_profile(self.h, self.STAGES[stage], reps, C.byref(ms)))
+        return ms.value
+
+
+class Server:
+    """Job server with continuous batching (bdk_serve_*)."""
+
+    def __init__(self, hbm_budget, max_jobs=256, tick_steps=256, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.bdk_serve_create(device, int(hbm_budget), max_jobs, tick_steps, C.byref(h))
+        if rc:
+            raise BdkError(rc, "bdk_serve_create failed")
+        self.h = h
+        self.sizes = {}
+
+    def _chk(self, rc):
+        if rc:
+            raise BdkError(rc, self.L.bdk_serve_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bdk_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, params, git, rel, steps):
+        git = np.ascontiguousarray(git, dtype=np.int32).reshape(-1, 2)
+        rel = np.ascontiguousarray(rel, dtype=np.float64).reshape(-1, 2)
+        jid = C.c_int64(0)
+        self._chk(self.L.bdk_serve_submit(self.h, C.byref(params), _p(git), _p(rel), steps, C.byref(jid)))
+        self.sizes[jid.value] = len(git)
+        return jid.value
+
+    def step(self):
+        a, q, d = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        self._chk(self.L.bdk_serve_step(self.h, C.byref(a), C.byref(q), C.byref(d)))
+        return a.value, q.value, d.value
+
+    def result(self, jid):
+        n = self.sizes[jid]
+        git = np.zeros((n, 2), np.int32)
+        rel = np.zeros((n, 2))
+        ta, td = C.c_double(0.0), C.c_double(0.0)
+        self._chk(self.L.bdk_serve_result(self.h, jid, _p(git), _p(rel), C.byref(ta), C.byref(td)))
+        return git, rel, ta.value, td.value
+
+    def stats(self):
+        ps, busy, used = C.c_int64(0), C.c_double(0.0), C.c_uint64(0)
+        self._chk(self.L.bdk_serve_stats(self.h, C.byref(ps), C.byref(busy), C.byref(used)))
+        return ps.value, busy.value, used.value
 
 
 def has_gpu():

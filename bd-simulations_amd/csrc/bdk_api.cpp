@@ -310,6 +310,75 @@ double dt_from_bits(unsigned long long b) {
 
 }  // namespace
 
+struct bdk_batch {
+    std::vector<bdk_t *> mem;
+    hipStream_t stream = nullptr;
+    int dev = 0;
+    bdkx::BatchPlan bp;
+    int ept_min[3] = {8, 8, 8};
+    hipGraph_t graph = nullptr, graph_multi = nullptr;
+    hipGraphExec_t gexec = nullptr, gexec_multi = nullptr;
+    std::string err;
+
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+    void drop_graphs() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (gexec_multi) (void)hipGraphExecDestroy(gexec_multi);
+        if (graph_multi) (void)hipGraphDestroy(graph_multi);
+        gexec = gexec_multi = nullptr;
+        graph = graph_multi = nullptr;
+    }
+    void free_tables() {
+        if (bp.reps) (void)hipFree(bp.reps);
+        for (int k = 0; k < bdkx::BS_COUNT; ++k)
+            if (bp.map[k]) (void)hipFree(bp.map[k]);
+        bp = bdkx::BatchPlan();
+    }
+    // device tables of the members for elements-per-thread choice ept
+    hipError_t rebuild(const int ept[3]) {
+        drop_graphs();
+        free_tables();
+        int zmax = 8;
+        for (bdk_t *h : mem) zmax = std::max(zmax, h->g.Z);
+        for (int k = 0; k < 3; ++k) ept_min[k] = bdkx::batch_min_ept(k, zmax);
+        int e3[3];
+        for (int k = 0; k < 3; ++k) e3[k] = std::max(ept[k], ept_min[k]);
+        std::vector<bdkx::RepDev> reps;
+        std::vector<int2> maps[bdkx::BS_COUNT];
+        hipError_t e = bdkx::batch_build(mem, e3, bp, reps, maps);
+        if (e == hipSuccess) e = hipMalloc((void **)&bp.reps, reps.size() * sizeof(bdkx::RepDev));
+        if (e == hipSuccess)
+            e = hipMemcpy(bp.reps, reps.data(), reps.size() * sizeof(bdkx::RepDev), hipMemcpyHostToDevice);
+        for (int k = 0; k < bdkx::BS_COUNT && e == hipSuccess; ++k) {
+            e = hipMalloc((void **)&bp.map[k], std::max<size_t>(1, maps[k].size()) * sizeof(int2));
+            if (e == hipSuccess && !maps[k].empty())
+                e = hipMemcpy(bp.map[k], maps[k].data(), maps[k].size() * sizeof(int2), hipMemcpyHostToDevice);
+        }
+        return e;
+    }
+    hipError_t capture(int nsteps, hipGraph_t *g, hipGraphExec_t *x) {
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = bdkx::batch_enqueue_step(bp, stream);
+        hipGraph_t gg = nullptr;
+        hipError_t e2 = hipStreamEndCapture(stream, &gg);
+        if (e == hipSuccess) e = e2;
+        if (e != hipSuccess) return e;
+        *g = gg;
+        return hipGraphInstantiate(x, gg, nullptr, nullptr, 0);
+    }
+    hipError_t ensure_graphs() {
+        if (gexec && gexec_multi) return hipSuccess;
+        hipError_t e = capture(1, &graph, &gexec);
+        if (e == hipSuccess) e = capture(kGraphSteps, &graph_multi, &gexec_multi);
+        return e;
+    }
+};
+
 extern "C" {
 
 int bdk_params_default(bd_params *p) {
@@ -1131,6 +1200,152 @@ int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, do
         std::fprintf(f, "\n");
         std::fclose(f);
     }
+    return BDK_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {
+    if (!out || !members || n < 1) return fail(nullptr, BDK_ERR_ARG, "bdk_batch_create: need n >= 1 members");
+    *out = nullptr;
+    std::vector<bdk_t *> mem(members, members + n);
+    int zmax = 8;
+    for (int i = 0; i < n; ++i) {
+        bdk_t *h = mem[i];
+        if (!h) return fail(nullptr, BDK_ERR_ARG, "bdk_batch_create: null member");
+        const std::string who = "member " + std::to_string(i) + ": ";
+        if (h->dev != mem[0]->dev) return fail(nullptr, BDK_ERR_ARG, who + "members must share the device");
+        if (h->p.precision != mem[0]->p.precision)
+            return fail(nullptr, BDK_ERR_ARG, who + "members must share the precision");
+        if (h->g.adaptive) return fail(nullptr, BDK_ERR_ARG, who + "batches need a fixed dt (max_travel = +inf)");
+        if (h->g.idgas) return fail(nullptr, BDK_ERR_ARG, who + "batches need capillary forces (idgas_only = 0)");
+        if (h->p.noise_mode == BDK_NOISE_UPLOAD)
+            return fail(nullptr, BDK_ERR_ARG, who + "batches do not take uploaded noise");
+        for (int j = 0; j < i; ++j)
+            if (mem[j] == h) return fail(nullptr, BDK_ERR_ARG, who + "a handle appears twice");
+        int rc = ensure(h);
+        if (rc) return fail(nullptr, rc, who + h->err);
+        if (hipStreamSynchronize(h->stream) != hipSuccess)
+            return fail(nullptr, BDK_ERR_HIP, who + "stream synchronisation failed");
+        zmax = std::max(zmax, h->g.Z);
+    }
+    bdk_batch_t *b = new bdk_batch();
+    b->mem = mem;
+    b->dev = mem[0]->dev;
+    int rc = BDK_OK;
+    hipError_t e = hipSetDevice(b->dev);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    int ept[3];
+    for (int k = 0; k < 3; ++k) ept[k] = bdkx::batch_min_ept(k, zmax);
+    if (e == hipSuccess) e = b->rebuild(ept);
+    if (e != hipSuccess) {
+        rc = fail(nullptr, BDK_ERR_HIP, std::string("bdk_batch_create: ") + hipGetErrorString(e));
+        bdk_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return BDK_OK;
+}
+
+void bdk_batch_destroy(bdk_batch_t *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->dev);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->drop_graphs();
+    b->free_tables();
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *bdk_batch_last_error(const bdk_batch_t *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int bdk_batch_run_steps(bdk_batch_t *b, int64_t k) {
+    if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
+    if (k < 0) return b->fail(BDK_ERR_ARG, "k must be >= 0");
+    hipError_t e = hipSetDevice(b->dev);
+    if (e == hipSuccess) e = b->ensure_graphs();
+    int64_t i = 0;
+    for (; e == hipSuccess && i + kGraphSteps <= k; i += kGraphSteps) e = hipGraphLaunch(b->gexec_multi, b->stream);
+    for (; e == hipSuccess && i < k; ++i) e = hipGraphLaunch(b->gexec, b->stream);
+    if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_run_steps: ") + hipGetErrorString(e));
+    for (bdk_t *h : b->mem) h->host_step += k;
+    return BDK_OK;
+}
+
+int bdk_batch_sync(bdk_batch_t *b) {
+    if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_sync: ") + hipGetErrorString(e));
+    for (size_t i = 0; i < b->mem.size(); ++i) {
+        DevScalars sc;
+        e = hipMemcpy(&sc, b->mem[i]->d.sc, sizeof(sc), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_sync: ") + hipGetErrorString(e));
+        if (sc.nonfinite)
+            return b->fail(BDK_ERR_NONFINITE, "member " + std::to_string(i) +
+                                                  ": a particle position became non-finite or ran away");
+    }
+    return BDK_OK;
+}
+
+int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_per_launch) {
+    if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
+    // the update and the list rebuild only keep the state consistent as parts
+    // of a whole step, so they are not timed alone
+    if (stage < 0 || stage > 7 || stage == 5 || stage == 6 || reps < 1 || !ms_per_launch)
+        return b->fail(BDK_ERR_ARG, "bad profile request (stages 0-4 and 7)");
+    hipError_t e = hipSetDevice(b->dev);
+    if (e == hipSuccess) e = b->ensure_graphs();
+    auto launch = [&]() -> hipError_t {
+        if (stage == 7) return hipGraphLaunch(b->gexec, b->stream);
+        return bdkx::batch_stage(b->bp, stage, b->stream);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+    for (int i = 0; e == hipSuccess && i < reps; ++i) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_profile: ") + hipGetErrorString(e));
+    // whole steps advance the members (the warm-up launch included)
+    if (stage == 7)
+        for (bdk_t *h : b->mem) h->host_step += reps + 1;
+    *ms_per_launch = ms / reps;
+    return BDK_OK;
+}
+
+int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *ept_out) {
+    if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
+    if (reps < 1) return b->fail(BDK_ERR_ARG, "reps must be >= 1");
+    hipError_t e = hipSetDevice(b->dev);
+    int best[3];
+    for (int k = 0; k < 3; ++k) best[k] = b->bp.ept[k];
+    for (int stage = 0; stage < 3 && e == hipSuccess; ++stage) {
+        double tbest = 1e30;
+        for (int cand : {8, 16, 32}) {
+            if (cand < b->ept_min[stage]) continue;
+            int ept[3] = {best[0], best[1], best[2]};
+            ept[stage] = cand;
+            e = b->rebuild(ept);
+            if (e != hipSuccess) break;
+            double ms = 0.0;
+            int rc = bdk_batch_profile(b, 2 + stage, reps, &ms);
+            if (rc) return rc;
+            if (ms < tbest) {
+                tbest = ms;
+                best[stage] = cand;
+            }
+        }
+    }
+    if (e == hipSuccess) e = b->rebuild(best);
+    if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_tune: ") + hipGetErrorString(e));
+    if (ept_out)
+        for (int k = 0; k < 3; ++k) ept_out[k] = best[k];
     return BDK_OK;
 }
 

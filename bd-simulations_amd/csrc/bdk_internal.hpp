@@ -86,7 +86,39 @@ struct DevBufs {
     DevScalars *sc = nullptr;
 };
 
-struct Launch;  // kernel sequence bookkeeping for profiling
+// Deposit launch arguments (one tile per block).
+struct DepArgs {
+    int tc;        // list cells per tile side
+    int seg_cap;   // staged list segments (rows, two per row when y wraps)
+    int win_cap;   // owned mesh points per tile (capacity)
+    double scale_hi, scale_lo;      // 2^S1, 2^S2
+    double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
+    int reset;     // re-arm the per-step scalars (block 0)
+};
+
+// One member of a batch, as the batched kernels read it (device memory).
+struct RepDev {
+    Geo g;
+    DevBufs d;
+    DepArgs dep;
+    int dep_tx;                      // deposit tiles per row
+    int rb, rc, rb2, sR1, sC, sR2;   // FFT rows per block and LDS row strides
+    int scan_nb;                     // scan blocks
+    int unit;                        // all sigma/epsilon ratios 1
+};
+
+// Block maps of the batched stages: blockIdx.x -> (member, block of member).
+enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_COUNT };
+
+struct BatchPlan {
+    int nmem = 0;
+    int f32 = 1;
+    RepDev *reps = nullptr;          // device [nmem]
+    int2 *map[BS_COUNT] = {};        // device maps
+    int nblk[BS_COUNT] = {};
+    int ept[3] = {8, 8, 8};          // FFT elements per thread (r2c, col, c2r)
+    size_t dep_lds = 0, l1 = 0, lc = 0, l2 = 0;
+};
 
 }  // namespace bdkx
 
@@ -126,6 +158,13 @@ hipError_t launch_refresh(bdk_t *h);              // refresh only (no motion)
 hipError_t launch_noise_out(bdk_t *h);            // noise of the next step, by pid
 hipError_t set_kernel_attributes();
 hipError_t build_tables(bdk_t *h);                // spectral tables in device memory
+
+// batches (a batch is a view over member handles, each with its own buffers)
+int batch_min_ept(int stage, int zmax);
+hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
+                       std::vector<int2> maps[BS_COUNT]);
+hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st);
+hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st);
 
 // one full step as a sequence of launches; names (if non-null) get the
 // kernel labels in launch order

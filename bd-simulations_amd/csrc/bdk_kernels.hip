@@ -306,9 +306,9 @@ __device__ int block_exscan4(int *a, int n, int *wsum) {
 constexpr unsigned LIST_SLOT = 0x7fffffffu;
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
+__device__ __forceinline__ void wca_ref_body(const Geo &g, const DevBufs &d, int blk) {
     using R2 = typename V2<R>::T;
-    const int qi = xcd_block(g.xcd) * blockDim.x + threadIdx.x;
+    const int qi = blk * blockDim.x + threadIdx.x;
     if (qi >= g.N) return;
     const int nc = g.nc;
     const unsigned *list = (const unsigned *)d.list;
@@ -353,6 +353,11 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
     ((R2 *)d.fwca)[qi] = o;
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
+    wca_ref_body<R>(g, d, xcd_block(g.xcd));
+}
+
 // FP32 forces for boxes of at least 6 cells per side: the displacement is
 // the difference of the in-cell offsets (FP32 copies centred on the cell,
 // lrelf) plus the cell offset, the minimum image by construction.  The
@@ -363,8 +368,8 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
 // are summed apart and added at the end (the grouping of dm:617-702 +
 // dm:973-1059).  Partners are visited in list order, cell by cell.
 template <bool UNIT>
-__global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
-    const int qi = xcd_block(g.xcd) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int blk) {
+    const int qi = blk * blockDim.x + threadIdx.x;
     if (qi >= g.N) return;
     const int nc = g.nc;
     const unsigned *list = (const unsigned *)d.list;
@@ -440,6 +445,11 @@ __global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
     ((float2 *)d.fwca)[qi] = o;
 }
 
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
+    wca_f32_body<UNIT>(g, d, xcd_block(g.xcd));
+}
+
 // ------------------------------------------------------------------ deposit
 // Charge assignment (gr:31-547) per tile of tc x tc list cells.  A mesh
 // point belongs to the tile whose cells contain its centre,
@@ -463,15 +473,6 @@ __device__ int first_owned(int c, const Geo &g) {
     while (k < g.Z && owner(k) < c) ++k;
     return k;
 }
-
-struct DepArgs {
-    int tc;        // list cells per tile side
-    int seg_cap;   // staged list segments (rows, two per row when y wraps)
-    int win_cap;   // owned mesh points per tile (capacity)
-    double scale_hi, scale_lo;      // 2^S1, 2^S2
-    double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
-    int reset;     // re-arm the per-step scalars (block 0)
-};
 
 template <typename R, int SCHEME>
 __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int q,
@@ -526,8 +527,9 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
     }
 }
 
+// lb: tile index, tiles numbered x fastest over tx tiles per row
 template <typename R, int SCHEME>
-__global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArgs a) {
+__device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, const DepArgs &a, int lb, int tx) {
     extern __shared__ unsigned char smem_raw[];
     unsigned long long *acc_hi = (unsigned long long *)smem_raw;
     unsigned long long *acc_lo = acc_hi + a.win_cap;
@@ -538,8 +540,7 @@ __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArg
     __shared__ int wsum[16];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int nc = g.nc, tc = a.tc;
-    const int lb = xcd_block(0);  // hardware order: measured faster than per-XCD ranges for this kernel
-    const int cx0 = (lb % gridDim.x) * tc, cy0 = (lb / gridDim.x) * tc;
+    const int cx0 = (lb % tx) * tc, cy0 = (lb / tx) * tc;
     const int nx = min(tc, nc - cx0), ny = min(tc, nc - cy0);
     const unsigned long long step = d.sc->step;
     const double e = dbits(d.sc->exc_bits[step & 1]) + 1e-9 * g.L;
@@ -629,6 +630,12 @@ __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArg
     }
 }
 
+// hardware block order: measured faster than per-XCD ranges for this kernel
+template <typename R, int SCHEME>
+__global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArgs a) {
+    deposit_body<R, SCHEME>(g, d, a, blockIdx.x + blockIdx.y * gridDim.x, gridDim.x);
+}
+
 // per-step scalar re-arm when no deposit runs (ideal-gas runs)
 __global__ void k_rearm(Geo g, DevScalars *sc) {
     const unsigned long long step = sc->step;
@@ -714,15 +721,15 @@ constexpr int FFT_THREADS = 256;
 // with X[0] packing (DC, Nyquist) (both real), written transposed:
 // ht[m*Z + k].  Length-Z/2 complex FFT of (x[2n], x[2n+1]) + split.
 template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
-                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
-                                                          int RB, int stride, int remap) {
+__device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
+                                              const typename V2<R>::T *__restrict__ tw, int Z, int logZ, int RB,
+                                              int stride, int blk) {
     using C = typename V2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = Z >> 1, logn = logZ - 1;
-    const int k0 = xcd_block(remap) * RB;
+    const int k0 = blk * RB;
     for (int idx = tid; idx < RB * n; idx += nt) {
         const int row = idx >> logn, m = idx & (n - 1);
         s[row * stride + m] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
@@ -752,6 +759,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ 
     }
 }
 
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
+                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
+                                                          int RB, int stride, int remap) {
+    r2c_rows_body<R, EPT>(rho, ht, tw, Z, logZ, RB, stride, xcd_block(remap));
+}
+
 // spectral factors of one (j, m') pair: gx = sq[j] G / Z / Z, gy = sq[m'] G / Z / Z,
 // G = 1/((lap[j] + lap[m']) + 1/lambda^2) (dm:585-593, dm:1384-1394)
 template <typename R>
@@ -770,10 +784,9 @@ __device__ __forceinline__ void symbols(int j, int mm, const Geo &g, const R *sq
 // Column pass: rows m0..m0+RC-1 of ht (length Z along x), forward FFT,
 // multiply by i*gx and i*gy, inverse FFT of both, out gt[m*Z+k] = (Fx, Fy).
 template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__restrict__ ht,
-                                                     typename V4<R>::T *__restrict__ gt,
-                                                     const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
-                                                     const R *__restrict__ lap, Geo g, int RC, int stride) {
+__device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ ht, typename V4<R>::T *__restrict__ gt,
+                                         const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
+                                         const R *__restrict__ lap, const Geo &g, int RC, int stride, int blk) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     extern __shared__ unsigned char smem_raw[];
@@ -781,7 +794,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__
     C *sB = s + RC * stride;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int Z = g.Z, logZ = g.logZ, Zh = Z >> 1;
-    const int m0 = blockIdx.x * RC;
+    const int m0 = blk * RC;
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = idx >> logZ, k = idx & (Z - 1);
         s[row * stride + k] = ht[(size_t)(m0 + row) * Z + k];
@@ -836,21 +849,29 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__
     }
 }
 
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__restrict__ ht,
+                                                     typename V4<R>::T *__restrict__ gt,
+                                                     const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
+                                                     const R *__restrict__ lap, Geo g, int RC, int stride) {
+    col_body<R, EPT>(ht, gt, tw, sq, lap, g, RC, stride, blockIdx.x);
+}
+
 // Row c2r of the two real signals Fx, Fy: rows k0..k0+RB2-1, spectrum
 // C[m] = X[m] + i Y[m] completed by Hermitian symmetry, inverse FFT of
 // length Z, out fmesh[k*Z + l] = (Re, Im) = (Fx, Fy).
 template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
-                                                          typename V2<R>::T *__restrict__ fm,
-                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
-                                                          int RB2, int stride, int remap) {
+__device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restrict__ gt,
+                                              typename V2<R>::T *__restrict__ fm,
+                                              const typename V2<R>::T *__restrict__ tw, int Z, int logZ, int RB2,
+                                              int stride, int blk) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int Zh = Z >> 1;
-    const int k0 = xcd_block(remap) * RB2;
+    const int k0 = blk * RB2;
     for (int idx = tid; idx < RB2 * Zh; idx += nt) {
         const int row = idx % RB2, m = idx / RB2;
         C4 v = gt[(size_t)m * Z + k0 + row];
@@ -869,6 +890,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::
         const int row = idx >> logZ, l = idx & (Z - 1);
         fm[(size_t)(k0 + row) * Z + l] = s[row * stride + l];
     }
+}
+
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
+                                                          typename V2<R>::T *__restrict__ fm,
+                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
+                                                          int RB2, int stride, int remap) {
+    c2r_rows_body<R, EPT>(gt, fm, tw, Z, logZ, RB2, stride, xcd_block(remap));
 }
 
 // ------------------------------------------------------------------ gather, update
@@ -981,9 +1010,9 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
 // Euler step (dm:116-157) + refresh (dm:216-325) + cell counting.
 // FUSED: capillary force gathered here (fixed dt); else read from fkap.
 template <typename R, int SCHEME, bool FUSED>
-__global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
+__device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int blk) {
     using R2 = typename V2<R>::T;
-    const int s = xcd_block(g.xcd) * blockDim.x + threadIdx.x;  // list position in, slot out
+    const int s = blk * blockDim.x + threadIdx.x;  // list position in, slot out
     if (s >= g.N) return;
     const unsigned long long step = d.sc->step;
     const int c = d.lcell[s];
@@ -1056,6 +1085,11 @@ __global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
     if (s == 0) d.sc->dt_last = bitsd(dt);
 }
 
+template <typename R, int SCHEME, bool FUSED>
+__global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
+    update_body<R, SCHEME, FUSED>(g, d, xcd_block(g.xcd));
+}
+
 // refresh (dm:216-325) without motion + counting; used by bdk_refresh
 __global__ void k_refresh(Geo g, DevBufs d) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1110,10 +1144,10 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_PER_THREAD;  // 4096 cells per block
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int ncell,
-                                                              int *__restrict__ bsum) {
+__device__ __forceinline__ void scan_reduce_body(const int *__restrict__ count, int ncell, int *__restrict__ bsum,
+                                                 int blk) {
     __shared__ int ws[16];
-    const int base = blockIdx.x * SCAN_BLOCK;
+    const int base = blk * SCAN_BLOCK;
     int s = 0;
     for (int q = threadIdx.x; q < SCAN_BLOCK; q += SCAN_THREADS) {
         int i = base + q;
@@ -1126,14 +1160,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restr
     if (threadIdx.x == 0) {
         int t = 0;
         for (int w = 0; w < SCAN_THREADS / 64; ++w) t += ws[w];
-        bsum[blockIdx.x] = t;
+        bsum[blk] = t;
     }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int ncell,
+                                                              int *__restrict__ bsum) {
+    scan_reduce_body(count, ncell, bsum, blockIdx.x);
 }
 
 // exclusive scan of the block sums (one block of 1024 threads, nb <= 4096);
 // also closes the step: start[ncell] = total, step counter += inc_step
-__global__ __launch_bounds__(1024) void k_scan_top(int *bsum, int nb, int *start, int ncell, DevScalars *sc,
-                                                   int inc_step) {
+__device__ __forceinline__ void scan_top_body(int *bsum, int nb, int *start, int ncell, DevScalars *sc,
+                                              int inc_step) {
     __shared__ int a[4096];
     __shared__ int ws[16];
     for (int i = threadIdx.x; i < nb; i += blockDim.x) a[i] = bsum[i];
@@ -1146,11 +1185,16 @@ __global__ __launch_bounds__(1024) void k_scan_top(int *bsum, int nb, int *start
     }
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ count, const int *__restrict__ bsum,
-                                                            int *__restrict__ start, int ncell) {
+__global__ __launch_bounds__(1024) void k_scan_top(int *bsum, int nb, int *start, int ncell, DevScalars *sc,
+                                                   int inc_step) {
+    scan_top_body(bsum, nb, start, ncell, sc, inc_step);
+}
+
+__device__ __forceinline__ void scan_down_body(const int *__restrict__ count, const int *__restrict__ bsum,
+                                               int *__restrict__ start, int ncell, int blk) {
     __shared__ int a[SCAN_BLOCK];
     __shared__ int ws[16];
-    const int base = blockIdx.x * SCAN_BLOCK;
+    const int base = blk * SCAN_BLOCK;
     for (int q = threadIdx.x; q < SCAN_BLOCK; q += SCAN_THREADS) {
         int i = base + q;
         a[q] = i < ncell ? count[i] : 0;
@@ -1173,7 +1217,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
     }
     if (lane == 63) ws[w] = x;
     __syncthreads();
-    int woff = bsum[blockIdx.x];
+    int woff = bsum[blk];
     for (int q = 0; q < w; ++q) woff += ws[q];
     int e = woff + x - s;
 #pragma unroll
@@ -1188,11 +1232,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(int *__restrict__ co
     }
 }
 
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int *__restrict__ count, const int *__restrict__ bsum,
+                                                            int *__restrict__ start, int ncell) {
+    scan_down_body(count, bsum, start, ncell, blockIdx.x);
+}
+
 // Rank of a particle that stayed in its cell during the step: its position
 // among the cell's previous entries, minus the entries before it that left.
 // Runs before the scan overwrites the previous starts.
-__global__ void k_rank(Geo g, DevBufs d) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rank_body(const Geo &g, const DevBufs &d, int blk) {
+    const int q = blk * blockDim.x + threadIdx.x;
     if (q >= g.N) return;
     const int c = d.lcell[q];
     if (d.cell[q] != c) return;  // moved: ranked among the arrivals
@@ -1202,11 +1251,13 @@ __global__ void k_rank(Geo g, DevBufs d) {
     d.rank[q] = r;
 }
 
+__global__ void k_rank(Geo g, DevBufs d) { rank_body(g, d, blockIdx.x); }
+
 // List entries from the ranks.  After a step (moved_ranks) a cell holds its
 // stayers first, then its arrivals; after a full binning every rank is the
 // position in the cell.
-__global__ void k_fill(Geo g, DevBufs d, int moved_ranks) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fill_body(const Geo &g, const DevBufs &d, int moved_ranks, int blk) {
+    const int s = blk * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
     const unsigned sp2 = d.pid[s] >= g.n1 ? 1u : 0u;
     const int c = d.cell[s];
@@ -1215,14 +1266,16 @@ __global__ void k_fill(Geo g, DevBufs d, int moved_ranks) {
     ((unsigned *)d.list)[pos] = (unsigned)s | (sp2 << 31);
 }
 
+__global__ void k_fill(Geo g, DevBufs d, int moved_ranks) { fill_body(g, d, moved_ranks, blockIdx.x); }
+
 __global__ void k_zero_int(int *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
 // each cell's list entries in ascending slot order (fixed summation order)
-__global__ void k_sort_cells(Geo g, DevBufs d) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, int blk) {
+    const int c = blk * blockDim.x + threadIdx.x;
     if (c >= g.ncell) return;
     d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
     const int b = d.start[c], n = d.start[c + 1] - b;
@@ -1239,9 +1292,11 @@ __global__ void k_sort_cells(Geo g, DevBufs d) {
     }
 }
 
+__global__ void k_sort_cells(Geo g, DevBufs d) { sort_cells_body(g, d, blockIdx.x); }
+
 // the particle state in list order, read by the force and update kernels
-__global__ void k_list_state(Geo g, DevBufs d) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void list_state_body(const Geo &g, const DevBufs &d, int blk) {
+    const int q = blk * blockDim.x + threadIdx.x;
     if (q >= g.N) return;
     const int sl = (int)(((const unsigned *)d.list)[q] & LIST_SLOT);
     const double2 r = d.rel[sl];
@@ -1249,6 +1304,117 @@ __global__ void k_list_state(Geo g, DevBufs d) {
     d.lrel[q] = r;
     d.lpid[q] = d.pid[sl];
     if (g.fast_wca) d.lrelf[q] = make_float2((float)(r.x - 0.5 * g.B), (float)(r.y - 0.5 * g.B));
+}
+
+__global__ void k_list_state(Geo g, DevBufs d) { list_state_body(g, d, blockIdx.x); }
+
+// ------------------------------------------------------------------ batches
+// A batch advances many independent simulations (members) with one launch
+// per stage.  A batched kernel maps its block through the stage's table to
+// (member, block of that member) and runs the same device body as the
+// single-simulation kernel on that member's geometry and buffers, so every
+// member's results are bit-identical alone and in a batch.
+
+template <typename R>
+__global__ __launch_bounds__(256) void kb_wca(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    if (std::is_same<R, float>::value && r.g.fast_wca) {
+        if (r.unit)
+            wca_f32_body<true>(r.g, r.d, t.y);
+        else
+            wca_f32_body<false>(r.g, r.d, t.y);
+    } else {
+        wca_ref_body<R>(r.g, r.d, t.y);
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    switch (r.g.scheme) {
+    case 0: deposit_body<R, 0>(r.g, r.d, r.dep, t.y, r.dep_tx); break;
+    case 1: deposit_body<R, 1>(r.g, r.d, r.dep, t.y, r.dep_tx); break;
+    default: deposit_body<R, 2>(r.g, r.d, r.dep, t.y, r.dep_tx); break;
+    }
+}
+
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int2 *map) {
+    using C = typename V2<R>::T;
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    r2c_rows_body<R, EPT>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb, r.sR1, t.y);
+}
+
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int2 *map) {
+    using C = typename V2<R>::T;
+    using C4 = typename V4<R>::T;
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    const R *sq = std::is_same<R, float>::value ? (const R *)r.d.sqf : (const R *)r.d.sq;
+    const R *lap = std::is_same<R, float>::value ? (const R *)r.d.lapf : (const R *)r.d.lap;
+    col_body<R, EPT>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
+}
+
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map) {
+    using C = typename V2<R>::T;
+    using C4 = typename V4<R>::T;
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb2, r.sR2, t.y);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    switch (r.g.scheme) {
+    case 0: update_body<R, 0, true>(r.g, r.d, t.y); break;
+    case 1: update_body<R, 1, true>(r.g, r.d, t.y); break;
+    default: update_body<R, 2, true>(r.g, r.d, t.y); break;
+    }
+}
+
+__global__ void kb_rank(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    rank_body(m[t.x].g, m[t.x].d, t.y);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    scan_reduce_body(r.d.count, r.g.ncell, r.d.bsum, t.y);
+}
+
+// one block per member
+__global__ __launch_bounds__(1024) void kb_scan_top(const RepDev *m) {
+    const RepDev &r = m[blockIdx.x];
+    scan_top_body(r.d.bsum, r.scan_nb, r.d.start, r.g.ncell, r.d.sc, 1);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    scan_down_body(r.d.count, r.d.bsum, r.d.start, r.g.ncell, t.y);
+}
+
+__global__ void kb_fill(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    fill_body(m[t.x].g, m[t.x].d, 1, t.y);
+}
+
+__global__ void kb_sort_cells(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    sort_cells_body(m[t.x].g, m[t.x].d, t.y);
+}
+
+__global__ void kb_list_state(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    list_state_body(m[t.x].g, m[t.x].d, t.y);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1327,6 +1493,17 @@ hipError_t set_kernel_attributes() {
     BDK_ATTR_E(16)
     BDK_ATTR_E(32)
 #undef BDK_ATTR_E
+#define BDK_ATTR_B(E)            \
+    BDK_ATTR((kb_r2c<float, E>))  \
+    BDK_ATTR((kb_r2c<double, E>)) \
+    BDK_ATTR((kb_col<float, E>))  \
+    BDK_ATTR((kb_col<double, E>)) \
+    BDK_ATTR((kb_c2r<float, E>))  \
+    BDK_ATTR((kb_c2r<double, E>))
+    BDK_ATTR_B(8)
+    BDK_ATTR_B(16)
+    BDK_ATTR_B(32)
+#undef BDK_ATTR_B
 #undef BDK_ATTR
     // the deposit also has a little static LDS: ask for 128 KB of dynamic LDS
 #define BDK_ATTR_D(K)                                                                          \
@@ -1337,6 +1514,8 @@ hipError_t set_kernel_attributes() {
     BDK_ATTR_D((k_deposit<double, 0>))
     BDK_ATTR_D((k_deposit<double, 1>))
     BDK_ATTR_D((k_deposit<double, 2>))
+    BDK_ATTR_D((kb_deposit<float>))
+    BDK_ATTR_D((kb_deposit<double>))
 #undef BDK_ATTR_D
     return e;
 }
@@ -1598,6 +1777,124 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     mark("k_list_state");
     chk(hipGetLastError());
     return e;
+}
+
+// ------------------------------------------------------------------ batch plans
+// FFT rows per block for elements-per-thread (e1, e2, e3) at mesh size Z.
+static void fft_rows(int Z, const int e[3], int &rb, int &rc, int &rb2) {
+    rb = std::min(Z, e[0] * FFT_THREADS / (Z / 2));
+    rc = std::max(1, std::min(Z / 2, e[1] * FFT_THREADS / (2 * Z)));
+    rb2 = std::min(Z, e[2] * FFT_THREADS / Z);
+}
+
+int batch_min_ept(int stage, int zmax) {
+    const int need = stage == 0 ? zmax / 2 : (stage == 1 ? 2 * zmax : zmax);
+    return ept_for(8, need);
+}
+
+hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
+                       std::vector<int2> maps[BS_COUNT]) {
+    const int n = (int)mem.size();
+    reps.assign(n, RepDev());
+    for (int k = 0; k < BS_COUNT; ++k) maps[k].clear();
+    bp.nmem = n;
+    bp.f32 = mem.empty() ? 1 : (mem[0]->p.precision == BDK_FP32);
+    for (int k = 0; k < 3; ++k) bp.ept[k] = ept[k];
+    bp.dep_lds = bp.l1 = bp.lc = bp.l2 = 0;
+    const size_t csz = bp.f32 ? 8 : 16;
+    for (int i = 0; i < n; ++i) {
+        const bdk_t *h = mem[i];
+        RepDev &r = reps[i];
+        r.g = h->g;
+        r.d = h->d;
+        const DepPlan pl = dep_plan(h, 1);
+        r.dep = pl.a;
+        r.dep_tx = (int)pl.grid.x;
+        bp.dep_lds = std::max(bp.dep_lds, pl.lds);
+        const int Z = h->g.Z;
+        fft_rows(Z, ept, r.rb, r.rc, r.rb2);
+        r.sR1 = Z / 2 + 1;
+        r.sC = Z + 1;
+        r.sR2 = Z + 1;
+        bp.l1 = std::max(bp.l1, (size_t)r.rb * r.sR1 * csz);
+        bp.lc = std::max(bp.lc, (size_t)2 * r.rc * r.sC * csz);
+        bp.l2 = std::max(bp.l2, (size_t)r.rb2 * r.sR2 * csz);
+        r.scan_nb = (int)nblk(h->g.ncell, SCAN_BLOCK);
+        r.unit = h->p.sigma11_22 == 1.0 && h->p.sigma11_12 == 1.0 && h->p.eps22_11 == 1.0 && h->p.eps12_11 == 1.0;
+        const int nb[BS_COUNT] = {(int)nblk(h->g.N, 256), (int)nblk(h->g.ncell, 256), r.scan_nb,
+                                  (int)(pl.grid.x * pl.grid.y), Z / r.rb, (Z / 2) / r.rc, Z / r.rb2};
+        for (int k = 0; k < BS_COUNT; ++k)
+            for (int b = 0; b < nb[k]; ++b) maps[k].push_back(make_int2(i, b));
+    }
+    for (int k = 0; k < BS_COUNT; ++k) bp.nblk[k] = (int)maps[k].size();
+    return hipSuccess;
+}
+
+#define BDK_EPT_SWITCH2(E, CALL) \
+    switch (E) {                 \
+    case 8: CALL(8); break;      \
+    case 16: CALL(16); break;    \
+    default: CALL(32); break;    \
+    }
+
+template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, hipStream_t st) {
+    const RepDev *m = bp.reps;
+    switch (stage) {
+    case 0:
+        hipLaunchKernelGGL(kb_wca<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        break;
+    case 1:
+        hipLaunchKernelGGL(kb_deposit<R>, dim3(bp.nblk[BS_DEP]), dim3(DP_THREADS), bp.dep_lds, st, m, bp.map[BS_DEP]);
+        break;
+    case 2: {
+#define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E>), dim3(bp.nblk[BS_R2C]), dim3(FFT_THREADS), bp.l1, st, m, bp.map[BS_R2C])
+        BDK_EPT_SWITCH2(bp.ept[0], R2C)
+#undef R2C
+        break;
+    }
+    case 3: {
+#define COL(E) hipLaunchKernelGGL((kb_col<R, E>), dim3(bp.nblk[BS_COL]), dim3(FFT_THREADS), bp.lc, st, m, bp.map[BS_COL])
+        BDK_EPT_SWITCH2(bp.ept[1], COL)
+#undef COL
+        break;
+    }
+    case 4: {
+#define C2R(E) hipLaunchKernelGGL((kb_c2r<R, E>), dim3(bp.nblk[BS_C2R]), dim3(FFT_THREADS), bp.l2, st, m, bp.map[BS_C2R])
+        BDK_EPT_SWITCH2(bp.ept[2], C2R)
+#undef C2R
+        break;
+    }
+    case 5:
+        hipLaunchKernelGGL(kb_update<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        break;
+    default: {
+        hipLaunchKernelGGL(kb_rank, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        hipLaunchKernelGGL(kb_scan_reduce, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
+        hipLaunchKernelGGL(kb_scan_top, dim3(bp.nmem), dim3(1024), 0, st, m);
+        hipLaunchKernelGGL(kb_scan_down, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
+        hipLaunchKernelGGL(kb_fill, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        hipLaunchKernelGGL(kb_sort_cells, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
+        hipLaunchKernelGGL(kb_list_state, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        break;
+    }
+    }
+}
+
+// stage 0 WCA, 1 deposit, 2 r2c, 3 col, 4 c2r, 5 update, 6 cell-list rebuild
+hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
+    if (bp.f32)
+        batch_stage_t<float>(bp, stage, st);
+    else
+        batch_stage_t<double>(bp, stage, st);
+    return hipGetLastError();
+}
+
+hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st) {
+    for (int k = 0; k <= 6; ++k) {
+        hipError_t e = batch_stage(bp, k, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace bdkx

@@ -199,6 +199,62 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
    header on nr == 0 with truncate, then one block per checkpoint. */
 int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
 
+/* ---------------------------------------------------------------- batches
+   Many independent simulations (the reference's `runs` and parameter sweeps
+   over N, L, lambda and scheme) advanced together with one kernel launch per
+   stage of the step.  No counterpart in the reference, which runs one
+   simulation at a time (main.cpp:129); a batch replaces a loop of
+   zeitschritt calls over independent RunZustand objects.
+
+   A batch is a view over member handles made with bdk_create and given
+   positions; it does not own them.  Members share the device and the
+   precision, use a fixed dt (max_travel = +inf), have capillary forces on
+   (idgas_only = 0) and do not use BDK_NOISE_UPLOAD.  Each member's state
+   stays in its own buffers: after bdk_batch_sync the ordinary per-handle
+   calls (bdk_get_positions, bdk_write_pos, ...) read it, and each member's
+   trajectory is bit-identical to running that handle alone.  A member must
+   not be stepped through its own handle while it belongs to a live batch. */
+typedef struct bdk_batch bdk_batch_t;
+
+int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out);
+void bdk_batch_destroy(bdk_batch_t *b);
+const char *bdk_batch_last_error(const bdk_batch_t *b);
+/* Enqueue k steps of every member (graph replay); returns before they end. */
+int bdk_batch_run_steps(bdk_batch_t *b, int64_t k);
+/* Wait for the batch; BDK_ERR_NONFINITE names the first member that ran away. */
+int bdk_batch_sync(bdk_batch_t *b);
+/* Project autotuner: times the FFT stages for elements per thread 8/16/32
+   (reps launches each) and keeps the fastest; ept_out[3] receives the choice. */
+int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *ept_out);
+/* ms per launch of one stage (0 WCA, 1 deposit, 2 r2c, 3 col, 4 c2r, 7 whole
+   step) over reps launches after one warm-up launch.  Stages 0-4 leave the
+   particles unchanged; stage 7 advances every member by reps + 1 steps. */
+int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_per_launch);
+
+/* ---------------------------------------------------------------- serving
+   A job server over batches (continuous batching).  No counterpart in the
+   reference.  Jobs (parameters, start positions species 1 first, step count)
+   are submitted at any time; bdk_serve_step runs one tick: it admits queued
+   jobs in arrival order while their HBM footprints fit `hbm_budget` and at
+   most `max_jobs` run, advances every running job by
+   min(tick_steps, smallest remaining step count) steps in one batch, and
+   retires the jobs that are done (positions copied back, device state
+   freed).  Served jobs use fixed dt, capillary forces and device noise.
+   Each job's result equals running it alone with bdk_run_steps. */
+typedef struct bdk_server bdk_server_t;
+
+int bdk_serve_create(int32_t device, uint64_t hbm_budget, int32_t max_jobs, int64_t tick_steps,
+                     bdk_server_t **out);
+void bdk_serve_destroy(bdk_server_t *s);
+const char *bdk_serve_last_error(const bdk_server_t *s);
+int bdk_serve_submit(bdk_server_t *s, const bd_params *p, const int32_t *git, const double *rel, int64_t steps,
+                     int64_t *job_id);
+int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_t *n_done);
+/* Positions of a finished job ([n1+n2][2], species 1 first) and its admission
+   and completion times in seconds after submission. */
+int bdk_serve_result(bdk_server_t *s, int64_t job_id, int32_t *git, double *rel, double *t_admit, double *t_done);
+int bdk_serve_stats(const bdk_server_t *s, int64_t *particle_steps, double *busy_s, uint64_t *hbm_used);
+
 #ifdef __cplusplus
 }
 #endif

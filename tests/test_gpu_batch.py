@@ -1,0 +1,97 @@
+"""Batched simulations (bdk_batch_*): each member of a batch of uneven
+replicas (different N, L, mesh size, scheme, lambda, seed) follows exactly
+the trajectory it follows alone: positions bit-identical after the same
+number of steps, in FP32 and FP64 modes, before and after the FFT autotuner
+changes the kernels' tiling."""
+import numpy as np
+import pytest
+
+import bdk
+from common import params, positions
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+MIX = [
+    # n1, n2, L, Z, scheme, lam, seed
+    (400, 400, 40.0, 64, bdk.CIC, 5.0, 11),
+    (1500, 700, 70.0, 128, bdk.TSC, 20.0, 12),
+    (300, 900, 48.0, 64, bdk.NGP, 8.0, 13),
+    (2500, 2500, 100.0, 256, bdk.TSC, 50.0, 14),
+    (900, 300, 50.0, 128, bdk.CIC, 12.0, 15),
+    (100, 60, 16.0, 32, bdk.TSC, 3.0, 16),
+]
+
+
+def make(prec, spec):
+    n1, n2, L, Z, scheme, lam, seed = spec
+    bp, _ = params(n1=n1, n2=n2, L=L, Z=Z, scheme=scheme, lam=lam, precision=prec, seed=seed, f2_f1=0.5)
+    git, rel = positions(bp, seed, 0.9)
+    return bp, git, rel
+
+
+def sims(prec):
+    out = []
+    for spec in MIX:
+        bp, git, rel = make(prec, spec)
+        s = bdk.RunDynamik(bp)
+        s.set_positions(git, rel)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("prec", [bdk.FP32, bdk.FP64])
+def test_batch_members_equal_alone(prec):
+    say("case: batch vs alone prec=%d" % prec)
+    alone, members = sims(prec), sims(prec)
+    b = bdk.Batch(members)
+    b.run_steps(37)
+    b.sync()
+    for s in alone:
+        s.run_steps(37)
+    for i, (a, m) in enumerate(zip(alone, members)):
+        ga, ra = a.positions()
+        gm, rm = m.positions()
+        assert np.array_equal(ga, gm), "member %d cells differ" % i
+        assert np.array_equal(ra, rm), "member %d positions differ (max %.3e)" % (i, np.abs(ra - rm).max())
+        assert a.step_counter() == m.step_counter() == 37
+    # the autotuner re-tiles the FFT stages; the arithmetic per row is unchanged
+    ept = b.tune(3)
+    say("tuned ept", ept)
+    assert all(e in (8, 16, 32) for e in ept)
+    b.run_steps(11)
+    b.sync()
+    for s in alone:
+        s.run_steps(11)
+    for i, (a, m) in enumerate(zip(alone, members)):
+        assert np.array_equal(a.positions()[1], m.positions()[1]), "member %d after tune" % i
+    # members stay usable through their own handles
+    f_alone, f_member = alone[3].berechne_WCA(), members[3].berechne_WCA()
+    assert np.array_equal(f_alone, f_member)
+    b.close()
+
+
+def test_batch_profile_and_errors():
+    say("case: batch profile and errors")
+    members = sims(bdk.FP32)
+    b = bdk.Batch(members)
+    for st in ("k_wca", "k_deposit", "k_r2c_rows", "k_col", "k_c2r_rows", "step"):
+        assert b.profile(st, 3) > 0.0
+    b.close()
+    # mixed precision, adaptive dt and uploaded noise are refused
+    bp64, git, rel = make(bdk.FP64, MIX[0])
+    s64 = bdk.RunDynamik(bp64)
+    s64.set_positions(git, rel)
+    with pytest.raises(bdk.BdkError):
+        bdk.Batch([members[0], s64])
+    bpa, _ = params(n1=400, n2=400, L=40.0, Z=64, max_travel=0.1, precision=bdk.FP32)
+    sa = bdk.RunDynamik(bpa)
+    sa.set_positions(git, rel)
+    with pytest.raises(bdk.BdkError):
+        bdk.Batch([sa])
+    with pytest.raises(bdk.BdkError):
+        bdk.Batch([members[0], members[0]])
