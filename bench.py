@@ -4,10 +4,14 @@ kT = 1, dt = 1e-4, FP32 forces with FP64 positions), synthetic clipped
 Gaussian-blob start (thinned hexagonal lattice of spacing 0.95 sigma, so every pair
 is farther apart than the 0.9 sigma exclusion).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ensemble]
 
 Prints one JSON line.  The timed region is K graph-replayed steps with the
-particle state resident in HBM, bracketed by stream synchronisation.
+particle state resident in HBM, bracketed by stream synchronisation.  The
+line also carries the ensemble figures: a batched mix of 256 uneven
+replicas (one launch per stage) against the plain one-at-a-time GPU path,
+and a served Poisson job stream (continuous batching) with p50/p99 job
+latency; see bd-simulations_amd/ensemble.py for the mixes.
 """
 import argparse
 import json
@@ -136,12 +140,50 @@ def cpu_baseline(p, git, rel, budget_s=20.0):
                       "TSC 2048^2 workload (%.1f s); host %s" % (n, dt, platform.processor() or platform.machine())}
 
 
+def ensemble_figures(log_fn):
+    """Batched mix (256 replicas) vs the plain path, and the served stream."""
+    import numpy as np
+
+    import ensemble
+
+    log_fn("ensemble: building the 256-replica mix")
+    mix = ensemble.batched_mix(256)
+    sims = ensemble.make_sims(mix)
+    n_mix = sum(s.N for s in sims)
+    plain_steps, batch_steps = 100, 500
+    log_fn("ensemble: plain path, %d steps per replica, one replica at a time" % plain_steps)
+    ps_plain, t_plain = ensemble.time_plain(sims, plain_steps)
+    log_fn("ensemble: batched, %d steps (autotuned FFT tiling)" % batch_steps)
+    ps_b, t_b, ept, stages = ensemble.time_batched(sims, batch_steps)
+    for s in sims:
+        s.close()
+    batched = {"replicas": len(mix), "particles": n_mix, "steps": batch_steps, "value": ps_b,
+               "unit": "particle-steps/s", "plain_value": ps_plain, "plain_steps": plain_steps,
+               "speedup_vs_plain": ps_b / ps_plain, "fft_ept": ept,
+               "stage_ms": {k: v for k, v in stages.items()},
+               "mix": "N log-uniform [2e3, 6.4e4], density 0.5, Z = pow2(L/0.5) in [64, 512], "
+                      "lambda U[5,50], CIC/TSC, FP32 forces, uniform start"}
+    jobs = ensemble.served_jobs(64)
+    work = float(np.mean([(p.n1 + p.n2) * st for p, _, _, st in jobs]))
+    rate = 1.5 * ps_b / work
+    log_fn("ensemble: serving %d jobs, Poisson arrivals at %.2f jobs/s" % (len(jobs), rate))
+    lat, ps_srv, wall, _ = ensemble.serve(jobs, rate, max_jobs=64)
+    served = {"jobs": len(jobs), "arrival_rate_per_s": rate, "p50_latency_s": float(np.percentile(lat, 50)),
+              "p99_latency_s": float(np.percentile(lat, 99)), "aggregate_value": ps_srv,
+              "unit": "particle-steps/s", "wall_s": wall,
+              "mix": "N log-uniform [1e3, 2.56e5], split U[0.2,0.8], density U[0.3,0.7], lambda U[5,100], "
+                     "NGP/CIC/TSC, steps log-uniform [1e3, 1e5], uniform or blob start; offered load 1.5x "
+                     "the batched throughput; budget 64 GiB, at most 64 jobs at once, 256-step ticks"}
+    return batched, served
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched and served figures")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--kernel-probe", type=int, default=0, metavar="REPS",
                     help="run each kernel REPS times plus a calibration copy and exit (for --pmc runs)")
@@ -193,7 +235,7 @@ def main():
     own_bytes = sum(algorithmic_bytes(k, N, w["Z"]) for k in kernels)
     canon = canonical_step_bytes(N, w["Z"])
     out = {
-        "metric": "particle-steps/s at 2M particles, 2048^2 TSC mesh",
+        "metric": "particle-steps/s at 2M particles, 2048\u00b2 TSC mesh; p50/p99 ensemble job latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 forces / f64 positions", "data": "synthetic (seeded clipped Gaussian blob, %.1f%% clipped)"
@@ -208,11 +250,13 @@ def main():
                           "step_ms_events": step_ms_events},
         "kernels_ms": kernels,
     }
+    sim.close()
+    if not args.no_ensemble:
+        out["batched"], out["served"] = ensemble_figures(log)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, git, rel)
     else:
         out["cpu_baseline"] = None
-    sim.close()
     print(json.dumps(out), flush=True)
 
 
