@@ -643,23 +643,70 @@ __global__ void k_rearm(Geo g, DevScalars *sc) {
     sc->exc_bits[(step + 1) & 1] = 0ull;
 }
 // ------------------------------------------------------------------ FFT in LDS
-// One Stockham pass of radix RAD over nrows rows of length n = 2^logn held
-// in LDS (row stride `stride`), in place: every thread loads its butterflies'
-// inputs into registers, the block synchronises, then it writes the outputs.
-// tw[k] = exp(-2 pi i k / 2^logZ); the inverse uses the conjugates.
+// tw[k] = exp(-2 pi i k / 2^logZ); the inverse transforms use the conjugates.
+// radix-4 butterfly in place, y = DFT4(v) (inverse: conjugate kernel)
+template <typename C>
+__device__ __forceinline__ void dft4(C *v, bool inv) {
+    C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    C t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+    C r3 = inv ? cmul_i(t3) : cmul_mi(t3);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, r3);
+    v[3] = csub(t1, r3);
+}
+
+// radix-8 butterfly: y[2m] = DFT4(v[t] + v[t+4])[m],
+// y[2m+1] = DFT4((v[t] - v[t+4]) W^t)[m], W = exp(-+2 pi i / 8)
+template <typename C>
+__device__ __forceinline__ void dft8(const C *v, C *y, bool inv) {
+    using R = decltype(C::x);
+    const R c = (R)0.70710678118654752440;
+    C a[4], b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        a[t] = cadd(v[t], v[t + 4]);
+        b[t] = csub(v[t], v[t + 4]);
+    }
+    // b[1] *= W, b[2] *= W^2, b[3] *= W^3
+    if (!inv) {
+        b[1] = cm<C>(c * (b[1].x + b[1].y), c * (b[1].y - b[1].x));
+        b[2] = cmul_mi(b[2]);
+        b[3] = cm<C>(c * (b[3].y - b[3].x), -c * (b[3].x + b[3].y));
+    } else {
+        b[1] = cm<C>(c * (b[1].x - b[1].y), c * (b[1].x + b[1].y));
+        b[2] = cmul_i(b[2]);
+        b[3] = cm<C>(-c * (b[3].x + b[3].y), c * (b[3].x - b[3].y));
+    }
+    dft4(a, inv);
+    dft4(b, inv);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        y[2 * m] = a[m];
+        y[2 * m + 1] = b[m];
+    }
+}
+
+// One Stockham pass of radix RAD (2, 4 or 8) over nrows rows of length
+// n = 2^logn held in LDS (row stride `stride`), in place: every thread reads
+// its butterflies' inputs (and, for small register footprints, their
+// twiddles from the FP64-built table) before the barrier and writes the
+// outputs after it.  Twiddle of input t of butterfly j: tw[(t k) << twsh],
+// k = j mod Ns (conjugated for the inverse transform).
 template <typename R, int EPT, int RAD>
 __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int nrows, int stride, int logNs,
                                          const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     using C = typename V2<R>::T;
-    constexpr int LR = RAD == 4 ? 2 : 1;
+    constexpr int LR = RAD == 8 ? 3 : (RAD == 4 ? 2 : 1);
     constexpr int NB = EPT / RAD;
+    constexpr bool PRE = sizeof(R) == 4 && NB * (RAD - 1) <= 16;  // twiddles fetched before the barrier
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lognb = logn - LR, nb = 1 << lognb;
-    const int total = nb << 0;
-    const int tot = total * nrows;
+    const int tot = nb * nrows;
     const int Ns = 1 << logNs;
     const int twsh = logZ - (logNs + LR);
     C v[NB][RAD];
+    C w[PRE ? NB : 1][RAD - 1];
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
         const int b = tid + e * nt;
@@ -668,6 +715,11 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
             const C *p = buf + row * stride + j;
 #pragma unroll
             for (int t = 0; t < RAD; ++t) v[e][t] = p[t * nb];
+            if (PRE) {
+                const int k = j & (Ns - 1);
+#pragma unroll
+                for (int t = 1; t < RAD; ++t) w[PRE ? e : 0][t - 1] = tw[(t * k) << twsh];
+            }
         }
     }
     __syncthreads();
@@ -679,22 +731,20 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
             const int k = j & (Ns - 1);
 #pragma unroll
             for (int t = 1; t < RAD; ++t) {
-                C w = tw[(t * k) << twsh];
-                if (inv) w.y = -w.y;
-                v[e][t] = cmul(v[e][t], w);
+                C ww = PRE ? w[PRE ? e : 0][t - 1] : tw[(t * k) << twsh];
+                if (inv) ww.y = -ww.y;
+                v[e][t] = cmul(v[e][t], ww);
             }
             C y[RAD];
             if (RAD == 2) {
                 y[0] = cadd(v[e][0], v[e][1]);
                 y[1] = csub(v[e][0], v[e][1]);
+            } else if (RAD == 4) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) y[t] = v[e][t];
+                dft4(y, inv);
             } else {
-                C t0 = cadd(v[e][0], v[e][2]), t1 = csub(v[e][0], v[e][2]);
-                C t2 = cadd(v[e][1], v[e][3]), t3 = csub(v[e][1], v[e][3]);
-                C r3 = inv ? cmul_i(t3) : cmul_mi(t3);
-                y[0] = cadd(t0, t2);
-                y[2] = csub(t0, t2);
-                y[1] = cadd(t1, r3);
-                y[3] = csub(t1, r3);
+                dft8(v[e], y, inv);
             }
             C *q = buf + row * stride + ((j - k) << LR) + k;
 #pragma unroll
@@ -704,15 +754,19 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
     __syncthreads();
 }
 
+// radix-8 passes, then one radix-4 or radix-2 pass for the remaining bits
 template <typename R, int EPT>
 __device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int logn, int nrows, int stride,
                         const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     int logNs = 0;
-    while (logn - logNs >= 2) {
-        fft_pass<R, EPT, 4>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
-        logNs += 2;
+    while (logn - logNs >= 3) {
+        fft_pass<R, EPT, 8>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+        logNs += 3;
     }
-    if (logn - logNs == 1) fft_pass<R, EPT, 2>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+    if (logn - logNs == 2)
+        fft_pass<R, EPT, 4>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+    else if (logn - logNs == 1)
+        fft_pass<R, EPT, 2>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
 }
 
 constexpr int FFT_THREADS = 256;
@@ -1422,11 +1476,11 @@ static inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); 
 
 // Elements per thread (EPT) of the three FFT kernels; a block of
 // FFT_THREADS threads holds EPT * FFT_THREADS complex values.  The defaults
-// (16, 16, 32) were the fastest of {8, 16, 32} per kernel at Z = 2048 (FP32):
-// 4 rows per r2c block (512 blocks), one column per k_col block (1024),
-// 4 rows per c2r block (512), so the transposed reads of c2r move 64 B per
-// row group.  Each EPT is raised to the smallest value whose block still
-// holds one transform.  BDK_FFT_EPT="a,b,c" (8, 16 or 32) overrides them.
+// (8, 16, 16) were the fastest of {8, 16, 32} per kernel at Z = 2048 (FP32):
+// 2 rows per r2c block (1024 blocks), one column per k_col block (1024),
+// 2 rows per c2r block (1024).  Each EPT is raised to the smallest value
+// whose block still holds one transform.  BDK_FFT_EPT="a,b,c" (8, 16 or 32)
+// overrides them.
 struct FftCfg {
     int e1, e2, e3;      // EPT of r2c rows, columns, c2r rows
     int RB, RC, RB2;     // rows per block
@@ -1437,7 +1491,7 @@ struct FftCfg {
 static const int *fft_ept_default() {
     static int v[3] = {0, 0, 0};
     if (!v[0]) {
-        int a = 16, b = 16, c = 32;
+        int a = 8, b = 16, c = 16;
         if (const char *e = getenv("BDK_FFT_EPT")) {
             int x, y, z;
             if (sscanf(e, "%d,%d,%d", &x, &y, &z) == 3) {
