@@ -489,6 +489,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     g.zero_mode_g = p->lambda > 0.2 * p->L;
     g.ncinv = 1.0f / (float)g.nc;
     g.fast_wca = (p->precision == BDK_FP32 && g.nc >= 6) ? 1 : 0;
+    g.f32noise = p->precision == BDK_FP32 ? 1 : 0;
     g.hinv = 1.0 / g.h;
     {
         // BDK_XCD_REMAP=0 keeps the hardware workgroup order (for comparison)

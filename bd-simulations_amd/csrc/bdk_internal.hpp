@@ -38,6 +38,7 @@ struct Geo {
     int xcd;         // renumber workgroups per XCD (launch order only)
     float ncinv;     // 1/nc (fast exact division of cell indices)
     int fast_wca;    // FP32 forces and nc >= 6: k_wca_f32 (reads lrelf)
+    int f32noise;    // FP32 forces: device noise evaluated in FP32
     double hinv;     // 1/h
     unsigned long long seed;
 };
@@ -68,7 +69,7 @@ struct DevBufs {
     int *arr = nullptr;       // [ncell] arrivals of the current step (zero between steps)
     int *start = nullptr;     // [ncell+1]
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
-    // the state in list order (k_list_state): cell, rel and particle id of list entry q
+    // the state in list order (k_fill): cell, rel and particle id of list entry q
     int *lcell = nullptr;
     double2 *lrel = nullptr;
     int *lpid = nullptr;

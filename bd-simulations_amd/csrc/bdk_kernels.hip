@@ -13,10 +13,9 @@
 //               updated by the particles that change cell;
 //   k_rank      rank of each particle that stayed in its cell (no atomics);
 //   k_scan_*    exclusive scan of the cell counts;
-//   k_fill      cell list = slots grouped by cell, species in bit 31;
-//   k_sort_cells  each cell's entries in slot order;
-//   k_list_state  the particle state gathered into list order (lcell,
-//               lrel, lpid).
+//   k_fill      cell list = slots grouped by cell, species in bit 31, and
+//               the particle state in list order (lcell, lrel, lpid, lrelf);
+//   k_sort_cells  the arrivals of each cell in slot order.
 // Every force, gather and update kernel runs over list positions q and
 // reads the list-ordered state; k_update writes the new state to slot q.
 // The slots are therefore re-sorted by cell every step: consecutive threads
@@ -104,6 +103,24 @@ __device__ __forceinline__ double2 philox_noise(unsigned long long seed, uint32_
     double2 o;
     o.x = r * ca;
     o.y = r * sa;
+    return o;
+}
+
+// The same draws in FP32 (FP32 force mode: the random force is a force):
+// u1, u2 are the high words of the FP64 path's 64-bit draws rounded to FP32
+// (relative error <= 2^-24 + 2^-32/u), r and the angle in FP32.
+__device__ __forceinline__ double2 philox_noise_f32(unsigned long long seed, uint32_t pid, unsigned long long step,
+                                                    int gauss, double var) {
+    uint32_t c[4] = {pid, (uint32_t)step, (uint32_t)(step >> 32), 0x4244u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float u1 = fminf((float)c[1] * 2.3283064365386963e-10f, 0.99999994f);  // 2^-32, below 1
+    const float u2 = (float)c[3] * 2.3283064365386963e-10f;
+    const float r = gauss ? sqrtf(-2.0f * (float)var * logf(1.0f - u1)) : sqrtf(5.0f * u1);
+    float sa, ca;
+    sincospif(2.0f * u2, &sa, &ca);
+    double2 o;
+    o.x = (double)(r * ca);
+    o.y = (double)(r * sa);
     return o;
 }
 
@@ -304,6 +321,9 @@ __device__ int block_exscan4(int *a, int n, int *wsum) {
 // cross-species contributions are summed separately and then added, the
 // grouping of dm:617-702 + dm:973-1059.
 constexpr unsigned LIST_SLOT = 0x7fffffffu;
+// rank of a particle that changed cell during the step: its arrival index
+// in the new cell, marked (ranks stay below 2^30)
+constexpr int RANK_MOVED = 1 << 30;
 
 template <typename R>
 __device__ __forceinline__ void wca_ref_body(const Geo &g, const DevBufs &d, int blk) {
@@ -1018,6 +1038,8 @@ __device__ __forceinline__ double2 particle_noise(const Geo &g, const DevBufs &d
         n.y = 0.0;
     } else if (g.noise_mode == BDK_NOISE_UPLOAD) {
         n = d.noise_up[pid];
+    } else if (g.f32noise) {
+        n = philox_noise_f32(g.seed, (uint32_t)pid, step, g.noise_mode == BDK_NOISE_GAUSS, g.noise_var);
     } else {
         n = philox_noise(g.seed, (uint32_t)pid, step, g.noise_mode == BDK_NOISE_GAUSS, g.noise_var);
     }
@@ -1132,7 +1154,7 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
         // from arr; particles that stay are ranked by k_rank without atomics
         atomicSub(&d.count[c], 1);
         atomicAdd(&d.count[nc_new], 1);
-        d.rank[s] = atomicAdd(&d.arr[nc_new], 1);
+        d.rank[s] = RANK_MOVED | atomicAdd(&d.arr[nc_new], 1);
     }
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
     if (ex > 0.0) atomicMax(&d.sc->exc_bits[(step + 1) & 1], bitsd(ex));
@@ -1307,17 +1329,25 @@ __device__ __forceinline__ void rank_body(const Geo &g, const DevBufs &d, int bl
 
 __global__ void k_rank(Geo g, DevBufs d) { rank_body(g, d, blockIdx.x); }
 
-// List entries from the ranks.  After a step (moved_ranks) a cell holds its
-// stayers first, then its arrivals; after a full binning every rank is the
-// position in the cell.
+// List entries and the list-ordered state from the ranks.  After a step
+// (moved_ranks) a cell holds its stayers first (ascending slot, from
+// k_rank), then its arrivals (marked ranks, ordered by k_sort_cells); after
+// a full binning every rank is the position in the cell (ordered by
+// k_sort_cells).
 __device__ __forceinline__ void fill_body(const Geo &g, const DevBufs &d, int moved_ranks, int blk) {
     const int s = blk * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
-    const unsigned sp2 = d.pid[s] >= g.n1 ? 1u : 0u;
+    const int pid = d.pid[s];
     const int c = d.cell[s];
-    int pos = d.start[c] + d.rank[s];
-    if (moved_ranks && c != d.lcell[s]) pos += d.start[c + 1] - d.start[c] - d.arr[c];
-    ((unsigned *)d.list)[pos] = (unsigned)s | (sp2 << 31);
+    const int rk = d.rank[s];
+    int pos = d.start[c] + (rk & (RANK_MOVED - 1));
+    if (moved_ranks && (rk & RANK_MOVED)) pos += d.start[c + 1] - d.start[c] - d.arr[c];
+    const double2 r = d.rel[s];
+    ((unsigned *)d.list)[pos] = (unsigned)s | ((pid >= g.n1 ? 1u : 0u) << 31);
+    d.lcell[pos] = c;
+    d.lrel[pos] = r;
+    d.lpid[pos] = pid;
+    if (g.fast_wca) d.lrelf[pos] = make_float2((float)(r.x - 0.5 * g.B), (float)(r.y - 0.5 * g.B));
 }
 
 __global__ void k_fill(Geo g, DevBufs d, int moved_ranks) { fill_body(g, d, moved_ranks, blockIdx.x); }
@@ -1327,40 +1357,42 @@ __global__ void k_zero_int(int *p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// each cell's list entries in ascending slot order (fixed summation order)
-__device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, int blk) {
+// Fixed order inside each cell: after a step only the arrivals (the last
+// arr[c] entries) need sorting by slot, after a full binning the whole cell.
+// Entries move together with their list-ordered state.
+__device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, int moved_ranks, int blk) {
     const int c = blk * blockDim.x + threadIdx.x;
     if (c >= g.ncell) return;
-    d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
     const int b = d.start[c], n = d.start[c + 1] - b;
-    if (n < 2) return;
+    const int lo = moved_ranks ? n - d.arr[c] : 0;
+    d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
+    if (n - lo < 2) return;
     unsigned *l = (unsigned *)d.list + b;
-    for (int i = 1; i < n; ++i) {
+    double2 *rl = d.lrel + b;
+    int *pl = d.lpid + b;
+    float2 *fl = d.lrelf + b;
+    const bool f = g.fast_wca;
+    for (int i = lo + 1; i < n; ++i) {
         const unsigned v = l[i];
+        const double2 vr = rl[i];
+        const int vp = pl[i];
+        const float2 vf = f ? fl[i] : make_float2(0.f, 0.f);
         int j = i - 1;
-        while (j >= 0 && (l[j] & LIST_SLOT) > (v & LIST_SLOT)) {
+        while (j >= lo && (l[j] & LIST_SLOT) > (v & LIST_SLOT)) {
             l[j + 1] = l[j];
+            rl[j + 1] = rl[j];
+            pl[j + 1] = pl[j];
+            if (f) fl[j + 1] = fl[j];
             --j;
         }
         l[j + 1] = v;
+        rl[j + 1] = vr;
+        pl[j + 1] = vp;
+        if (f) fl[j + 1] = vf;
     }
 }
 
-__global__ void k_sort_cells(Geo g, DevBufs d) { sort_cells_body(g, d, blockIdx.x); }
-
-// the particle state in list order, read by the force and update kernels
-__device__ __forceinline__ void list_state_body(const Geo &g, const DevBufs &d, int blk) {
-    const int q = blk * blockDim.x + threadIdx.x;
-    if (q >= g.N) return;
-    const int sl = (int)(((const unsigned *)d.list)[q] & LIST_SLOT);
-    const double2 r = d.rel[sl];
-    d.lcell[q] = d.cell[sl];
-    d.lrel[q] = r;
-    d.lpid[q] = d.pid[sl];
-    if (g.fast_wca) d.lrelf[q] = make_float2((float)(r.x - 0.5 * g.B), (float)(r.y - 0.5 * g.B));
-}
-
-__global__ void k_list_state(Geo g, DevBufs d) { list_state_body(g, d, blockIdx.x); }
+__global__ void k_sort_cells(Geo g, DevBufs d, int moved_ranks) { sort_cells_body(g, d, moved_ranks, blockIdx.x); }
 
 // ------------------------------------------------------------------ batches
 // A batch advances many independent simulations (members) with one launch
@@ -1463,12 +1495,7 @@ __global__ void kb_fill(const RepDev *m, const int2 *map) {
 
 __global__ void kb_sort_cells(const RepDev *m, const int2 *map) {
     const int2 t = map[blockIdx.x];
-    sort_cells_body(m[t.x].g, m[t.x].d, t.y);
-}
-
-__global__ void kb_list_state(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    list_state_body(m[t.x].g, m[t.x].d, t.y);
+    sort_cells_body(m[t.x].g, m[t.x].d, 1, t.y);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1741,8 +1768,7 @@ static void launch_scan_fill_inner(bdk_t *h, int inc_step) {
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bsum, h->d.start,
                        ncell);
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
-    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
-    hipLaunchKernelGGL(k_list_state, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
 }
 
 hipError_t launch_scan_fill(bdk_t *h) {
@@ -1825,10 +1851,8 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     mark("k_scan_down");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     mark("k_fill");
-    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     mark("k_sort_cells");
-    hipLaunchKernelGGL(k_list_state, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
-    mark("k_list_state");
     chk(hipGetLastError());
     return e;
 }
@@ -1928,7 +1952,6 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
         hipLaunchKernelGGL(kb_scan_down, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
         hipLaunchKernelGGL(kb_fill, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         hipLaunchKernelGGL(kb_sort_cells, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
-        hipLaunchKernelGGL(kb_list_state, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         break;
     }
     }
