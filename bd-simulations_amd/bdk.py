@@ -90,6 +90,8 @@ def lib():
             "bdk_init_uniform": (C.c_int, [vp, u64, dbl, vp, vp]),
             "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_radial_profile": (C.c_int, [vp, i32, dbl, vp, vp]),
+            "bdk_rho_k": (C.c_int, [vp, i32, dbl, i32, vp, vp]),
             "bdk_batch_create": (C.c_int, [vp, i32, C.POINTER(vp)]),
             "bdk_batch_destroy": (None, [vp]),
             "bdk_batch_last_error": (C.c_char_p, [vp]),
@@ -122,7 +124,7 @@ def exported_symbols():
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos",
-            "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
+            "bdk_radial_profile", "bdk_rho_k", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
@@ -370,6 +372,24 @@ class RunDynamik:
         ms = C.c_double(0.0)
         self._chk(self.L.bdk_profile_kernel(self.h, self.KERNELS[name], reps, C.byref(ms)))
         return ms.value
+
+    def radial_profile(self, nbins, dr):
+        """centres [6] and profiles [6, nbins] (rho1ring, rho2ring, rho1zen1,
+        rho1zen2, rho2zen1, rho2zen2), observablen_auswertung.cpp:140-330."""
+        cen = np.zeros(6)
+        prof = np.zeros((6, nbins))
+        self._chk(self.L.bdk_radial_profile(self.h, nbins, dr, _p(cen), _p(prof)))
+        return cen, prof
+
+    def rho_k(self, species, dq_ratio=0.5, nbins=None):
+        """record_rhoFFTW (auswertung.cpp:529-626): shell averages of the
+        centred FFT of one species' density; returns (re, im)."""
+        if nbins is None:
+            nbins = int(0.5 * self.p.Z * dq_ratio)
+        re = np.zeros(nbins)
+        im = np.zeros(nbins)
+        self._chk(self.L.bdk_rho_k(self.h, species, dq_ratio, nbins, _p(re), _p(im)))
+        return re, im
 
     def write_pos(self, path1, path2, nr, obs_dt):
         self._chk(self.L.bdk_write_pos(self.h, path1.encode(), path2.encode() if path2 else None, nr, obs_dt))

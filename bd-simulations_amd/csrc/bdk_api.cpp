@@ -1204,6 +1204,73 @@ int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, do
     return BDK_OK;
 }
 
+// ---------------------------------------------------------------- observables
+
+int bdk_radial_profile(bdk_t *h, int32_t nbins, double dr, double *centers, double *profiles) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (nbins < 1 || !(dr > 0.0) || !profiles) return fail(h, BDK_ERR_ARG, "need nbins >= 1, dr > 0, profiles");
+    const int nb = (h->g.N + 255) / 256;
+    double *part = nullptr, *cen = nullptr;
+    int *hist = nullptr;
+    hipError_t e = hipMalloc((void **)&part, (size_t)nb * 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&cen, 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&hist, (size_t)6 * nbins * sizeof(int));
+    if (e == hipSuccess) e = bdkx::launch_radial(h, nbins, dr, part, cen, hist);
+    std::vector<int> cnt((size_t)6 * nbins);
+    double c6[6] = {0, 0, 0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), hist, cnt.size() * 4, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c6, cen, sizeof(c6), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (part) (void)hipFree(part);
+    if (cen) (void)hipFree(cen);
+    if (hist) (void)hipFree(hist);
+    if (e != hipSuccess) return hipfail(h, e, "bdk_radial_profile");
+    const double L = h->p.L;
+    const double v1 = h->p.n1 ? L * L / (h->p.n1 * dr * dr * M_PI) : 0.0;  // observablen_auswertung.cpp:268
+    const double v2 = h->p.n2 ? L * L / (h->p.n2 * dr * dr * M_PI) : 0.0;
+    const double vf[6] = {v1, v2, v1, v1, v2, v2};
+    for (int k = 0; k < 6; ++k)
+        for (int b = 0; b < nbins; ++b)
+            profiles[(size_t)k * nbins + b] = cnt[(size_t)k * nbins + b] * (vf[k] / (2 * b + 1));
+    if (centers) std::memcpy(centers, c6, sizeof(c6));
+    return BDK_OK;
+}
+
+int bdk_rho_k(bdk_t *h, int32_t species, double dq_ratio, int32_t nbins, double *re, double *im) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if ((species != 1 && species != 2) || !(dq_ratio > 0.0) || nbins < 1 || !re)
+        return fail(h, BDK_ERR_ARG, "need species 1 or 2, dq_ratio > 0, nbins >= 1, re");
+    const int Z = h->g.Z;
+    double *part = nullptr, *out = nullptr;
+    hipError_t e = hipMalloc((void **)&part, (size_t)(Z / 2 + 1) * nbins * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&out, (size_t)nbins * sizeof(double));
+    if (e == hipSuccess) e = bdkx::launch_rhok(h, species, dq_ratio, nbins, part, out);
+    std::vector<double> sum(nbins);
+    if (e == hipSuccess) e = hipMemcpyAsync(sum.data(), out, nbins * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (part) (void)hipFree(part);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return hipfail(h, e, "bdk_rho_k");
+    // lattice points per shell over the full plane (auswertung.cpp:497-510)
+    std::vector<int64_t> cnt(nbins, 0);
+    for (int i = 0; i < Z; ++i) {
+        const double xq = ((i + Z / 2) % Z) - Z / 2;
+        for (int j = 0; j < Z; ++j) {
+            const double yq = ((j + Z / 2) % Z) - Z / 2;
+            const int bin = (int)(dq_ratio * std::sqrt(xq * xq + yq * yq) + 0.5);
+            if (bin >= 0 && bin < nbins) ++cnt[bin];
+        }
+    }
+    const double L = h->p.L;
+    for (int b = 0; b < nbins; ++b) {  // auswertung.cpp:572-576
+        re[b] = cnt[b] ? sum[b] * (L * L / ((double)Z * Z)) / cnt[b] : 0.0;
+        if (im) im[b] = 0.0;
+    }
+    return BDK_OK;
+}
+
 // ---------------------------------------------------------------- batches
 
 int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {

@@ -95,6 +95,7 @@ struct DepArgs {
     double scale_hi, scale_lo;      // 2^S1, 2^S2
     double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
     int reset;     // re-arm the per-step scalars (block 0)
+    double q1, q2; // species charges (1 and f2_f1 for the capillary force)
 };
 
 // One member of a batch, as the batched kernels read it (device memory).
@@ -157,6 +158,10 @@ hipError_t launch_bin(bdk_t *h);                  // counting after an upload/re
 hipError_t launch_scan_fill(bdk_t *h);            // exclusive scan + fill of the cell list
 hipError_t launch_refresh(bdk_t *h);              // refresh only (no motion)
 hipError_t launch_noise_out(bdk_t *h);            // noise of the next step, by pid
+// rho(k) of one species: part [(Z/2+1)*nbins] scratch, out[nbins] shell sums
+hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *part, double *out);
+// radial density profiles: part [ceil(N/256)*8] scratch, centers[6], hist[6*nbins]
+hipError_t launch_radial(bdk_t *h, int nbins, double dr, double *part, double *centers, int *hist);
 hipError_t set_kernel_attributes();
 hipError_t build_tables(bdk_t *h);                // spectral tables in device memory
 

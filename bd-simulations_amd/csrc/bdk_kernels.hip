@@ -522,20 +522,20 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
             if (std::is_same<R, float>::value) {
                 float cf;
                 if (SCHEME == 0)
-                    cf = s2 ? (float)(g.drho * g.f2_f1) : (float)g.drho;
+                    cf = (float)(g.drho * (s2 ? a.q2 : a.q1));
                 else {
                     cf = (float)wx[ia] * (float)wy[ib] * (float)g.drho;
-                    if (s2) cf = cf * (float)g.f2_f1;
+                    cf = cf * (float)(s2 ? a.q2 : a.q1);
                 }
                 const long long X = __double2ll_rn((double)cf * a.scale_hi);
                 atomicAdd(&acc_hi[w], (unsigned long long)X);
             } else {
                 double cd;
                 if (SCHEME == 0)
-                    cd = s2 ? g.drho * g.f2_f1 : g.drho;
+                    cd = g.drho * (s2 ? a.q2 : a.q1);
                 else {
                     cd = wx[ia] * wy[ib] * g.drho;
-                    if (s2) cd = cd * g.f2_f1;
+                    cd = cd * (s2 ? a.q2 : a.q1);
                 }
                 const long long Xh = __double2ll_rn(cd * a.scale_hi);
                 const double rem = cd - (double)Xh * a.iscale_hi;
@@ -1651,6 +1651,8 @@ static DepPlan dep_plan(const bdk_t *h, int reset) {
     pl.a.scale_lo = std::ldexp(1.0, s2);
     pl.a.iscale_lo = std::ldexp(1.0, -s2);
     pl.a.reset = reset;
+    pl.a.q1 = 1.0;  // species charges of the capillary deposit (gr:31-547)
+    pl.a.q2 = g.f2_f1;
     pl.grid = dim3(nblk(g.nc, tc), nblk(g.nc, tc));
     pl.lds = (size_t)pl.a.win_cap * (f64 ? 16 : 8) + (size_t)(3 * pl.a.seg_cap + 1) * 4;
     return pl;
@@ -1793,6 +1795,194 @@ hipError_t launch_refresh(bdk_t *h) {
 
 hipError_t launch_noise_out(bdk_t *h) {
     hipLaunchKernelGGL(k_noise_out, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
+    return hipGetLastError();
+}
+
+// rho(k) of one species (record_rhoFFTW, auswertung.cpp:529-626): the
+// species' number density (weights drho, gridRoutinen.cpp:672-1040) is
+// transformed (r2c rows + this column pass), multiplied by (-1)^(i+j)
+// (the origin moved to the box centre) and the real parts are summed over
+// shells bin = (int)(dq_dqFT |k|/dq + 1/2) of the full Z x Z plane: a
+// column m of the half spectrum stands for itself and its mirror
+// (Z-j, Z-m), whose conjugate adds the same real part (the imaginary parts
+// cancel).  One thread per column sums in a fixed order into that column's
+// partial row; k_rhok_final adds the partial rows in order.
+template <typename R, int EPT>
+__global__ __launch_bounds__(FFT_THREADS) void k_col_rhok(const typename V2<R>::T *__restrict__ ht,
+                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
+                                                          double dqr, int nbins, double *part) {
+    using C = typename V2<R>::T;
+    extern __shared__ unsigned char smem_raw[];
+    C *s = (C *)smem_raw;
+    const int m = blockIdx.x, tid = threadIdx.x, Zh = Z >> 1;
+    for (int k = tid; k < Z; k += blockDim.x) s[k] = ht[(size_t)m * Z + k];
+    double *p0 = part + (size_t)m * nbins, *pn = part + (size_t)Zh * nbins;
+    for (int b = tid; b < nbins; b += blockDim.x) {
+        p0[b] = 0.0;
+        if (m == 0) pn[b] = 0.0;
+    }
+    __syncthreads();
+    lds_fft<R, EPT>(s, logZ, 1, Z + 1, tw, logZ, false);
+    if (tid != 0) return;
+    for (int j = 0; j < Z; ++j) {
+        const double xq = (double)(((j + Zh) & (Z - 1)) - Zh);
+        const double sg = ((j + m) & 1) ? -1.0 : 1.0;
+        if (m > 0) {
+            const int bin = (int)(dqr * sqrt(xq * xq + (double)m * m) + 0.5);
+            if (bin < nbins) p0[bin] += 2.0 * sg * (double)s[j].x;
+        } else {
+            const C P = s[j], Q = s[(Z - j) & (Z - 1)];
+            const double dc = 0.5 * ((double)P.x + (double)Q.x);  // Re of the DC column
+            const double ny = 0.5 * ((double)P.y + (double)Q.y);  // Re of the Nyquist column
+            int bin = (int)(dqr * sqrt(xq * xq) + 0.5);
+            if (bin < nbins) p0[bin] += sg * dc;
+            bin = (int)(dqr * sqrt(xq * xq + (double)Zh * Zh) + 0.5);
+            if (bin < nbins) pn[bin] += sg * ny;  // (-1)^(j + Z/2) = (-1)^j
+        }
+    }
+}
+
+__global__ void k_rhok_final(const double *part, int ncol, int nbins, double *out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    double v = 0.0;
+    for (int m = 0; m < ncol; ++m) v += part[(size_t)m * nbins + b];
+    out[b] = v;
+}
+
+template <typename R> static void launch_rhok_t(bdk_t *h, double dqr, int nbins, double *part, double *out) {
+    using C = typename V2<R>::T;
+    const int Z = h->g.Z, logZ = h->g.logZ;
+    const int ept = ept_for(8, Z);
+    const size_t lds = (size_t)(Z + 1) * sizeof(C);
+#define RHOK(E)                                                                                              \
+    (void)hipFuncSetAttribute((const void *)k_col_rhok<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                              160 * 1024);                                                                     \
+    hipLaunchKernelGGL((k_col_rhok<R, E>), dim3(Z / 2), dim3(FFT_THREADS), lds, h->stream, (const C *)h->d.ht, \
+                       (const C *)h->d.tw, Z, logZ, dqr, nbins, part)
+    BDK_EPT_SWITCH(ept, RHOK)
+#undef RHOK
+    hipLaunchKernelGGL(k_rhok_final, dim3(nblk(nbins, 256)), dim3(256), 0, h->stream, part, Z / 2 + 1, nbins, out);
+}
+
+// species density (weights q1, q2 per species) -> rho, r2c rows, column pass + shells
+hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *part, double *out) {
+    DepPlan pl = dep_plan(h, 0);
+    pl.a.q1 = species == 1 ? 1.0 : 0.0;
+    pl.a.q2 = species == 2 ? 1.0 : 0.0;
+    const bool f32 = h->p.precision == BDK_FP32;
+    switch (h->g.scheme) {
+    case 0: f32 ? launch_deposit_t<float, 0>(h, pl) : launch_deposit_t<double, 0>(h, pl); break;
+    case 1: f32 ? launch_deposit_t<float, 1>(h, pl) : launch_deposit_t<double, 1>(h, pl); break;
+    default: f32 ? launch_deposit_t<float, 2>(h, pl) : launch_deposit_t<double, 2>(h, pl); break;
+    }
+    launch_fft_pass(h, 0);
+    if (f32)
+        launch_rhok_t<float>(h, dqr, nbins, part, out);
+    else
+        launch_rhok_t<double>(h, dqr, nbins, part, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ observables
+// Radial density profiles of the blob (berechne_dichteprofile,
+// observablen_auswertung.cpp:227-330) around the circular-mean centres of
+// the species (berechne_mittelpunkte, observablen_auswertung.cpp:140-223).
+// Sums use a fixed-order reduction (per-block tree, then one block over the
+// block partials in order) and the histograms integer atomics, so the
+// result is deterministic.
+constexpr int OBS_THREADS = 256;
+
+// block partial sums of cos/sin(2 pi x / L), cos/sin(2 pi y / L) per species
+__global__ __launch_bounds__(OBS_THREADS) void k_circ_partial(Geo g, DevBufs d, double *part) {
+    __shared__ double sm[OBS_THREADS][8];
+    const int q = blockIdx.x * OBS_THREADS + threadIdx.x;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (q < g.N) {
+        const int c = d.lcell[q];
+        const int gx = div_nc(c, g), gy = c - gx * g.nc;
+        const double2 r = d.lrel[q];
+        const double k = 2.0 * M_PI / g.L;
+        const double x = gx * g.B + r.x, y = gy * g.B + r.y;
+        const int o = d.lpid[q] >= g.n1 ? 4 : 0;
+        v[o + 0] = cos(x * k);
+        v[o + 1] = sin(x * k);
+        v[o + 2] = cos(y * k);
+        v[o + 3] = sin(y * k);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sm[threadIdx.x][i] = v[i];
+    __syncthreads();
+    for (int w = OBS_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sm[threadIdx.x][i] += sm[threadIdx.x + w][i];
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) part[blockIdx.x * 8 + threadIdx.x] = sm[0][threadIdx.x];
+}
+
+// centres S1, S2, Sgamma = (S1+S2)/2 from the block partials (one block)
+__global__ __launch_bounds__(OBS_THREADS) void k_circ_final(Geo g, const double *part, int nb, double *centers) {
+    __shared__ double sm[OBS_THREADS][8];
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += OBS_THREADS)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] += part[b * 8 + i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) sm[threadIdx.x][i] = v[i];
+    __syncthreads();
+    for (int w = OBS_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sm[threadIdx.x][i] += sm[threadIdx.x + w][i];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double k = 2.0 * M_PI / g.L;
+        double S[4];
+        for (int a = 0; a < 4; ++a)  // (S1x, S1y, S2x, S2y)
+            S[a] = (atan2(-sm[0][2 * a + 1], -sm[0][2 * a]) + M_PI) / k;
+        centers[0] = S[0];
+        centers[1] = S[1];
+        centers[2] = S[2];
+        centers[3] = S[3];
+        centers[4] = 0.5 * (S[0] + S[2]);
+        centers[5] = 0.5 * (S[1] + S[3]);
+    }
+}
+
+// counts per bin: rho1ring (Sgamma), rho2ring (S2), rho1zen1, rho1zen2,
+// rho2zen1, rho2zen2; distances without periodic images, beyond the last
+// bin into the last bin (as the reference)
+__global__ __launch_bounds__(OBS_THREADS) void k_radial_hist(Geo g, DevBufs d, const double *centers, int nbins,
+                                                             double dr, int *hist) {
+    const int q = blockIdx.x * OBS_THREADS + threadIdx.x;
+    if (q >= g.N) return;
+    const int c = d.lcell[q];
+    const int gx = div_nc(c, g), gy = c - gx * g.nc;
+    const double2 r = d.lrel[q];
+    const double x = gx * g.B + r.x, y = gy * g.B + r.y;
+    const bool s2 = d.lpid[q] >= g.n1;
+    // (histogram, centre index) pairs of this species
+    const int hs[3] = {s2 ? 1 : 0, s2 ? 4 : 2, s2 ? 5 : 3};
+    const int cs[3] = {s2 ? 1 : 2, 0, 1};  // 0: S1, 1: S2, 2: Sgamma
+    for (int i = 0; i < 3; ++i) {
+        const double dx = x - centers[2 * cs[i]], dy = y - centers[2 * cs[i] + 1];
+        const double rr = sqrt(dx * dx + dy * dy);
+        int bin = (int)(rr / dr);
+        if (bin < 0 || bin >= nbins) bin = nbins - 1;
+        atomicAdd(&hist[hs[i] * nbins + bin], 1);
+    }
+}
+
+hipError_t launch_radial(bdk_t *h, int nbins, double dr, double *part, double *centers, int *hist) {
+    const int nb = (int)nblk(h->g.N, OBS_THREADS);
+    hipLaunchKernelGGL(k_circ_partial, dim3(nb), dim3(OBS_THREADS), 0, h->stream, h->g, h->d, part);
+    hipLaunchKernelGGL(k_circ_final, dim3(1), dim3(OBS_THREADS), 0, h->stream, h->g, part, nb, centers);
+    hipLaunchKernelGGL(k_zero_int, dim3(nblk(6 * nbins, 256)), dim3(256), 0, h->stream, hist, 6 * nbins);
+    hipLaunchKernelGGL(k_radial_hist, dim3(nb), dim3(OBS_THREADS), 0, h->stream, h->g, h->d, centers, nbins, dr,
+                       hist);
     return hipGetLastError();
 }
 

@@ -199,6 +199,30 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
    header on nr == 0 with truncate, then one block per checkpoint. */
 int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
 
+/* ---------------------------------------------------------------- observables
+   Radial density profiles of a blob (berechne_dichteprofile,
+   observablen_auswertung.cpp:227-330) on the current positions.  Centres
+   (berechne_mittelpunkte, observablen_auswertung.cpp:140-223): per species and
+   axis the circular mean S = (atan2(-sum sin, -sum cos) + pi) L / 2pi;
+   centers[6] = S1x, S1y, S2x, S2y, Sgx, Sgy with Sg = (S1 + S2)/2.
+   profiles[6][nbins] = rho1ring (species 1 around Sg), rho2ring (species 2
+   around S2, as the reference computes it), rho1zen1, rho1zen2, rho2zen1,
+   rho2zen2 (species a around centre b); bin = (int)(r/dr), r without
+   periodic images, overflow into the last bin; value = count * L^2 /
+   (N_a dr^2 pi) / (2 bin + 1). */
+int bdk_radial_profile(bdk_t *h, int32_t nbins, double dr, double *centers, double *profiles);
+
+/* rho(k) of one species (record_rhoFFTW, auswertung.cpp:529-626): the number
+   density of `species` on the mesh (the force deposit's scheme and weights
+   drho, gridRoutinen.cpp:672-1040), its 2D FFT times (-1)^(i+j), averaged
+   over shells bin = (int)(dq_ratio * |(i',j')| + 1/2) of the frequency
+   indices, times L^2/Z^2.  The reference's dq_dqFT = dq/dq_rhoFFTW = 1/2 and
+   nbins = Z * dq_ratio / 2 (parameter.h:194-200).  Shells at or beyond nbins
+   are left out (the reference's bound check lets bin == nbins through).
+   im[] receives the shell sums of the imaginary parts, which cancel over
+   the full plane (zero here; rounding-size in the reference). */
+int bdk_rho_k(bdk_t *h, int32_t species, double dq_ratio, int32_t nbins, double *re, double *im);
+
 /* ---------------------------------------------------------------- batches
    Many independent simulations (the reference's `runs` and parameter sweeps
    over N, L, lambda and scheme) advanced together with one kernel launch per

@@ -153,3 +153,53 @@ def symbols(Z, L, lam, kap):
     gx = kap * np.sin(q * h)[:, None] / h * G / Z / Z
     gy = kap * np.sin(q * h)[None, :] / h * G / Z / Z
     return gx, gy
+
+
+def radial_profile_ref(x1, x2, L, nbins, dr):
+    """Radial density profiles as observablen_auswertung.cpp computes them:
+    centres by circular means (berechne_mittelpunkte, :164-217), then per
+    particle vorfaktor/(2 bin + 1) into the bin of its distance from a centre
+    (berechne_dichteprofile, :262-322).  x1, x2: absolute positions [n][2]."""
+    k = 2 * np.pi / L
+
+    def centre(x):
+        s = []
+        for a in range(2):
+            c, sn = np.cos(x[:, a] * k).sum(), np.sin(x[:, a] * k).sum()
+            s.append((np.arctan2(-sn, -c) + np.pi) / k)
+        return np.array(s)
+
+    S1, S2 = centre(x1), centre(x2)
+    Sg = 0.5 * (S1 + S2)
+
+    def prof(x, S, n):
+        out = np.zeros(nbins)
+        v = L * L / (n * dr * dr * np.pi)
+        r = np.sqrt(((x - S) ** 2).sum(1))
+        b = (r / dr).astype(np.int64)
+        b[(b < 0) | (b >= nbins)] = nbins - 1
+        np.add.at(out, b, v / (2 * b + 1))
+        return out
+
+    n1, n2 = len(x1), len(x2)
+    profiles = np.array([prof(x1, Sg, n1), prof(x2, S2, n2), prof(x1, S1, n1), prof(x1, S2, n1),
+                         prof(x2, S1, n2), prof(x2, S2, n2)])
+    return np.concatenate([S1, S2, Sg]), profiles
+
+
+def rho_k_ref(x, L, Z, scheme, dqr, nbins):
+    """record_rhoFFTW (auswertung.cpp:529-626) on one species' absolute
+    positions: density by the force deposit's weights, unnormalised forward
+    2D FFT, (-1)^(i+j), shell sums of the real parts over the full plane
+    divided by the shell's lattice-point count, times L^2/Z^2."""
+    h = L / Z
+    rho = deposit_numpy(x, np.ones(len(x)), Z, h, scheme)
+    F = np.fft.fft2(rho)
+    i = np.arange(Z)
+    F = F * np.where((i[:, None] + i[None, :]) % 2 == 1, -1.0, 1.0)
+    fq = ((i + Z // 2) % Z) - Z // 2
+    b = (dqr * np.sqrt(fq[:, None] ** 2 + fq[None, :] ** 2) + 0.5).astype(np.int64)
+    ok = b < nbins
+    re = np.bincount(b[ok], weights=F.real[ok], minlength=nbins)[:nbins]
+    cnt = np.bincount(b[ok], minlength=nbins)[:nbins]
+    return np.where(cnt > 0, re * L * L / (Z * Z) / np.maximum(cnt, 1), 0.0)
