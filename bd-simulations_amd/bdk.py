@@ -90,8 +90,12 @@ def lib():
             "bdk_init_uniform": (C.c_int, [vp, u64, dbl, vp, vp]),
             "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_read_startpos": (C.c_int, [vp, C.c_char_p, vp, vp]),
+            "bdk_save_state": (C.c_int, [vp, C.c_char_p]),
+            "bdk_load_state": (C.c_int, [vp, C.c_char_p]),
             "bdk_radial_profile": (C.c_int, [vp, i32, dbl, vp, vp]),
             "bdk_rho_k": (C.c_int, [vp, i32, dbl, i32, vp, vp]),
+            "bdk_pair_correlation": (C.c_int, [vp, i32, dbl, vp, vp, vp]),
             "bdk_batch_create": (C.c_int, [vp, i32, C.POINTER(vp)]),
             "bdk_batch_destroy": (None, [vp]),
             "bdk_batch_last_error": (C.c_char_p, [vp]),
@@ -124,7 +128,8 @@ def exported_symbols():
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos",
-            "bdk_radial_profile", "bdk_rho_k", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
+            "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
+            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
@@ -167,6 +172,17 @@ def init_uniform(p, seed, dmin=0.9):
     git = np.zeros((N, 2), np.int32)
     rel = np.zeros((N, 2))
     rc = lib().bdk_init_uniform(C.byref(p), seed, dmin, _p(git), _p(rel))
+    if rc:
+        raise BdkError(rc, lib().bdk_last_error(None).decode())
+    return git, rel
+
+
+def read_startpos(p, path):
+    """init_pos_aus_datei (dm:1576-1643) with P8 fixed; (git, rel), species 1 first."""
+    N = p.n1 + p.n2
+    git = np.zeros((N, 2), np.int32)
+    rel = np.zeros((N, 2))
+    rc = lib().bdk_read_startpos(C.byref(p), path.encode(), _p(git), _p(rel))
     if rc:
         raise BdkError(rc, lib().bdk_last_error(None).decode())
     return git, rel
@@ -373,6 +389,12 @@ class RunDynamik:
         self._chk(self.L.bdk_profile_kernel(self.h, self.KERNELS[name], reps, C.byref(ms)))
         return ms.value
 
+    def save_state(self, path):
+        self._chk(self.L.bdk_save_state(self.h, path.encode()))
+
+    def load_state(self, path):
+        self._chk(self.L.bdk_load_state(self.h, path.encode()))
+
     def radial_profile(self, nbins, dr):
         """centres [6] and profiles [6, nbins] (rho1ring, rho2ring, rho1zen1,
         rho1zen2, rho2zen1, rho2zen2), observablen_auswertung.cpp:140-330."""
@@ -380,6 +402,12 @@ class RunDynamik:
         prof = np.zeros((6, nbins))
         self._chk(self.L.bdk_radial_profile(self.h, nbins, dr, _p(cen), _p(prof)))
         return cen, prof
+
+    def pair_correlation(self, nbins, dr):
+        """g11, g12, g22 (auswertung.cpp:110-188)."""
+        out = [np.zeros(nbins) for _ in range(3)]
+        self._chk(self.L.bdk_pair_correlation(self.h, nbins, dr, _p(out[0]), _p(out[1]), _p(out[2])))
+        return out
 
     def rho_k(self, species, dq_ratio=0.5, nbins=None):
         """record_rhoFFTW (auswertung.cpp:529-626): shell averages of the

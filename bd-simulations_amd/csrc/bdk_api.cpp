@@ -1204,6 +1204,142 @@ int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, do
     return BDK_OK;
 }
 
+// ---------------------------------------------------------------- restart
+
+// init_pos_aus_datei (dm:1576-1643) with P8 fixed: two header lines, then N1
+// lines "nr 1 x y" and N2 lines "nr 2 x y" (nr counting from 0 per species),
+// x and y read as float like the reference, 0 <= x, y <= L;
+// git = (int)(x/B), rel = x - git*B.  Particles species 1 first.
+int bdk_read_startpos(const bd_params *p, const char *path, int32_t *git, double *rel) {
+    if (!p || !path || !git || !rel) return fail(nullptr, BDK_ERR_ARG, "null argument");
+    std::string v = validate(*p);
+    if (!v.empty()) return fail(nullptr, BDK_ERR_ARG, v);
+    FILE *in = std::fopen(path, "r");
+    if (!in) return fail(nullptr, BDK_ERR_ARG, std::string("cannot open ") + path);
+    char buf[256];
+    if (!std::fgets(buf, sizeof buf, in) || !std::fgets(buf, sizeof buf, in)) {
+        std::fclose(in);
+        return fail(nullptr, BDK_ERR_ARG, "startpos: cannot read header lines 1-2");
+    }
+    const double B = derive(*p).B, L = p->L;
+    int line = 0;
+    for (int sp = 1; sp <= 2; ++sp) {
+        const int n = sp == 1 ? p->n1 : p->n2, off = sp == 1 ? 0 : p->n1;
+        for (int i = 0; i < n; ++i, ++line) {
+            int nr = -1, typ = -1;
+            float xf = 0.f, yf = 0.f;
+            if (std::fscanf(in, "%d %d %g %g", &nr, &typ, &xf, &yf) != 4 || nr != i || typ != sp) {
+                std::fclose(in);
+                return fail(nullptr, BDK_ERR_ARG, "startpos: bad particle line " + std::to_string(line + 3));
+            }
+            const double x = xf, y = yf;
+            if (x < 0.0 || x > L || y < 0.0 || y > L) {
+                std::fclose(in);
+                return fail(nullptr, BDK_ERR_ARG, "startpos: coordinates out of range on line " + std::to_string(line + 3));
+            }
+            const int gx = (int)(x / B), gy = (int)(y / B);
+            git[2 * (size_t)(off + i)] = gx;
+            git[2 * (size_t)(off + i) + 1] = gy;
+            rel[2 * (size_t)(off + i)] = x - gx * B;
+            rel[2 * (size_t)(off + i) + 1] = y - gy * B;
+        }
+    }
+    std::fclose(in);
+    return BDK_OK;
+}
+
+namespace {
+const char kStateMagic[8] = {'B', 'D', 'K', 'S', 'T', 'A', 'T', '1'};
+}
+
+// Device snapshot: parameters, step counters and the particle state in list
+// order (lcell, lrel, lpid).  Loading it reproduces the list order, so a
+// resumed run continues bit for bit.
+int bdk_save_state(bdk_t *h, const char *path) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (!path) return fail(h, BDK_ERR_ARG, "null path");
+    const int N = h->g.N;
+    std::vector<int32_t> cell(N), pid(N);
+    std::vector<double> rel(2 * (size_t)N);
+    DevScalars sc;
+    HIPCHK(h, hipMemcpyAsync(cell.data(), h->d.lcell, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.lpid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(rel.data(), h->d.lrel, (size_t)N * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(h, BDK_ERR_ARG, std::string("cannot open ") + path);
+    bd_params p = h->p;
+    p.device = 0;
+    const int64_t hs = h->host_step;
+    const uint64_t step = sc.step;
+    bool ok = std::fwrite(kStateMagic, 1, 8, f) == 8 && std::fwrite(&p, sizeof p, 1, f) == 1 &&
+              std::fwrite(&step, 8, 1, f) == 1 && std::fwrite(&hs, 8, 1, f) == 1 &&
+              std::fwrite(cell.data(), 4, N, f) == (size_t)N && std::fwrite(rel.data(), 16, N, f) == (size_t)N &&
+              std::fwrite(pid.data(), 4, N, f) == (size_t)N;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? BDK_OK : fail(h, BDK_ERR_ARG, std::string("write failed: ") + path);
+}
+
+int bdk_load_state(bdk_t *h, const char *path) {
+    if (!h || !path) return fail(h, BDK_ERR_ARG, "null argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(h, BDK_ERR_ARG, std::string("cannot open ") + path);
+    char magic[8];
+    bd_params p;
+    uint64_t step = 0;
+    int64_t hs = 0;
+    const int N = h->g.N;
+    std::vector<int32_t> cell(N), pid(N);
+    std::vector<double> rel(2 * (size_t)N);
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kStateMagic, 8) == 0 &&
+              std::fread(&p, sizeof p, 1, f) == 1 && std::fread(&step, 8, 1, f) == 1 && std::fread(&hs, 8, 1, f) == 1;
+    if (ok) {
+        bd_params mine = h->p;
+        mine.device = 0;
+        ok = std::memcmp(&p, &mine, sizeof p) == 0;
+        if (!ok) {
+            std::fclose(f);
+            return fail(h, BDK_ERR_ARG, "state file was written with different parameters");
+        }
+        ok = std::fread(cell.data(), 4, N, f) == (size_t)N && std::fread(rel.data(), 16, N, f) == (size_t)N &&
+             std::fread(pid.data(), 4, N, f) == (size_t)N;
+    }
+    std::fclose(f);
+    if (!ok) return fail(h, BDK_ERR_ARG, std::string("not a readable state file: ") + path);
+    // the slots take the saved list order; host copies per species for later uploads
+    for (int sp = 0; sp < 2; ++sp) {
+        const int n = sp == 0 ? h->p.n1 : h->p.n2;
+        h->hgit[sp].assign(2 * (size_t)n, 0);
+        h->hrel[sp].assign(2 * (size_t)n, 0.0);
+    }
+    std::vector<char> seen(N, 0);
+    for (int q = 0; q < N; ++q) {
+        const int id = pid[q];
+        if (id < 0 || id >= N || seen[id] || cell[q] < 0 || cell[q] >= h->g.ncell)
+            return fail(h, BDK_ERR_ARG, "corrupt state file");
+        seen[id] = 1;
+        const int sp = id < h->p.n1 ? 0 : 1, i = sp ? id - h->p.n1 : id;
+        h->hgit[sp][2 * (size_t)i] = cell[q] / h->g.nc;
+        h->hgit[sp][2 * (size_t)i + 1] = cell[q] % h->g.nc;
+        h->hrel[sp][2 * (size_t)i] = rel[2 * (size_t)q];
+        h->hrel[sp][2 * (size_t)i + 1] = rel[2 * (size_t)q + 1];
+    }
+    drop_graph(h);
+    HIPCHK(h, hipMemcpyAsync(h->d.cell, cell.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.pid, pid.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.rel, rel.data(), (size_t)N * 16, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d.sc, &step, 8, hipMemcpyHostToDevice, h->stream));  // DevScalars::step
+    HIPCHK(h, hipMemsetAsync(h->d.count, 0, (size_t)h->g.ncell * 4, h->stream));
+    HIPCHK(h, bdkx::launch_bin(h));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->pos_set[0] = h->pos_set[1] = true;
+    h->dev_valid = true;
+    h->host_step = hs;
+    return BDK_OK;
+}
+
 // ---------------------------------------------------------------- observables
 
 int bdk_radial_profile(bdk_t *h, int32_t nbins, double dr, double *centers, double *profiles) {
@@ -1234,6 +1370,34 @@ int bdk_radial_profile(bdk_t *h, int32_t nbins, double dr, double *centers, doub
         for (int b = 0; b < nbins; ++b)
             profiles[(size_t)k * nbins + b] = cnt[(size_t)k * nbins + b] * (vf[k] / (2 * b + 1));
     if (centers) std::memcpy(centers, c6, sizeof(c6));
+    return BDK_OK;
+}
+
+int bdk_pair_correlation(bdk_t *h, int32_t nbins, double dr, double *g11, double *g12, double *g22) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (nbins < 1 || nbins > 4096 || !(dr > 0.0)) return fail(h, BDK_ERR_ARG, "need 1 <= nbins <= 4096, dr > 0");
+    unsigned long long *hist = nullptr;
+    const size_t hb = (size_t)3 * nbins * sizeof(unsigned long long);
+    hipError_t e = hipMalloc((void **)&hist, hb);
+    if (e == hipSuccess) e = hipMemsetAsync(hist, 0, hb, h->stream);
+    if (e == hipSuccess) e = bdkx::launch_pair_hist(h, nbins, dr, hist);
+    std::vector<unsigned long long> cnt((size_t)3 * nbins);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), hist, hb, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (hist) (void)hipFree(hist);
+    if (e != hipSuccess) return hipfail(h, e, "bdk_pair_correlation");
+    // auswertung.cpp:120-121, 149-150, 173-174 (g12 keeps the reference's N1-1)
+    const double L = h->p.L, n1 = h->p.n1, n2 = h->p.n2;
+    const double v11 = n1 > 1 ? 2.0 / ((n1 - 1) * M_PI * (n1 / L / L) * dr * dr) : 0.0;
+    const double v12 = (n1 > 1 && n2 > 0) ? 1.0 / ((n1 - 1) * M_PI * (n2 / L / L) * dr * dr) : 0.0;
+    const double v22 = n2 > 1 ? 2.0 / ((n2 - 1) * M_PI * (n2 / L / L) * dr * dr) : 0.0;
+    for (int b = 0; b < nbins; ++b) {
+        // same-species pairs were counted in both orders
+        if (g11) g11[b] = (double)(cnt[b] / 2) * (v11 / (2 * b + 1));
+        if (g12) g12[b] = (double)cnt[(size_t)nbins + b] * (v12 / (2 * b + 1));
+        if (g22) g22[b] = (double)(cnt[2 * (size_t)nbins + b] / 2) * (v22 / (2 * b + 1));
+    }
     return BDK_OK;
 }
 

@@ -158,6 +158,8 @@ hipError_t launch_bin(bdk_t *h);                  // counting after an upload/re
 hipError_t launch_scan_fill(bdk_t *h);            // exclusive scan + fill of the cell list
 hipError_t launch_refresh(bdk_t *h);              // refresh only (no motion)
 hipError_t launch_noise_out(bdk_t *h);            // noise of the next step, by pid
+// ordered-pair distance histograms [3][nbins] (11, 12, 22), zeroed by the caller
+hipError_t launch_pair_hist(bdk_t *h, int nbins, double dr, unsigned long long *hist);
 // rho(k) of one species: part [(Z/2+1)*nbins] scratch, out[nbins] shell sums
 hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *part, double *out);
 // radial density profiles: part [ceil(N/256)*8] scratch, centers[6], hist[6*nbins]

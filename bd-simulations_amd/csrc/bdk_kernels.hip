@@ -710,8 +710,8 @@ __device__ __forceinline__ void dft8(const C *v, C *y, bool inv) {
 // One Stockham pass of radix RAD (2, 4 or 8) over nrows rows of length
 // n = 2^logn held in LDS (row stride `stride`), in place: every thread reads
 // its butterflies' inputs (and, for small register footprints, their
-// twiddles from the FP64-built table) before the barrier and writes the
-// outputs after it.  Twiddle of input t of butterfly j: tw[(t k) << twsh],
+// twiddles: powers 1, 2, 4 from the FP64-built table, 3, 5, 6, 7 as
+// products) before the barrier and writes the outputs after it.  Twiddle of input t of butterfly j: tw[(t k) << twsh],
 // k = j mod Ns (conjugated for the inverse transform).
 template <typename R, int EPT, int RAD>
 __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int nrows, int stride, int logNs,
@@ -736,9 +736,21 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
 #pragma unroll
             for (int t = 0; t < RAD; ++t) v[e][t] = p[t * nb];
             if (PRE) {
+                // tw^1, tw^2, tw^4 from the table, the other powers as products
+                // (one to two roundings deeper; 3 loads instead of 7 for radix 8)
                 const int k = j & (Ns - 1);
-#pragma unroll
-                for (int t = 1; t < RAD; ++t) w[PRE ? e : 0][t - 1] = tw[(t * k) << twsh];
+                C *ww = w[PRE ? e : 0];
+                ww[0] = tw[k << twsh];
+                if (RAD >= 4) {
+                    ww[1] = tw[(2 * k) << twsh];
+                    ww[2] = cmul(ww[0], ww[1]);
+                }
+                if (RAD == 8) {
+                    ww[3] = tw[(4 * k) << twsh];
+                    ww[4] = cmul(ww[3], ww[0]);
+                    ww[5] = cmul(ww[3], ww[1]);
+                    ww[6] = cmul(ww[3], ww[2]);
+                }
             }
         }
     }
@@ -1881,6 +1893,68 @@ hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *par
         launch_rhok_t<float>(h, dqr, nbins, part, out);
     else
         launch_rhok_t<double>(h, dqr, nbins, part, out);
+    return hipGetLastError();
+}
+
+// Pair correlation g11, g12, g22 (record_korrelationsfunktion11/12/22,
+// auswertung.cpp:110-188): minimum-image distances (abstand2, am:100-166)
+// of all pairs closer than nbins*dr, found through the cells within R cells,
+// binned by (int)(r/dr).  Ordered pairs (12 pairs from the species-1 side)
+// are counted with integer atomics in LDS per block, then added to the
+// global histograms: the counts are exact and deterministic.
+__global__ __launch_bounds__(256) void k_pair_hist(Geo g, DevBufs d, int nbins, double dr, int R,
+                                                   unsigned long long *hist) {
+    extern __shared__ unsigned int shist[];  // [3][nbins]
+    for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x) shist[b] = 0u;
+    __syncthreads();
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < g.N) {
+        const int nc = g.nc, c = d.lcell[q];
+        const int cx = div_nc(c, g), cy = c - cx * nc;
+        const double2 ri = d.lrel[q];
+        const double xi = cx * g.B + ri.x, yi = cy * g.B + ri.y, L = g.L;
+        const bool s2i = d.lpid[q] >= g.n1;
+        const bool all = 2 * R + 1 >= nc;
+        const int a0 = all ? 0 : cx - R, a1 = all ? nc - 1 : cx + R;
+        const int b0 = all ? 0 : cy - R, b1 = all ? nc - 1 : cy + R;
+        for (int a = a0; a <= a1; ++a) {
+            const int ux = wrapc(a, nc);
+            for (int b = b0; b <= b1; ++b) {
+                const int uy = wrapc(b, nc), cc = ux * nc + uy;
+                const int qe = d.start[cc + 1];
+                for (int j = d.start[cc]; j < qe; ++j) {
+                    if (j == q) continue;
+                    const bool s2j = d.lpid[j] >= g.n1;
+                    if (s2i && !s2j) continue;
+                    const double2 rj = d.lrel[j];
+                    double t1 = xi - (ux * g.B + rj.x), t2 = t1 + L, t3 = t1 - L;
+                    double m = t3 * t3 < t2 * t2 ? t3 * t3 : t2 * t2;
+                    const double dx2 = m < t1 * t1 ? m : t1 * t1;
+                    t1 = yi - (uy * g.B + rj.y);
+                    t2 = t1 + L;
+                    t3 = t1 - L;
+                    m = t3 * t3 < t2 * t2 ? t3 * t3 : t2 * t2;
+                    const double dy2 = m < t1 * t1 ? m : t1 * t1;
+                    const int idx = (int)(sqrt(dx2 + dy2) / dr);
+                    if (idx < 0 || idx >= nbins) continue;
+                    const int type = s2i ? 2 : (s2j ? 1 : 0);
+                    atomicAdd(&shist[type * nbins + idx], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 3 * nbins; b += blockDim.x)
+        if (shist[b]) atomicAdd(&hist[b], (unsigned long long)shist[b]);
+}
+
+hipError_t launch_pair_hist(bdk_t *h, int nbins, double dr, unsigned long long *hist) {
+    int R = (int)ceil(nbins * dr / h->g.B);
+    if (R < 1) R = 1;
+    const size_t lds = (size_t)3 * nbins * sizeof(unsigned int);
+    (void)hipFuncSetAttribute((const void *)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024);
+    hipLaunchKernelGGL(k_pair_hist, dim3(nblk(h->g.N, 256)), dim3(256), lds, h->stream, h->g, h->d, nbins, dr, R,
+                       hist);
     return hipGetLastError();
 }
 

@@ -199,6 +199,19 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
    header on nr == 0 with truncate, then one block per checkpoint. */
 int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
 
+/* ---------------------------------------------------------------- restart
+   init_pos_aus_datei (dm:1576-1643) with its P8 defect fixed (rel = x - git*B):
+   reads startpos.txt-format files (two header lines, then "nr typ x y" for
+   species 1 then 2, nr from 0 per species, coordinates parsed as float like
+   the reference, 0 <= x, y <= L) into git/rel, species 1 first. */
+int bdk_read_startpos(const bd_params *p, const char *path, int32_t *git, double *rel);
+/* Device snapshot (no counterpart in the reference, whose restart path is
+   broken): parameters, step counters and the particle state in cell-list
+   order.  bdk_load_state on a handle created with the same parameters
+   continues the run bit for bit. */
+int bdk_save_state(bdk_t *h, const char *path);
+int bdk_load_state(bdk_t *h, const char *path);
+
 /* ---------------------------------------------------------------- observables
    Radial density profiles of a blob (berechne_dichteprofile,
    observablen_auswertung.cpp:227-330) on the current positions.  Centres
@@ -211,6 +224,14 @@ int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, do
    periodic images, overflow into the last bin; value = count * L^2 /
    (N_a dr^2 pi) / (2 bin + 1). */
 int bdk_radial_profile(bdk_t *h, int32_t nbins, double dr, double *centers, double *profiles);
+
+/* Pair correlation functions g11, g12, g22 (record_korrelationsfunktion11/12/22,
+   auswertung.cpp:110-188): minimum-image pair distances binned by (int)(r/dr)
+   for r < nbins*dr; each pair adds vorfaktor/(2 bin + 1) with vorfaktor =
+   2/((N1-1) pi rho1 dr^2) (11), 1/((N1-1) pi rho2 dr^2) (12, as the reference
+   writes it), 2/((N2-1) pi rho2 dr^2) (22), rho_a = N_a/L^2.  Any of the
+   three outputs may be null. */
+int bdk_pair_correlation(bdk_t *h, int32_t nbins, double dr, double *g11, double *g12, double *g22);
 
 /* rho(k) of one species (record_rhoFFTW, auswertung.cpp:529-626): the number
    density of `species` on the mesh (the force deposit's scheme and weights

@@ -203,3 +203,26 @@ def rho_k_ref(x, L, Z, scheme, dqr, nbins):
     re = np.bincount(b[ok], weights=F.real[ok], minlength=nbins)[:nbins]
     cnt = np.bincount(b[ok], minlength=nbins)[:nbins]
     return np.where(cnt > 0, re * L * L / (Z * Z) / np.maximum(cnt, 1), 0.0)
+
+
+def pair_correlation_ref(x1, x2, L, nbins, dr):
+    """record_korrelationsfunktion11/12/22 (auswertung.cpp:110-188) by brute
+    force over all pairs: abstand2 (am:100-166) per axis min over d, d+L, d-L."""
+    def d2(a, b):
+        d = a[:, None, :] - b[None, :, :]
+        m = np.minimum(np.minimum(d * d, (d + L) ** 2), (d - L) ** 2)
+        return m.sum(2)
+
+    def hist(a2, v, mask):
+        idx = np.floor(np.sqrt(a2[mask]) / dr).astype(np.int64)
+        idx = idx[idx < nbins]
+        return np.bincount(idx, minlength=nbins)[:nbins] * (v / (2 * np.arange(nbins) + 1))
+
+    n1, n2 = len(x1), len(x2)
+    v11 = 2.0 / ((n1 - 1) * np.pi * (n1 / L / L) * dr * dr)
+    v12 = 1.0 / ((n1 - 1) * np.pi * (n2 / L / L) * dr * dr)
+    v22 = 2.0 / ((n2 - 1) * np.pi * (n2 / L / L) * dr * dr)
+    a11, a12, a22 = d2(x1, x1), d2(x1, x2), d2(x2, x2)
+    tri1 = np.tril(np.ones((n1, n1), bool), -1)
+    tri2 = np.tril(np.ones((n2, n2), bool), -1)
+    return hist(a11, v11, tri1), hist(a12, v12, np.ones((n1, n2), bool)), hist(a22, v22, tri2)

@@ -137,3 +137,31 @@ def test_init_blob_profile_clip_and_exclusion():
     lattice = 2.0 / (math.sqrt(3.0) * 0.81)
     assert 0.9 * lattice < core <= 1.02 * lattice
     assert 0.0 < clipped < 1.0
+
+
+def test_read_startpos_reference_format(tmp_path):
+    """init_pos_aus_datei's format (dm:1576-1643): two header lines, species 1
+    then 2 as 'nr typ x y', float-parsed; git = (int)(x/B), rel = x - git B
+    (the reference's P8 computes rel from an uninitialised value)."""
+    p = bdk.make_params(n1=3, n2=2, L=20.0, Z=16)
+    B = bdk.derived(p)["B"]
+    pts = [(1, 0, 1.25, 3.5), (1, 1, 19.75, 0.0), (1, 2, 10.0, 20.0), (2, 0, 5.5, 5.5), (2, 1, 0.3, 17.9)]
+    f = tmp_path / "startpos.txt"
+    f.write_text("# header\n\n" + "".join("%d \t %d \t %g \t %g\n" % (nr, t, x, y) for t, nr, x, y in pts))
+    git, rel = bdk.read_startpos(p, str(f))
+    for k, (t, nr, x, y) in enumerate(pts):
+        xf, yf = float(np.float32(x)), float(np.float32(y))
+        assert tuple(git[k]) == (int(xf / B), int(yf / B))
+        assert rel[k, 0] == xf - git[k, 0] * B and rel[k, 1] == yf - git[k, 1] * B
+    bad = tmp_path / "bad.txt"
+    bad.write_text("#\n\n" + "".join("%d \t %d \t %g \t %g\n" % (nr, 2 if k == 1 else t, x, y)
+                                      for k, (t, nr, x, y) in enumerate(pts)))
+    with pytest.raises(bdk.BdkError):
+        bdk.read_startpos(p, str(bad))
+    out = tmp_path / "out.txt"
+    out.write_text("#\n\n" + "".join("%d \t %d \t %g \t %g\n" % (nr, t, x + (30 if k == 3 else 0), y)
+                                      for k, (t, nr, x, y) in enumerate(pts)))
+    with pytest.raises(bdk.BdkError):
+        bdk.read_startpos(p, str(out))
+    with pytest.raises(bdk.BdkError):
+        bdk.read_startpos(p, str(tmp_path / "missing.txt"))

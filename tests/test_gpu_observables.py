@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import bdk
-from common import params
+from common import params, positions
 from helpers import radial_profile_ref, rho_k_ref
 
 pytestmark = pytest.mark.gpu
@@ -78,3 +78,26 @@ def test_rho_k_matches_reference_formula(scheme):
     t.run_steps(26)
     s.run_steps(1)
     assert np.array_equal(s.positions()[1], t.positions()[1])
+
+
+@pytest.mark.parametrize("nbins,dr", [(40, 0.25), (30, 1.0)])
+def test_pair_correlation_matches_brute_force(nbins, dr):
+    """g11, g12, g22 against all pairs by brute force (the reference's loops);
+    (30, 1.0) reaches past half the box, so every pair is seen once through
+    the full cell range.  Far beyond the exclusion an ideal-gas-like start
+    gives g near 1."""
+    from helpers import pair_correlation_ref
+    print("case: g(r) nbins=%d dr=%g" % (nbins, dr), flush=True)
+    L = 50.0
+    bp, _ = params(n1=700, n2=500, L=L, Z=64, scheme=bdk.CIC, precision=bdk.FP64, seed=61)
+    git, rel = positions(bp, 61)
+    s = bdk.RunDynamik(bp)
+    s.set_positions(git, rel)
+    s.run_steps(10)
+    g = s.pair_correlation(nbins, dr)
+    x = s.absolute()
+    want = pair_correlation_ref(x[:700], x[700:], L, nbins, dr)
+    for got, ref in zip(g, want):
+        assert np.allclose(got, ref, rtol=1e-12, atol=0)
+    if dr == 0.25:
+        assert g[0][:3].max() == 0.0  # nothing closer than 0.75 sigma
