@@ -46,6 +46,15 @@ class Params(C.Structure):
     ]
 
 
+class StepInfo(C.Structure):
+    """bdk_step_info_t (TimestepInfo, signaturen2.h:71-145)."""
+    _fields_ = [("num_wca", C.c_int64), ("max_wca", C.c_double * 2), ("max_wca_type", C.c_int32),
+                ("max_wca_index", C.c_int32), ("max_wca_nlist", C.c_int32 * 2), ("max_kap", C.c_double * 2),
+                ("max_kap_type", C.c_int32), ("max_kap_index", C.c_int32), ("max_rnd", C.c_double * 2),
+                ("max_rnd_type", C.c_int32), ("max_rnd_index", C.c_int32), ("dt", C.c_double),
+                ("cell_changes", C.c_int32 * 2)]
+
+
 _lib = None
 
 
@@ -90,6 +99,7 @@ def lib():
             "bdk_init_uniform": (C.c_int, [vp, u64, dbl, vp, vp]),
             "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_step_info": (C.c_int, [vp, dbl, vp]),
             "bdk_read_startpos": (C.c_int, [vp, C.c_char_p, vp, vp]),
             "bdk_save_state": (C.c_int, [vp, C.c_char_p]),
             "bdk_load_state": (C.c_int, [vp, C.c_char_p]),
@@ -128,7 +138,7 @@ def exported_symbols():
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos",
-            "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
+            "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
             "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
@@ -388,6 +398,12 @@ class RunDynamik:
         ms = C.c_double(0.0)
         self._chk(self.L.bdk_profile_kernel(self.h, self.KERNELS[name], reps, C.byref(ms)))
         return ms.value
+
+    def zeitschritt_debug(self, tmax=1.0):
+        """One step with the TimestepInfo counters (dm:168-211)."""
+        info = StepInfo()
+        self._chk(self.L.bdk_step_info(self.h, tmax, C.byref(info)))
+        return info
 
     def save_state(self, path):
         self._chk(self.L.bdk_save_state(self.h, path.encode()))

@@ -109,16 +109,25 @@ Footprint footprint_of(const bd_params &p) {
     Footprint f;
     // cell, rel, pid, fwca, fkap, noise_up, noise_out, rank, list
     f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4 + 8);
-    f.cells = ncell * 12 + 4 + 4096 * 4;
+    f.cells = ncell * 12 + 4 + 4096 * 32 * 4;
     // rho, ht, gt, fmesh, tables
     f.mesh = Z * Z * R + (Z / 2) * Z * 2 * R + (Z / 2) * Z * 4 * R + Z * Z * 2 * R + Z * (2 * R + 3 * 8 + 3 * 4);
     f.total = f.particles + f.cells + f.mesh + sizeof(DevScalars);
     return f;
 }
 
+void release_streams(bdk_t *h) {
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->aux) (void)hipStreamDestroy(h->aux);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    h->ev_fork = h->ev_join = nullptr;
+    h->aux = h->stream = nullptr;
+}
+
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bsum, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -185,6 +194,8 @@ int check_nonfinite(bdk_t *h) {
     DevScalars sc;
     HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (sc.nonfinite & 2)
+        return fail(h, BDK_ERR_HIP, "internal error: inconsistent cell lists (cell counts do not sum to N)");
     if (sc.nonfinite)
         return fail(h, BDK_ERR_NONFINITE,
                     "a particle position became non-finite or ran away (|rel| >= 1e9); it was held in place");
@@ -313,6 +324,8 @@ double dt_from_bits(unsigned long long b) {
 struct bdk_batch {
     std::vector<bdk_t *> mem;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;  // WCA stage beside the mesh stages
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int dev = 0;
     bdkx::BatchPlan bp;
     int ept_min[3] = {8, 8, 8};
@@ -363,7 +376,8 @@ struct bdk_batch {
     hipError_t capture(int nsteps, hipGraph_t *g, hipGraphExec_t *x) {
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
-        for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = bdkx::batch_enqueue_step(bp, stream);
+        for (int i = 0; i < nsteps && e == hipSuccess; ++i)
+            e = bdkx::batch_enqueue_step(bp, stream, aux, ev_fork, ev_join);
         hipGraph_t gg = nullptr;
         hipError_t e2 = hipStreamEndCapture(stream, &gg);
         if (e == hipSuccess) e = e2;
@@ -482,6 +496,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     g.f2_f1 = p->f2_f1;
     g.T = p->T;
     g.dt_max = p->dt_max;
+    g.sq_fixed = std::sqrt(2.0 * p->T * p->dt_max);
     g.mrd = p->max_travel;
     g.noise_var = p->noise_var;
     g.lam2inv = 1.0 / (p->lambda * p->lambda);
@@ -507,6 +522,9 @@ int bdk_create(const bd_params *p, bdk_t **out) {
         if (ee != hipSuccess) rc = hipfail(nullptr, ee, #expr);    \
     }
     ALLOC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    ALLOC(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+    ALLOC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    ALLOC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
     ALLOC(dalloc(d.cell, N));
     ALLOC(dalloc(d.rel, N));
     ALLOC(dalloc(d.pid, N));
@@ -524,7 +542,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.lrel, N + 1));
     ALLOC(dalloc(d.lpid, N + 1));
     ALLOC(dalloc(d.lrelf, N + 1));
-    ALLOC(dalloc(d.bsum, 4096));
+    ALLOC(dalloc(d.bcount, 4096 * 32));  // one entry per 4096 cells, 128 bytes apart
     ALLOC(dalloc_v(d.rho, Z * Z * R));
     ALLOC(dalloc_v(d.ht, (Z / 2) * Z * 2 * R));
     ALLOC(dalloc_v(d.gt, (Z / 2) * Z * 4 * R));
@@ -541,7 +559,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
 #undef ALLOC
     if (rc != BDK_OK) {
         free_dev(h);
-        if (h->stream) (void)hipStreamDestroy(h->stream);
+        release_streams(h);
         delete h;
         return rc;
     }
@@ -560,7 +578,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     if (e != hipSuccess) {
         rc = hipfail(nullptr, e, "initialising device state");
         free_dev(h);
-        (void)hipStreamDestroy(h->stream);
+        release_streams(h);
         delete h;
         return rc;
     }
@@ -568,7 +586,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     if (rc != BDK_OK) {
         g_create_error = h->err;
         free_dev(h);
-        (void)hipStreamDestroy(h->stream);
+        release_streams(h);
         delete h;
         return rc;
     }
@@ -585,9 +603,10 @@ void bdk_destroy(bdk_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->aux) (void)hipStreamSynchronize(h->aux);
     drop_graph(h);
     free_dev(h);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    release_streams(h);
     delete h;
 }
 
@@ -732,6 +751,48 @@ int bdk_step(bdk_t *h, double tmax, double *dt_out) {
         *dt_out = h->p.dt_max;
     }
     return BDK_OK;
+}
+
+int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out) {
+    if (!h || !out) return fail(h, BDK_ERR_ARG, "null argument");
+    if (!(tmax > 0.0)) return fail(h, BDK_ERR_ARG, "Fehler: Negative Zeitschrittweite gefordert (tmax <= 0)");
+    int rc = ensure(h);
+    if (rc) return rc;
+    bdkx::StepInfoDev *dinfo = nullptr;
+    HIPCHK(h, hipMalloc((void **)&dinfo, sizeof(bdkx::StepInfoDev)));
+    bdkx::StepInfoDev hi;
+    hipError_t e = hipMemsetAsync(dinfo, 0, sizeof hi, h->stream);
+    if (e == hipSuccess && h->g.idgas)
+        e = hipMemsetAsync(h->d.fmesh, 0, (size_t)h->g.Z * h->g.Z * 2 * real_size(h->p), h->stream);
+    if (e == hipSuccess) e = bdkx::enqueue_step(h, nullptr, nullptr, dinfo);
+    DevScalars sc;
+    if (e == hipSuccess) e = hipMemcpyAsync(&hi, dinfo, sizeof hi, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&sc, h->d.sc, sizeof sc, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dinfo);
+    if (e != hipSuccess) return hipfail(h, e, "bdk_step_info");
+    h->host_step++;
+    const int n1 = h->p.n1;
+    auto who = [&](int pid, int32_t *type, int32_t *index) {
+        *type = pid < n1 ? 1 : 2;
+        *index = pid < n1 ? pid : pid - n1;
+    };
+    out->num_wca = (int64_t)hi.num_wca;
+    out->max_wca[0] = hi.wca[0];
+    out->max_wca[1] = hi.wca[1];
+    who(hi.pid[0], &out->max_wca_type, &out->max_wca_index);
+    out->max_wca_nlist[0] = hi.nlist[0];
+    out->max_wca_nlist[1] = hi.nlist[1];
+    out->max_kap[0] = hi.kap[0];
+    out->max_kap[1] = hi.kap[1];
+    who(hi.pid[1], &out->max_kap_type, &out->max_kap_index);
+    out->max_rnd[0] = hi.rnd[0];
+    out->max_rnd[1] = hi.rnd[1];
+    who(hi.pid[2], &out->max_rnd_type, &out->max_rnd_index);
+    out->dt = h->g.adaptive ? dt_from_bits(sc.dt_last) : h->p.dt_max;
+    out->cell_changes[0] = hi.moves[0];
+    out->cell_changes[1] = hi.moves[1];
+    return check_nonfinite(h);
 }
 
 int bdk_run_steps(bdk_t *h, int64_t k, double *t_out) {
@@ -912,7 +973,7 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
         Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.pid, N * 4},
                        {nullptr, h->d.rank, N * 4},
                        {nullptr, h->d.count, nc2 * 4}, {nullptr, h->d.arr, nc2 * 4},
-                       {nullptr, h->d.sc, sizeof(DevScalars)}};
+                       {nullptr, h->d.bcount, 4096 * 32 * 4}, {nullptr, h->d.sc, sizeof(DevScalars)}};
         for (Snap sn : list) {
             HIPCHK(h, hipMalloc(&sn.p, sn.n));
             HIPCHK(h, hipMemcpyAsync(sn.p, sn.src, sn.n, hipMemcpyDeviceToDevice, h->stream));
@@ -1467,6 +1528,9 @@ int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {
     int rc = BDK_OK;
     hipError_t e = hipSetDevice(b->dev);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     int ept[3];
     for (int k = 0; k < 3; ++k) ept[k] = bdkx::batch_min_ept(k, zmax);
     if (e == hipSuccess) e = b->rebuild(ept);
@@ -1483,8 +1547,12 @@ void bdk_batch_destroy(bdk_batch_t *b) {
     if (!b) return;
     (void)hipSetDevice(b->dev);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->aux) (void)hipStreamSynchronize(b->aux);
     b->drop_graphs();
     b->free_tables();
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->aux) (void)hipStreamDestroy(b->aux);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -32,6 +32,7 @@ struct Geo {
     double maxpair;
     double f2_f1;
     double T, dt_max, mrd, noise_var;
+    double sq_fixed;  // sqrt(2 T dt_max): the noise amplitude at fixed dt
     double lam2inv;  // 1/(lambda*lambda)
     double kap;
     int zero_mode_g; // G(0,0) = 0 when lambda > 0.2 L (dm:1389)
@@ -74,7 +75,7 @@ struct DevBufs {
     double2 *lrel = nullptr;
     int *lpid = nullptr;
     float2 *lrelf = nullptr;  // FP32 rel - B/2 of list entry q (FP32 WCA)
-    int *bsum = nullptr;      // scan block sums
+    int *bcount = nullptr;    // particles per block of 4096 cells, kept current by the steps
     // mesh
     void *rho = nullptr;      // Z*Z real
     void *ht = nullptr;       // (Z/2)*Z complex, transposed half spectrum
@@ -129,6 +130,8 @@ struct bdk {
     bdkx::Geo g;
     bdkx::DevBufs d;
     hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;            // WCA runs here, beside the mesh chain
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int dev = 0;
     bool pos_set[2] = {false, false};
     bool dev_valid = false;       // device particle state uploaded
@@ -146,7 +149,7 @@ struct bdk {
 namespace bdkx {
 
 // Launchers implemented in bdk_kernels.hip.  All enqueue on h->stream.
-hipError_t launch_wca(bdk_t *h);                  // WCA forces -> fwca
+hipError_t launch_wca(bdk_t *h, hipStream_t st = nullptr);  // WCA forces -> fwca (default: h->stream)
 hipError_t launch_deposit(bdk_t *h, int reset);   // charges -> rho (reset: re-arm per-step scalars)
 hipError_t launch_rearm(bdk_t *h);                // per-step scalars when no deposit runs
 size_t deposit_lds_bytes(const bdk_t *h);          // LDS of one deposit block
@@ -172,10 +175,22 @@ int batch_min_ept(int stage, int zmax);
 hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
                        std::vector<int2> maps[BS_COUNT]);
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st);
-hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st);
+hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux = nullptr,
+                              hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
+
+// Counters of one step (TimestepInfo, signaturen2.h:71-145), device side.
+struct StepInfoDev {
+    unsigned long long key[3];   // max |F|^2 (float bits) << 32 | list position: WCA, capillary, noise
+    unsigned long long num_wca;  // candidate pairs i != j in the 3x3 cells
+    int moves[2];                // particles that changed cell, per species
+    int pid[3];                  // particle ids of the three maxima
+    int nlist[2];                // species counts in the 3x3 cells of the WCA maximum
+    double wca[2], kap[2], rnd[2];
+};
 
 // one full step as a sequence of launches; names (if non-null) get the
-// kernel labels in launch order
-hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev);
+// kernel labels in launch order; info (if non-null) receives the counters
+hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev,
+                        StepInfoDev *info = nullptr);
 
 }  // namespace bdkx

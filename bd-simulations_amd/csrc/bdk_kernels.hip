@@ -324,6 +324,17 @@ constexpr unsigned LIST_SLOT = 0x7fffffffu;
 // rank of a particle that changed cell during the step: its arrival index
 // in the new cell, marked (ranks stay below 2^30)
 constexpr int RANK_MOVED = 1 << 30;
+// Scan of the cell counts: blocks of SCAN_BLOCK cells, SCAN_PER_THREAD
+// consecutive cells per thread; bcount[c >> SCAN_SHIFT] is the particle count
+// of the block holding cell c
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_SHIFT = 12;
+constexpr int SCAN_BLOCK = 1 << SCAN_SHIFT;  // 4096 cells per block
+constexpr int SCAN_PER_THREAD = SCAN_BLOCK / SCAN_THREADS;
+static_assert(SCAN_PER_THREAD % 4 == 0, "int4 loads");
+// bcount entries sit 128 bytes apart: the movers' atomics on the blocks of
+// a dense region then spread over many L2 lines instead of a few
+constexpr int BCOUNT_STRIDE = 32;
 
 template <typename R>
 __device__ __forceinline__ void wca_ref_body(const Geo &g, const DevBufs &d, int blk) {
@@ -612,7 +623,7 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
             }
             const int qa = d.start[row + ya];
             seg_q[i] = qa;
-            seg_pre[i] = d.start[row + yb] - qa;
+            seg_pre[i] = max(d.start[row + yb] - qa, 0);
             seg_row[i] = gx;
         }
         __syncthreads();
@@ -1123,7 +1134,7 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     }
     const double2 n = particle_noise(g, d, pid, step);
     const double dt = g.adaptive ? dbits(d.sc->dt_bits) : g.dt_max;
-    const double sq = sqrt(2.0 * g.T * dt);
+    const double sq = g.adaptive ? sqrt(2.0 * g.T * dt) : g.sq_fixed;  // same expression, correctly rounded
     const double dx = dt * (fwx + fkx) + sq * n.x;
     const double dy = dt * (fwy + fky) + sq * n.y;
     double2 rn;
@@ -1166,6 +1177,10 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
         // from arr; particles that stay are ranked by k_rank without atomics
         atomicSub(&d.count[c], 1);
         atomicAdd(&d.count[nc_new], 1);
+        if ((c >> SCAN_SHIFT) != (nc_new >> SCAN_SHIFT)) {
+            atomicSub(&d.bcount[(c >> SCAN_SHIFT) * BCOUNT_STRIDE], 1);
+            atomicAdd(&d.bcount[(nc_new >> SCAN_SHIFT) * BCOUNT_STRIDE], 1);
+        }
         d.rank[s] = RANK_MOVED | atomicAdd(&d.arr[nc_new], 1);
     }
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
@@ -1228,13 +1243,11 @@ __global__ void k_noise_out(Geo g, DevBufs d) {
 }
 
 // ------------------------------------------------------------------ scan + fill
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_PER_THREAD = 16;
-constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_PER_THREAD;  // 4096 cells per block
-
-__device__ __forceinline__ void scan_reduce_body(const int *__restrict__ count, int ncell, int *__restrict__ bsum,
+// Full recount of bcount from the cell counts (after an upload or a
+// refresh; during steps the update keeps bcount current).
+__device__ __forceinline__ void scan_reduce_body(const int *__restrict__ count, int ncell, int *__restrict__ bcount,
                                                  int blk) {
-    __shared__ int ws[16];
+    __shared__ int ws[SCAN_THREADS / 64];
     const int base = blk * SCAN_BLOCK;
     int s = 0;
     for (int q = threadIdx.x; q < SCAN_BLOCK; q += SCAN_THREADS) {
@@ -1248,81 +1261,91 @@ __device__ __forceinline__ void scan_reduce_body(const int *__restrict__ count, 
     if (threadIdx.x == 0) {
         int t = 0;
         for (int w = 0; w < SCAN_THREADS / 64; ++w) t += ws[w];
-        bsum[blk] = t;
+        bcount[blk * BCOUNT_STRIDE] = t;
     }
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *__restrict__ count, int ncell,
-                                                              int *__restrict__ bsum) {
-    scan_reduce_body(count, ncell, bsum, blockIdx.x);
+                                                              int *__restrict__ bcount) {
+    scan_reduce_body(count, ncell, bcount, blockIdx.x);
 }
 
-// exclusive scan of the block sums (one block of 1024 threads, nb <= 4096);
-// also closes the step: start[ncell] = total, step counter += inc_step
-__device__ __forceinline__ void scan_top_body(int *bsum, int nb, int *start, int ncell, DevScalars *sc,
-                                              int inc_step) {
-    __shared__ int a[4096];
-    __shared__ int ws[16];
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) a[i] = bsum[i];
+// Exclusive scan of the cell counts into start[] in one pass: a block scans
+// its SCAN_BLOCK cells (int4 loads, wave scans, one LDS exchange) and adds
+// the particles of the blocks before it, summed from bcount.  The last
+// thread of the last block closes the step: start[ncell] = total, step
+// counter += inc_step.
+__device__ __forceinline__ void scan_body(const int *__restrict__ count, const int *__restrict__ bcount,
+                                          int *__restrict__ start, int ncell, int nb, int nmax, DevScalars *sc,
+                                          int inc_step, int blk) {
+    constexpr int NW = SCAN_THREADS / 64;
+    __shared__ int ws[2 * NW];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int p = 0;
+    for (int i = tid; i < blk; i += SCAN_THREADS) p += bcount[i * BCOUNT_STRIDE];
+    const int base = blk * SCAN_BLOCK + tid * SCAN_PER_THREAD;
+    const bool full = base + SCAN_PER_THREAD <= ncell;
+    int loc[SCAN_PER_THREAD];
+    if (full) {
+        const int4 *c4 = (const int4 *)(count + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD / 4; ++k) {
+            const int4 v = c4[k];
+            loc[4 * k] = v.x;
+            loc[4 * k + 1] = v.y;
+            loc[4 * k + 2] = v.z;
+            loc[4 * k + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < SCAN_PER_THREAD; ++q) loc[q] = base + q < ncell ? count[base + q] : 0;
+    }
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER_THREAD; ++q) s += loc[q];
+    int x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) p += __shfl_xor(p, o, 64);
+    if (lane == 63) ws[w] = x;
+    if (lane == 0) ws[NW + w] = p;
     __syncthreads();
-    int total = block_exscan4(a, nb, ws);
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) bsum[i] = a[i];
-    if (threadIdx.x == 0) {
-        start[ncell] = total;
+    int e = x - s;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) e += ws[NW + k] + (k < w ? ws[k] : 0);
+    // starts are clamped into [0, total of the recount]: list ranges stay
+    // inside the arrays even if the counts were inconsistent (then flagged)
+    int out[SCAN_PER_THREAD];
+#pragma unroll
+    for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+        out[q] = min(max(e, 0), nmax);
+        e += loc[q];
+    }
+    if (full) {
+        int4 *s4 = (int4 *)(start + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD / 4; ++k)
+            s4[k] = make_int4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < SCAN_PER_THREAD; ++q)
+            if (base + q < ncell) start[base + q] = out[q];
+    }
+    if (blk == nb - 1 && tid == SCAN_THREADS - 1) {
+        start[ncell] = min(max(e, 0), nmax);
+        if (e != nmax) atomicOr(&sc->nonfinite, 2);
         if (inc_step) sc->step += 1;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_top(int *bsum, int nb, int *start, int ncell, DevScalars *sc,
-                                                   int inc_step) {
-    scan_top_body(bsum, nb, start, ncell, sc, inc_step);
-}
-
-__device__ __forceinline__ void scan_down_body(const int *__restrict__ count, const int *__restrict__ bsum,
-                                               int *__restrict__ start, int ncell, int blk) {
-    __shared__ int a[SCAN_BLOCK];
-    __shared__ int ws[16];
-    const int base = blk * SCAN_BLOCK;
-    for (int q = threadIdx.x; q < SCAN_BLOCK; q += SCAN_THREADS) {
-        int i = base + q;
-        a[q] = i < ncell ? count[i] : 0;
-    }
-    __syncthreads();
-    // 16 consecutive elements per thread
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    int loc[SCAN_PER_THREAD];
-    int s = 0;
-#pragma unroll
-    for (int q = 0; q < SCAN_PER_THREAD; ++q) {
-        loc[q] = a[tid * SCAN_PER_THREAD + q];
-        s += loc[q];
-    }
-    int x = s;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) ws[w] = x;
-    __syncthreads();
-    int woff = bsum[blk];
-    for (int q = 0; q < w; ++q) woff += ws[q];
-    int e = woff + x - s;
-#pragma unroll
-    for (int q = 0; q < SCAN_PER_THREAD; ++q) {
-        a[tid * SCAN_PER_THREAD + q] = e;
-        e += loc[q];
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < SCAN_BLOCK; q += SCAN_THREADS) {
-        int i = base + q;
-        if (i < ncell) start[i] = a[q];
-    }
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const int *__restrict__ count, const int *__restrict__ bsum,
-                                                            int *__restrict__ start, int ncell) {
-    scan_down_body(count, bsum, start, ncell, blockIdx.x);
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *__restrict__ count, const int *__restrict__ bcount,
+                                                       int *__restrict__ start, int ncell, int nb, int nmax,
+                                                       DevScalars *sc, int inc_step) {
+    scan_body(count, bcount, start, ncell, nb, nmax, sc, inc_step, blockIdx.x);
 }
 
 // Rank of a particle that stayed in its cell during the step: its position
@@ -1354,6 +1377,10 @@ __device__ __forceinline__ void fill_body(const Geo &g, const DevBufs &d, int mo
     const int rk = d.rank[s];
     int pos = d.start[c] + (rk & (RANK_MOVED - 1));
     if (moved_ranks && (rk & RANK_MOVED)) pos += d.start[c + 1] - d.start[c] - d.arr[c];
+    if ((unsigned)pos >= (unsigned)g.N) {  // inconsistent counts: flagged, nothing written
+        atomicOr(&d.sc->nonfinite, 2);
+        return;
+    }
     const double2 r = d.rel[s];
     ((unsigned *)d.list)[pos] = (unsigned)s | ((pid >= g.n1 ? 1u : 0u) << 31);
     d.lcell[pos] = c;
@@ -1378,7 +1405,7 @@ __device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, 
     const int b = d.start[c], n = d.start[c + 1] - b;
     const int lo = moved_ranks ? n - d.arr[c] : 0;
     d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
-    if (n - lo < 2) return;
+    if (lo < 0 || n - lo < 2) return;  // (lo < 0: inconsistent counts, flagged by the scan)
     unsigned *l = (unsigned *)d.list + b;
     double2 *rl = d.lrel + b;
     int *pl = d.lpid + b;
@@ -1482,22 +1509,10 @@ __global__ void kb_rank(const RepDev *m, const int2 *map) {
     rank_body(m[t.x].g, m[t.x].d, t.y);
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_reduce(const RepDev *m, const int2 *map) {
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan(const RepDev *m, const int2 *map) {
     const int2 t = map[blockIdx.x];
     const RepDev &r = m[t.x];
-    scan_reduce_body(r.d.count, r.g.ncell, r.d.bsum, t.y);
-}
-
-// one block per member
-__global__ __launch_bounds__(1024) void kb_scan_top(const RepDev *m) {
-    const RepDev &r = m[blockIdx.x];
-    scan_top_body(r.d.bsum, r.scan_nb, r.d.start, r.g.ncell, r.d.sc, 1);
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan_down(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    scan_down_body(r.d.count, r.d.bsum, r.d.start, r.g.ncell, t.y);
+    scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 1, t.y);
 }
 
 __global__ void kb_fill(const RepDev *m, const int2 *map) {
@@ -1613,21 +1628,22 @@ hipError_t set_kernel_attributes() {
     return e;
 }
 
-hipError_t launch_wca(bdk_t *h) {
+hipError_t launch_wca(bdk_t *h, hipStream_t st) {
     // FP32 forces use the cell-relative displacement when the box holds at
     // least 6 cells per side (the neighbour offset is then the minimum image)
     const dim3 grid(nblk(h->g.N, 256)), blk(256);
+    if (!st) st = h->stream;
     if (h->g.fast_wca) {
         const bool unit = h->p.sigma11_22 == 1.0 && h->p.sigma11_12 == 1.0 && h->p.eps22_11 == 1.0 &&
                           h->p.eps12_11 == 1.0;
         if (unit)
-            hipLaunchKernelGGL(k_wca_f32<true>, grid, blk, 0, h->stream, h->g, h->d);
+            hipLaunchKernelGGL(k_wca_f32<true>, grid, blk, 0, st, h->g, h->d);
         else
-            hipLaunchKernelGGL(k_wca_f32<false>, grid, blk, 0, h->stream, h->g, h->d);
+            hipLaunchKernelGGL(k_wca_f32<false>, grid, blk, 0, st, h->g, h->d);
     } else if (h->p.precision == BDK_FP32) {
-        hipLaunchKernelGGL(k_wca<float>, grid, blk, 0, h->stream, h->g, h->d);
+        hipLaunchKernelGGL(k_wca<float>, grid, blk, 0, st, h->g, h->d);
     } else {
-        hipLaunchKernelGGL(k_wca<double>, grid, blk, 0, h->stream, h->g, h->d);
+        hipLaunchKernelGGL(k_wca<double>, grid, blk, 0, st, h->g, h->d);
     }
     return hipGetLastError();
 }
@@ -1776,11 +1792,9 @@ hipError_t launch_update(bdk_t *h, int fused) {
 static void launch_scan_fill_inner(bdk_t *h, int inc_step) {
     const int ncell = h->g.ncell;
     const int nb = (int)nblk(ncell, SCAN_BLOCK);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, ncell, h->d.bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, h->stream, h->d.bsum, nb, h->d.start, ncell, h->d.sc,
-                       inc_step);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bsum, h->d.start,
-                       ncell);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, ncell, h->d.bcount);
+    hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start, ncell,
+                       nb, h->g.N, h->d.sc, inc_step);
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 0);
 }
@@ -2060,9 +2074,93 @@ hipError_t launch_radial(bdk_t *h, int nbins, double dr, double *part, double *c
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ step counters
+// TimestepInfo (signaturen2.h:71-145, filled by zeitschritt_debug,
+// dm:168-211): the largest force of each kind by |F|^2 with its particle,
+// the number of candidate pairs the cell lists present, and the particles
+// that changed cell.  Maxima are 64-bit keys (|F|^2 as FP32 bits, list
+// position) under atomicMax, so the result does not depend on the order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_info_forces(Geo g, DevBufs d, StepInfoDev *info) {
+    using R2 = typename V2<R>::T;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cand = 0;
+    if (q < g.N) {
+        const R2 fw = ((const R2 *)d.fwca)[q], fk = ((const R2 *)d.fkap)[q];
+        const double2 n = particle_noise(g, d, d.lpid[q], d.sc->step);
+        const double v[3] = {(double)fw.x * fw.x + (double)fw.y * fw.y, (double)fk.x * fk.x + (double)fk.y * fk.y,
+                             n.x * n.x + n.y * n.y};
+        for (int k = 0; k < 3; ++k) {
+            const unsigned long long key = ((unsigned long long)__float_as_uint((float)v[k]) << 32) | (unsigned)q;
+            atomicMax(&info->key[k], key);
+        }
+        // as the reference counts them: a species-1 particle against both
+        // lists, a species-2 particle against list 2 (the 12 pairs once)
+        const bool s2 = d.lpid[q] >= g.n1;
+        const int c = d.lcell[q], cx = div_nc(c, g), cy = c - cx * g.nc;
+        for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b) {
+                const int cc = wrap1(cx + a, g.nc) * g.nc + wrap1(cy + b, g.nc);
+                if (!s2) {
+                    cand += d.start[cc + 1] - d.start[cc];
+                } else {
+                    for (int j = d.start[cc]; j < d.start[cc + 1]; ++j)
+                        cand += ((const unsigned *)d.list)[j] >> 31;
+                }
+            }
+        cand -= 1;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cand += __shfl_xor(cand, o, 64);
+    if ((threadIdx.x & 63) == 0 && cand) atomicAdd(&info->num_wca, cand);
+}
+
+// the vectors and particles of the maxima (one thread), before the update
+template <typename R>
+__global__ void k_info_fetch(Geo g, DevBufs d, StepInfoDev *info) {
+    using R2 = typename V2<R>::T;
+    for (int k = 0; k < 3; ++k) {
+        const int q = (int)(info->key[k] & 0xffffffffull);
+        info->pid[k] = d.lpid[q];
+        if (k == 0) {
+            const R2 f = ((const R2 *)d.fwca)[q];
+            info->wca[0] = f.x;
+            info->wca[1] = f.y;
+            const int c = d.lcell[q], cx = div_nc(c, g), cy = c - cx * g.nc;
+            int n1 = 0, n2 = 0;
+            for (int a = -1; a <= 1; ++a)
+                for (int b = -1; b <= 1; ++b) {
+                    const int cc = wrap1(cx + a, g.nc) * g.nc + wrap1(cy + b, g.nc);
+                    for (int j = d.start[cc]; j < d.start[cc + 1]; ++j) {
+                        if (((const unsigned *)d.list)[j] >> 31) ++n2;
+                        else ++n1;
+                    }
+                }
+            info->nlist[0] = n1;
+            info->nlist[1] = n2;
+        } else if (k == 1) {
+            const R2 f = ((const R2 *)d.fkap)[q];
+            info->kap[0] = f.x;
+            info->kap[1] = f.y;
+        } else {
+            const double2 n = particle_noise(g, d, d.lpid[q], d.sc->step);
+            info->rnd[0] = n.x;
+            info->rnd[1] = n.y;
+        }
+    }
+}
+
+// after the update, before the list rebuild: cell changes per species
+__global__ void k_info_moves(Geo g, DevBufs d, StepInfoDev *info) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.N) return;
+    if (d.cell[q] != d.lcell[q]) atomicAdd(&info->moves[d.lpid[q] >= g.n1 ? 1 : 0], 1);
+}
+
 // One step.  With ev != nullptr an event is recorded after every launch
 // (profiling); names receives the labels.
-hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev) {
+hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev,
+                        StepInfoDev *info) {
     hipError_t e = hipSuccess;
     auto mark = [&](const char *nm) {
         if (names) names->push_back(nm);
@@ -2079,9 +2177,20 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     };
     if (ev) mark("begin");
     const bool kap = !h->g.idgas;
+    // the WCA forces and the mesh chain (deposit, FFTs) only read the
+    // particle state: the WCA runs on a second stream beside the chain
+    // (fork and join by events; in a captured graph they are branches)
+    const bool fork = kap && !ev && h->aux;
     if (kap) {
-        chk(launch_wca(h));
-        mark("k_wca");
+        if (fork) {
+            chk(hipEventRecord(h->ev_fork, h->stream));
+            chk(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
+            chk(launch_wca(h, h->aux));
+            chk(hipEventRecord(h->ev_join, h->aux));
+        } else {
+            chk(launch_wca(h));
+            mark("k_wca");
+        }
         chk(launch_deposit(h, 1));
         mark("k_deposit");
         chk(launch_fft_pass(h, 0));
@@ -2090,9 +2199,22 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         mark("k_col");
         chk(launch_fft_pass(h, 2));
         mark("k_c2r_rows");
+        if (fork) chk(hipStreamWaitEvent(h->stream, h->ev_join, 0));
     } else {
         chk(launch_rearm(h));
         mark("k_rearm");
+    }
+    if (info) {
+        // capillary forces on their own (the step fuses them into the update)
+        if (kap && !h->g.adaptive) chk(launch_gather(h, 0));
+        const bool f32 = h->p.precision == BDK_FP32;
+        if (f32) {
+            hipLaunchKernelGGL(k_info_forces<float>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
+            hipLaunchKernelGGL(k_info_fetch<float>, dim3(1), dim3(1), 0, h->stream, h->g, h->d, info);
+        } else {
+            hipLaunchKernelGGL(k_info_forces<double>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
+            hipLaunchKernelGGL(k_info_fetch<double>, dim3(1), dim3(1), 0, h->stream, h->g, h->d, info);
+        }
     }
     if (h->g.adaptive) {
         chk(launch_gather(h, 1));
@@ -2102,17 +2224,15 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         chk(launch_update(h, kap ? 1 : 0));
     }
     mark("k_update");
+    if (info) hipLaunchKernelGGL(k_info_moves, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
     hipLaunchKernelGGL(k_rank, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
     mark("k_rank");
     const int ncell = h->g.ncell;
     const int nb = (int)nblk(ncell, SCAN_BLOCK);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, ncell, h->d.bsum);
-    mark("k_scan_reduce");
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, h->stream, h->d.bsum, nb, h->d.start, ncell, h->d.sc, 1);
-    mark("k_scan_top");
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bsum, h->d.start,
-                       ncell);
-    mark("k_scan_down");
+    // bcount is current (the update adjusted it with the counts)
+    hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start, ncell,
+                       nb, h->g.N, h->d.sc, 1);
+    mark("k_scan");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
@@ -2211,9 +2331,7 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
         break;
     default: {
         hipLaunchKernelGGL(kb_rank, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
-        hipLaunchKernelGGL(kb_scan_reduce, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
-        hipLaunchKernelGGL(kb_scan_top, dim3(bp.nmem), dim3(1024), 0, st, m);
-        hipLaunchKernelGGL(kb_scan_down, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
+        hipLaunchKernelGGL(kb_scan, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
         hipLaunchKernelGGL(kb_fill, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         hipLaunchKernelGGL(kb_sort_cells, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
         break;
@@ -2230,12 +2348,23 @@ hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st) {
-    for (int k = 0; k <= 6; ++k) {
-        hipError_t e = batch_stage(bp, k, st);
-        if (e != hipSuccess) return e;
+// with aux (and the two events): the WCA stage runs on aux beside the mesh
+// stages, as in the single-simulation step
+hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
+                              hipEvent_t join) {
+    hipError_t e = hipSuccess;
+    if (aux) {
+        e = hipEventRecord(fork, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+        if (e == hipSuccess) e = batch_stage(bp, 0, aux);
+        if (e == hipSuccess) e = hipEventRecord(join, aux);
+    } else {
+        e = batch_stage(bp, 0, st);
     }
-    return hipSuccess;
+    for (int k = 1; k <= 4 && e == hipSuccess; ++k) e = batch_stage(bp, k, st);
+    if (aux && e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    for (int k = 5; k <= 6 && e == hipSuccess; ++k) e = batch_stage(bp, k, st);
+    return e;
 }
 
 }  // namespace bdkx

@@ -199,6 +199,29 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
    header on nr == 0 with truncate, then one block per checkpoint. */
 int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
 
+/* One step like bdk_step that also fills the counters of TimestepInfo
+   (signaturen2.h:71-145, zeitschritt_debug dm:168-211): the largest WCA,
+   capillary and random force of the step by |F|^2 (vector, species 1/2,
+   index within the species), the species counts of the 3x3 cells around
+   the WCA maximum, the number of candidate pairs i != j the cell lists
+   present (as dm:704-1078 counts them: species-1 particles against both
+   lists, species-2 particles against list 2), the step's dt, and the
+   particles that changed cell per species.
+   Ties in |F|^2 at FP32 resolution go to the later list position. */
+typedef struct {
+    int64_t num_wca;
+    double max_wca[2];
+    int32_t max_wca_type, max_wca_index;
+    int32_t max_wca_nlist[2];
+    double max_kap[2];
+    int32_t max_kap_type, max_kap_index;
+    double max_rnd[2];
+    int32_t max_rnd_type, max_rnd_index;
+    double dt;
+    int32_t cell_changes[2];
+} bdk_step_info_t;
+int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out);
+
 /* ---------------------------------------------------------------- restart
    init_pos_aus_datei (dm:1576-1643) with its P8 defect fixed (rel = x - git*B):
    reads startpos.txt-format files (two header lines, then "nr typ x y" for
