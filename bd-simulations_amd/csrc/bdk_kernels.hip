@@ -718,13 +718,24 @@ __device__ __forceinline__ void dft8(const C *v, C *y, bool inv) {
     }
 }
 
+// LDS layout of the FFT rows: element i of a row sits at lp(i), one padding
+// element after every 8.  The output strides of the first Stockham passes
+// (8 and 64 elements) then spread over the banks instead of piling onto a
+// few (measured: half the LDS cycles of the unpadded passes were bank
+// conflicts).  A row of n elements takes n + n/8 + 1 (odd) elements.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 3); }
+__host__ __device__ __forceinline__ int lds_row(int n) { return n + n / 8 + 1; }
+
 // One Stockham pass of radix RAD (2, 4 or 8) over nrows rows of length
-// n = 2^logn held in LDS (row stride `stride`), in place: every thread reads
+// n = 2^logn held in LDS (row stride `stride`, layout lp), in place: every thread reads
 // its butterflies' inputs (and, for small register footprints, their
 // twiddles: powers 1, 2, 4 from the FP64-built table, 3, 5, 6, 7 as
 // products) before the barrier and writes the outputs after it.  Twiddle of input t of butterfly j: tw[(t k) << twsh],
 // k = j mod Ns (conjugated for the inverse transform).
-template <typename R, int EPT, int RAD>
+// Output index ((j - k) << LR) + k + t Ns with Ns a power of 8 maps to
+// lp(base) + t (Ns + Ns/8); input index j + t nb to lp(j) + t (nb + nb/8)
+// when nb is a multiple of 8 (SMALL: nb < 8, mapped per element).
+template <typename R, int EPT, int RAD, bool SMALL>
 __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int nrows, int stride, int logNs,
                                          const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     using C = typename V2<R>::T;
@@ -736,6 +747,7 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
     const int tot = nb * nrows;
     const int Ns = 1 << logNs;
     const int twsh = logZ - (logNs + LR);
+    const int nbp = nb + (nb >> 3), Nsp = Ns + (Ns >> 3);
     C v[NB][RAD];
     C w[PRE ? NB : 1][RAD - 1];
 #pragma unroll
@@ -743,9 +755,15 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
         const int b = tid + e * nt;
         if (b < tot) {
             const int row = b >> lognb, j = b & (nb - 1);
-            const C *p = buf + row * stride + j;
+            const C *p = buf + row * stride;
+            if (SMALL) {
 #pragma unroll
-            for (int t = 0; t < RAD; ++t) v[e][t] = p[t * nb];
+                for (int t = 0; t < RAD; ++t) v[e][t] = p[lp(j + t * nb)];
+            } else {
+                p += lp(j);
+#pragma unroll
+                for (int t = 0; t < RAD; ++t) v[e][t] = p[t * nbp];
+            }
             if (PRE) {
                 // tw^1, tw^2, tw^4 from the table, the other powers as products
                 // (one to two roundings deeper; 3 loads instead of 7 for radix 8)
@@ -789,9 +807,9 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
             } else {
                 dft8(v[e], y, inv);
             }
-            C *q = buf + row * stride + ((j - k) << LR) + k;
+            C *q = buf + row * stride + lp(((j - k) << LR) + k);
 #pragma unroll
-            for (int t = 0; t < RAD; ++t) q[t * Ns] = y[t];
+            for (int t = 0; t < RAD; ++t) q[t * Nsp] = y[t];
         }
     }
     __syncthreads();
@@ -802,14 +820,25 @@ template <typename R, int EPT>
 __device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int logn, int nrows, int stride,
                         const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     int logNs = 0;
+    const bool small8 = logn - 3 < 3;  // nb = n/8 below 8
     while (logn - logNs >= 3) {
-        fft_pass<R, EPT, 8>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+        if (small8)
+            fft_pass<R, EPT, 8, true>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+        else
+            fft_pass<R, EPT, 8, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
         logNs += 3;
     }
-    if (logn - logNs == 2)
-        fft_pass<R, EPT, 4>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
-    else if (logn - logNs == 1)
-        fft_pass<R, EPT, 2>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+    if (logn - logNs == 2) {
+        if (logn - 2 < 3)
+            fft_pass<R, EPT, 4, true>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+        else
+            fft_pass<R, EPT, 4, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+    } else if (logn - logNs == 1) {
+        if (logn - 1 < 3)
+            fft_pass<R, EPT, 2, true>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+        else
+            fft_pass<R, EPT, 2, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+    }
 }
 
 constexpr int FFT_THREADS = 256;
@@ -829,7 +858,7 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
     const int k0 = blk * RB;
     for (int idx = tid; idx < RB * n; idx += nt) {
         const int row = idx >> logn, m = idx & (n - 1);
-        s[row * stride + m] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
+        s[row * stride + lp(m)] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
     }
     __syncthreads();
     lds_fft<R, EPT>(s, logn, RB, stride, tw, logZ, false);
@@ -841,18 +870,18 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
             C z = p[0];
             p[0] = cm<C>(z.x + z.y, z.x - z.y);
         } else {
-            C a = p[m], bc = cconj(p[n - m]);
+            C a = p[lp(m)], bc = cconj(p[lp(n - m)]);
             C E = cscale(cadd(a, bc), (R)0.5);
             C O = cmul_mi(cscale(csub(a, bc), (R)0.5));
             C WO = cmul(tw[m], O);
-            p[m] = cadd(E, WO);
-            if (m != n - m) p[n - m] = cconj(csub(E, WO));
+            p[lp(m)] = cadd(E, WO);
+            if (m != n - m) p[lp(n - m)] = cconj(csub(E, WO));
         }
     }
     __syncthreads();
     for (int idx = tid; idx < RB * n; idx += nt) {
         const int row = idx % RB, m = idx / RB;
-        ht[(size_t)m * Z + k0 + row] = s[row * stride + m];
+        ht[(size_t)m * Z + k0 + row] = s[row * stride + lp(m)];
     }
 }
 
@@ -894,7 +923,7 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
     const int m0 = blk * RC;
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = idx >> logZ, k = idx & (Z - 1);
-        s[row * stride + k] = ht[(size_t)(m0 + row) * Z + k];
+        s[row * stride + lp(k)] = ht[(size_t)(m0 + row) * Z + k];
     }
     __syncthreads();
     lds_fft<R, EPT>(s, logZ, RC, stride, tw, logZ, false);
@@ -905,7 +934,7 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
             // packed (DC, Nyquist) column: pairs (j, Z-j) by one thread
             if (j > Zh) continue;
             const int jn = (Z - j) & (Z - 1);
-            C P = s[j], Q = s[jn];
+            C P = s[lp(j)], Q = s[lp(jn)];
             C Sdc = cscale(cadd(P, cconj(Q)), (R)0.5);
             C Sny = cmul_mi(cscale(csub(P, cconj(Q)), (R)0.5));
             R gx0, gy0, gxN, gyN, hx0, hy0, hxN, hyN;
@@ -918,25 +947,27 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
             C fyj = cadd(cmul_i(cscale(Sdc, gy0)), cmul_i(cmul_i(cscale(Sny, gyN))));
             C fxn = cadd(cmul_i(cscale(cconj(Sdc), hx0)), cmul_i(cmul_i(cscale(cconj(Sny), hxN))));
             C fyn = cadd(cmul_i(cscale(cconj(Sdc), hy0)), cmul_i(cmul_i(cscale(cconj(Sny), hyN))));
-            s[j] = fxj;
-            sB[j] = fyj;
+            s[lp(j)] = fxj;
+            sB[lp(j)] = fyj;
             if (jn != j) {
-                s[jn] = fxn;
-                sB[jn] = fyn;
+                s[lp(jn)] = fxn;
+                sB[lp(jn)] = fyn;
             }
         } else {
             R gx, gy;
             symbols<R>(j, m, g, sq, lap, gx, gy);
-            C S = s[row * stride + j];
-            s[row * stride + j] = cm<C>(-gx * S.y, gx * S.x);
-            sB[row * stride + j] = cm<C>(-gy * S.y, gy * S.x);
+            const int o = row * stride + lp(j);
+            C S = s[o];
+            s[o] = cm<C>(-gx * S.y, gx * S.x);
+            sB[o] = cm<C>(-gy * S.y, gy * S.x);
         }
     }
     __syncthreads();
     lds_fft<R, EPT>(s, logZ, 2 * RC, stride, tw, logZ, true);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = idx >> logZ, k = idx & (Z - 1);
-        C a = s[row * stride + k], b = sB[row * stride + k];
+        const int so = row * stride + lp(k);
+        C a = s[so], b = sB[so];
         C4 o;
         o.x = a.x;
         o.y = a.y;
@@ -975,17 +1006,17 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
         C *p = s + row * stride;
         if (m == 0) {
             p[0] = cm<C>(v.x, v.z);
-            p[Zh] = cm<C>(v.y, v.w);
+            p[lp(Zh)] = cm<C>(v.y, v.w);
         } else {
-            p[m] = cm<C>(v.x - v.w, v.y + v.z);
-            p[Z - m] = cm<C>(v.x + v.w, -v.y + v.z);
+            p[lp(m)] = cm<C>(v.x - v.w, v.y + v.z);
+            p[lp(Z - m)] = cm<C>(v.x + v.w, -v.y + v.z);
         }
     }
     __syncthreads();
     lds_fft<R, EPT>(s, logZ, RB2, stride, tw, logZ, true);
     for (int idx = tid; idx < RB2 * Z; idx += nt) {
         const int row = idx >> logZ, l = idx & (Z - 1);
-        fm[(size_t)(k0 + row) * Z + l] = s[row * stride + l];
+        fm[(size_t)(k0 + row) * Z + l] = s[row * stride + lp(l)];
     }
 }
 
@@ -1576,9 +1607,9 @@ template <typename R> static FftCfg fft_cfg(int Z) {
     c.RB = std::min(Z, c.e1 * FFT_THREADS / (Z / 2));
     c.RC = std::max(1, std::min(Z / 2, c.e2 * FFT_THREADS / (2 * Z)));
     c.RB2 = std::min(Z, c.e3 * FFT_THREADS / Z);
-    c.sR1 = Z / 2 + 1;
-    c.sC = Z + 1;
-    c.sR2 = Z + 1;
+    c.sR1 = lds_row(Z / 2);
+    c.sC = lds_row(Z);
+    c.sR2 = lds_row(Z);
     c.l1 = (size_t)c.RB * c.sR1 * sizeof(C);
     c.lc = (size_t)2 * c.RC * c.sC * sizeof(C);
     c.l2 = (size_t)c.RB2 * c.sR2 * sizeof(C);
@@ -1841,23 +1872,23 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col_rhok(const typename V2<R>::
     extern __shared__ unsigned char smem_raw[];
     C *s = (C *)smem_raw;
     const int m = blockIdx.x, tid = threadIdx.x, Zh = Z >> 1;
-    for (int k = tid; k < Z; k += blockDim.x) s[k] = ht[(size_t)m * Z + k];
+    for (int k = tid; k < Z; k += blockDim.x) s[lp(k)] = ht[(size_t)m * Z + k];
     double *p0 = part + (size_t)m * nbins, *pn = part + (size_t)Zh * nbins;
     for (int b = tid; b < nbins; b += blockDim.x) {
         p0[b] = 0.0;
         if (m == 0) pn[b] = 0.0;
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, logZ, 1, Z + 1, tw, logZ, false);
+    lds_fft<R, EPT>(s, logZ, 1, lds_row(Z), tw, logZ, false);
     if (tid != 0) return;
     for (int j = 0; j < Z; ++j) {
         const double xq = (double)(((j + Zh) & (Z - 1)) - Zh);
         const double sg = ((j + m) & 1) ? -1.0 : 1.0;
         if (m > 0) {
             const int bin = (int)(dqr * sqrt(xq * xq + (double)m * m) + 0.5);
-            if (bin < nbins) p0[bin] += 2.0 * sg * (double)s[j].x;
+            if (bin < nbins) p0[bin] += 2.0 * sg * (double)s[lp(j)].x;
         } else {
-            const C P = s[j], Q = s[(Z - j) & (Z - 1)];
+            const C P = s[lp(j)], Q = s[lp((Z - j) & (Z - 1))];
             const double dc = 0.5 * ((double)P.x + (double)Q.x);  // Re of the DC column
             const double ny = 0.5 * ((double)P.y + (double)Q.y);  // Re of the Nyquist column
             int bin = (int)(dqr * sqrt(xq * xq) + 0.5);
@@ -1880,7 +1911,7 @@ template <typename R> static void launch_rhok_t(bdk_t *h, double dqr, int nbins,
     using C = typename V2<R>::T;
     const int Z = h->g.Z, logZ = h->g.logZ;
     const int ept = ept_for(8, Z);
-    const size_t lds = (size_t)(Z + 1) * sizeof(C);
+    const size_t lds = (size_t)lds_row(Z) * sizeof(C);
 #define RHOK(E)                                                                                              \
     (void)hipFuncSetAttribute((const void *)k_col_rhok<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                               160 * 1024);                                                                     \
@@ -2275,9 +2306,9 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchP
         bp.dep_lds = std::max(bp.dep_lds, pl.lds);
         const int Z = h->g.Z;
         fft_rows(Z, ept, r.rb, r.rc, r.rb2);
-        r.sR1 = Z / 2 + 1;
-        r.sC = Z + 1;
-        r.sR2 = Z + 1;
+        r.sR1 = lds_row(Z / 2);
+        r.sC = lds_row(Z);
+        r.sR2 = lds_row(Z);
         bp.l1 = std::max(bp.l1, (size_t)r.rb * r.sR1 * csz);
         bp.lc = std::max(bp.lc, (size_t)2 * r.rc * r.sC * csz);
         bp.l2 = std::max(bp.l2, (size_t)r.rb2 * r.sR2 * csz);
