@@ -140,8 +140,36 @@ def cpu_baseline(p, git, rel, budget_s=20.0):
                       "TSC 2048^2 workload (%.1f s); host %s" % (n, dt, platform.processor() or platform.machine())}
 
 
-def ensemble_figures(log_fn):
-    """Batched mix (256 replicas) vs the plain path, and the served stream."""
+def cpu_ensemble_baseline(mix, budget_s=10.0):
+    """All host cores over independent replicas of the batched mix: the FP64
+    oracle, one replica per worker process (at most 16), each stepped for
+    budget_s after one warm-up step; aggregate particle-steps/s."""
+    import multiprocessing as mp
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import pool
+
+    O.build()
+    cores = min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+    jobs = []
+    for p, git, rel in mix[:cores]:
+        kw = dict(n1=p.n1, n2=p.n2, L=p.L, Z=p.Z, scheme=p.scheme, lam=p.lam, kap=p.kap, T=p.T,
+                  dt_max=p.dt_max, mrd=float("inf"), f2_f1=p.f2_f1, noise_mode=1, seed=p.seed)
+        jobs.append((kw, git, rel, budget_s))
+    with mp.get_context("spawn").Pool(len(jobs)) as workers:
+        res = workers.map(pool.run_replica, jobs)
+    value = sum(ps / t for ps, t in res)
+    return {"value": value, "unit": "particle-steps/s", "cores": len(jobs), "kind": "port",
+            "sample": "FP64 C restatement, %d processes side by side, each one replica of the batched mix "
+                      "(the first %d, %d particles in all) stepped %.0f s after 1 warm-up step; host %s"
+                      % (len(jobs), len(jobs), sum(j[0]["n1"] + j[0]["n2"] for j in jobs), budget_s,
+                         platform.processor() or platform.machine())}
+
+
+def ensemble_figures(log_fn, cpu=False):
+    """Batched mix (256 replicas) vs the plain path, and the served stream;
+    with cpu, the all-core host baseline of the batched mix."""
     import numpy as np
 
     import ensemble
@@ -163,6 +191,9 @@ def ensemble_figures(log_fn):
                "stage_ms": {k: v for k, v in stages.items()},
                "mix": "N log-uniform [2e3, 6.4e4], density 0.5, Z = pow2(L/0.5) in [64, 512], "
                       "lambda U[5,50], CIC/TSC, FP32 forces, uniform start"}
+    if cpu:
+        log_fn("ensemble: all-core CPU baseline of the mix")
+        batched["cpu_baseline"] = cpu_ensemble_baseline(mix)
     jobs = ensemble.served_jobs(64)
     work = float(np.mean([(p.n1 + p.n2) * st for p, _, _, st in jobs]))
     rate = 1.5 * ps_b / work
@@ -252,7 +283,7 @@ def main():
     }
     sim.close()
     if not args.no_ensemble:
-        out["batched"], out["served"] = ensemble_figures(log)
+        out["batched"], out["served"] = ensemble_figures(log, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, git, rel)
     else:
