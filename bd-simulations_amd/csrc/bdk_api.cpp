@@ -127,7 +127,7 @@ void release_streams(bdk_t *h) {
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -543,6 +543,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.lpid, N + 1));
     ALLOC(dalloc(d.lrelf, N + 1));
     ALLOC(dalloc(d.bcount, 4096 * 32));  // one entry per 4096 cells, 128 bytes apart
+    ALLOC(dalloc(d.own, h->g.nc + 1));
     ALLOC(dalloc_v(d.rho, Z * Z * R));
     ALLOC(dalloc_v(d.ht, (Z / 2) * Z * 2 * R));
     ALLOC(dalloc_v(d.gt, (Z / 2) * Z * 4 * R));
@@ -556,6 +557,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.qvf, Z));
     ALLOC(dalloc(d.sc, 1));
     ALLOC(bdkx::set_kernel_attributes());
+    ALLOC(bdkx::launch_own(h));
 #undef ALLOC
     if (rc != BDK_OK) {
         free_dev(h);

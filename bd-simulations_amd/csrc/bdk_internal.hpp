@@ -76,6 +76,7 @@ struct DevBufs {
     int *lpid = nullptr;
     float2 *lrelf = nullptr;  // FP32 rel - B/2 of list entry q (FP32 WCA)
     int *bcount = nullptr;    // particles per block of 4096 cells, kept current by the steps
+    int *own = nullptr;       // [nc+1] first mesh index owned by list cell c (deposit tiles)
     // mesh
     void *rho = nullptr;      // Z*Z real
     void *ht = nullptr;       // (Z/2)*Z complex, transposed half spectrum
@@ -93,6 +94,9 @@ struct DepArgs {
     int tc;        // list cells per tile side
     int seg_cap;   // staged list segments (rows, two per row when y wraps)
     int win_cap;   // owned mesh points per tile (capacity)
+    int win_fast;  // FP32: points of the stencil window of a tile (0: no windowed path)
+    float scale_f; // FP32 windowed path: 2^S with |charge| 2^S < 2^31
+    double iscale_f;
     double scale_hi, scale_lo;      // 2^S1, 2^S2
     double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
     int reset;     // re-arm the per-step scalars (block 0)
@@ -158,6 +162,7 @@ hipError_t launch_fft_pass(bdk_t *h, int pass);   // 0 r2c rows, 1 columns, 2 c2
 hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)
 hipError_t launch_update(bdk_t *h, int fused);    // noise, Euler step, refresh, counting
 hipError_t launch_bin(bdk_t *h);                  // counting after an upload/refresh
+hipError_t launch_own(bdk_t *h);                  // owned-mesh-index table of the deposit tiles
 hipError_t launch_scan_fill(bdk_t *h);            // exclusive scan + fill of the cell list
 hipError_t launch_refresh(bdk_t *h);              // refresh only (no motion)
 hipError_t launch_noise_out(bdk_t *h);            // noise of the next step, by pid

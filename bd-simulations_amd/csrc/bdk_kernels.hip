@@ -505,6 +505,50 @@ __device__ int first_owned(int c, const Geo &g) {
     return k;
 }
 
+__global__ void k_own(Geo g, int *own) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= g.nc) own[c] = first_owned(c, g);
+}
+
+// FP32 deposit into a window that holds the whole stencil of every staged
+// particle (mesh rows kw0 .. kw0 + nwx - 1, columns lw0 .., unwrapped): one
+// base index per particle and nine fixed offsets, no per-point bounds.  The
+// charges are rounded to 32-bit fixed point (scale 2^S, |charge| 2^S < 2^31,
+// far below FP32 resolution) and summed in 64 bits, so the sums stay exact
+// and independent of the order of the atomics.
+template <int SCHEME>
+__device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs &d, const DepArgs &a, int q,
+                                                     int gx, int gy, int kw0, int lw0, int nwx, int nwy,
+                                                     unsigned long long *acc) {
+    const bool s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
+    const double2 rr = d.lrel[q];
+    const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
+    int kp, lp;
+    float wx[3], wy[3];
+    weights_r<float, SCHEME>(x, g, kp, wx);
+    weights_r<float, SCHEME>(y, g, lp, wy);
+    const int kb = (kp - 1 - kw0) & (g.Z - 1), lb = (lp - 1 - lw0) & (g.Z - 1);
+    if (kb > nwx - 3 || lb > nwy - 3) {  // outside the window: not reachable, flagged
+        atomicOr(&d.sc->nonfinite, 4);
+        return;
+    }
+    const float dq = (float)g.drho * (float)(s2 ? a.q2 : a.q1);
+    unsigned long long *p = acc + kb * nwy + lb;
+    if (SCHEME == 0) {
+        const int X = __float2int_rn(dq * a.scale_f);
+        atomicAdd(&p[nwy + 1], (unsigned long long)(long long)X);
+        return;
+    }
+#pragma unroll
+    for (int ia = 0; ia < 3; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < 3; ++ib) {
+            const float cf = (wx[ia] * wy[ib]) * dq;
+            const int X = __float2int_rn(cf * a.scale_f);
+            atomicAdd(&p[ia * nwy + ib], (unsigned long long)(long long)X);
+        }
+}
+
 template <typename R, int SCHEME>
 __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int q,
                                                  int gx, int gy, int klo, int khi, int llo, int lhi,
@@ -564,7 +608,8 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     extern __shared__ unsigned char smem_raw[];
     unsigned long long *acc_hi = (unsigned long long *)smem_raw;
     unsigned long long *acc_lo = acc_hi + a.win_cap;
-    int *seg_q = (int *)(acc_lo + (std::is_same<R, double>::value ? a.win_cap : 0));
+    const int acc_n = max(a.win_cap * (std::is_same<R, double>::value ? 2 : 1), a.win_fast);
+    int *seg_q = (int *)(acc_hi + acc_n);
     int *seg_pre = seg_q + a.seg_cap;       // [seg_cap + 1]
     int *seg_row = seg_pre + a.seg_cap + 1;
     __shared__ int info[4];
@@ -580,19 +625,9 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
         d.sc->dt_bits = bitsd(g.dt_max);
         d.sc->exc_bits[(step + 1) & 1] = 0ull;
     }
-    if (tid == 0) {
-        info[0] = first_owned(cx0, g);
-        info[1] = first_owned(cx0 + nx, g);
-        info[2] = first_owned(cy0, g);
-        info[3] = first_owned(cy0 + ny, g);
-    }
+    if (tid < 4) info[tid] = d.own[tid == 0 ? cx0 : tid == 1 ? cx0 + nx : tid == 2 ? cy0 : cy0 + ny];
     __syncthreads();
     const int klo = info[0], khi = info[1], llo = info[2], lhi = info[3];
-    const int nwin = (khi - klo) * (lhi - llo);
-    for (int w = tid; w < nwin; w += nt) {
-        acc_hi[w] = 0ull;
-        if (std::is_same<R, double>::value) acc_lo[w] = 0ull;
-    }
     int hx = (int)floor((1.5 * g.h + e) / g.B) + 1;
     if (hx < 1) hx = 1;
     // staged cells: the tile plus hx halo cells, each distinct cell once (a
@@ -600,6 +635,18 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     const bool allx = nx + 2 * hx >= nc, ally = ny + 2 * hx >= nc;
     const int SX = allx ? nc : nx + 2 * hx, SY = ally ? nc : ny + 2 * hx;
     const int sx0 = allx ? 0 : cx0 - hx, sy0 = ally ? 0 : cy0 - hx;
+    // FP32: the window of mesh points reachable from the staged cells (their
+    // positions lie within e of them; margins cover the index rounding)
+    const int kw0 = (int)floor((sx0 * g.B - e) * g.hinv) - 3, lw0 = (int)floor((sy0 * g.B - e) * g.hinv) - 3;
+    const int nwx = (int)floor(((sx0 + SX) * g.B + e) * g.hinv) + 4 - kw0;
+    const int nwy = (int)floor(((sy0 + SY) * g.B + e) * g.hinv) + 4 - lw0;
+    const bool win = std::is_same<R, float>::value && !allx && !ally && nwx <= g.Z && nwy <= g.Z &&
+                     nwx * nwy <= a.win_fast;
+    const int nwin = win ? nwx * nwy : (khi - klo) * (lhi - llo);
+    for (int w = tid; w < nwin; w += nt) {
+        acc_hi[w] = 0ull;
+        if (std::is_same<R, double>::value) acc_lo[w] = 0ull;
+    }
     // The list holds the particles in cell order, so the staged cells of one
     // row (fixed x) are one contiguous list range, or two when the y range
     // wraps: segments of consecutive list positions.
@@ -639,24 +686,37 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
             }
             const int q = seg_q[lo] + (t - seg_pre[lo]);
             const int gx = seg_row[lo];
-            deposit_particle<R, SCHEME>(g, d, a, q, gx, d.lcell[q] - gx * nc, klo, khi, llo, lhi, acc_hi, acc_lo);
+            if (win)
+                deposit_particle_win<SCHEME>(g, d, a, q, gx, d.lcell[q] - gx * nc, kw0, lw0, nwx, nwy, acc_hi);
+            else
+                deposit_particle<R, SCHEME>(g, d, a, q, gx, d.lcell[q] - gx * nc, klo, khi, llo, lhi, acc_hi,
+                                            acc_lo);
         }
     } else {
         for (int idx = tid; idx < SX * SY; idx += nt) {
             const int ia = idx / SY, ib = idx - ia * SY;
             const int gx = wrap1(sx0 + ia, nc), gy = wrap1(sy0 + ib, nc);
             const int c = gx * nc + gy;
-            for (int q = d.start[c]; q < d.start[c + 1]; ++q)
-                deposit_particle<R, SCHEME>(g, d, a, q, gx, gy, klo, khi, llo, lhi, acc_hi, acc_lo);
+            for (int q = d.start[c]; q < d.start[c + 1]; ++q) {
+                if (win)
+                    deposit_particle_win<SCHEME>(g, d, a, q, gx, gy, kw0, lw0, nwx, nwy, acc_hi);
+                else
+                    deposit_particle<R, SCHEME>(g, d, a, q, gx, gy, klo, khi, llo, lhi, acc_hi, acc_lo);
+            }
         }
     }
     __syncthreads();
     R *rho = (R *)d.rho;
-    const int nl = lhi - llo;
-    for (int w = tid; w < nwin; w += nt) {
+    const int nl = lhi - llo, nown = (khi - klo) * nl;
+    for (int w = tid; w < nown; w += nt) {
         const int k = klo + w / nl, l = llo + w - (w / nl) * nl;
-        double v = (double)(long long)acc_hi[w] * a.iscale_hi;
-        if (std::is_same<R, double>::value) v += (double)(long long)acc_lo[w] * a.iscale_lo;
+        double v;
+        if (win) {
+            v = (double)(long long)acc_hi[((k - kw0) & (g.Z - 1)) * nwy + ((l - lw0) & (g.Z - 1))] * a.iscale_f;
+        } else {
+            v = (double)(long long)acc_hi[w] * a.iscale_hi;
+            if (std::is_same<R, double>::value) v += (double)(long long)acc_lo[w] * a.iscale_lo;
+        }
         rho[(size_t)k * g.Z + l] = (R)v;
     }
 }
@@ -1712,8 +1772,21 @@ static DepPlan dep_plan(const bdk_t *h, int reset) {
     pl.a.reset = reset;
     pl.a.q1 = 1.0;  // species charges of the capillary deposit (gr:31-547)
     pl.a.q2 = g.f2_f1;
+    // FP32 stencil window, sized for halos of up to 2 cells and excursions
+    // up to B/4 (larger ones take the bounds-checked path): the staged span
+    // plus the excursion on both sides and the index margins
+    pl.a.win_fast = 0;
+    pl.a.scale_f = 1.0f;
+    pl.a.iscale_f = 1.0;
+    if (!f64) {
+        const int wside = (int)std::ceil((tc + 4.5) * g.B / g.h) + 8;
+        pl.a.win_fast = wside * wside;
+        const int sf = (int)std::floor(30.0 - std::log2(cmax));
+        pl.a.scale_f = std::ldexp(1.0f, sf);
+        pl.a.iscale_f = std::ldexp(1.0, -sf);
+    }
     pl.grid = dim3(nblk(g.nc, tc), nblk(g.nc, tc));
-    pl.lds = (size_t)pl.a.win_cap * (f64 ? 16 : 8) + (size_t)(3 * pl.a.seg_cap + 1) * 4;
+    pl.lds = (size_t)std::max(pl.a.win_cap * (f64 ? 2 : 1), pl.a.win_fast) * 8 + (size_t)(3 * pl.a.seg_cap + 1) * 4;
     return pl;
 }
 
@@ -1832,6 +1905,11 @@ static void launch_scan_fill_inner(bdk_t *h, int inc_step) {
 
 hipError_t launch_scan_fill(bdk_t *h) {
     launch_scan_fill_inner(h, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_own(bdk_t *h) {
+    hipLaunchKernelGGL(k_own, dim3(nblk(h->g.nc + 1, 256)), dim3(256), 0, h->stream, h->g, h->d.own);
     return hipGetLastError();
 }
 
