@@ -398,6 +398,58 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
 // collects all partners; otherwise same-species and cross-species partners
 // are summed apart and added at the end (the grouping of dm:617-702 +
 // dm:973-1059).  Partners are visited in list order, cell by cell.
+// FP32 WCA of one particle against candidates q0..qe-1 of one neighbour
+// row.  The cell offset is folded into this particle's coordinates: y + B
+// below m1 (yb), y below m2 (ym), y - B from m2 on (ya), x + ox (xo).
+// SELF: the row holds this particle (position qi, skipped).  Contraction
+// into FMAs is allowed in the FP32 force arithmetic, unlike the FP64
+// position arithmetic.  (Scalar arguments: a parameter struct here was
+// placed in LDS by the compiler.)
+template <bool UNIT, bool SELF>
+__device__ __forceinline__ void wca_f32_row(const unsigned *__restrict__ list, const float2 *__restrict__ rf, int qi,
+                                            bool s2i, int q0, int qe, int m1, int m2, float xo, float yb, float ym,
+                                            float ya, float cut2, float cap, float sigA, float prefA, float sigB,
+                                            float prefB, float &fsx, float &fsy, float &fcx, float &fcy) {
+#pragma clang fp contract(fast)
+    unsigned ent = list[q0];
+    float2 rj = rf[q0];
+    for (int q = q0; q < qe; ++q) {
+        const unsigned e = ent;
+        const float2 r = rj;
+        ent = list[q + 1];  // q + 1 <= N: padding entry
+        rj = rf[q + 1];
+        const float yo = q < m1 ? yb : (q < m2 ? ym : ya);
+        const float dx = xo - r.x, dy = yo - r.y;
+        const float r2 = dx * dx + dy * dy;
+        if (UNIT) {
+            if (r2 <= cut2 && (!SELF || q != qi)) {
+                const float inv = __builtin_amdgcn_rcpf(r2);
+                const float i2 = inv * inv, i4 = i2 * i2;
+                const float absF = fminf(24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
+                fsx += absF * dx;
+                fsy += absF * dy;
+            }
+        } else {
+            const bool s2j = (e >> 31) != 0;
+            const float sig = s2j ? sigB : sigA;
+            const float r2s = r2 * sig * sig;
+            if (r2s <= cut2 && (!SELF || q != qi)) {
+                const float inv = __builtin_amdgcn_rcpf(r2s);
+                const float i2 = inv * inv, i4 = i2 * i2;
+                const float absF = fminf((s2j ? prefB : prefA) * 24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
+                const float fx = absF * dx * sig, fy = absF * dy * sig;
+                if (s2i == s2j) {
+                    fsx += fx;
+                    fsy += fy;
+                } else {
+                    fcx += fx;
+                    fcy += fy;
+                }
+            }
+        }
+    }
+}
+
 template <bool UNIT>
 __device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int blk) {
     const int qi = blk * blockDim.x + threadIdx.x;
@@ -410,66 +462,39 @@ __device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int
     const float2 ri = rf[qi];
     const bool s2i = d.lpid[qi] >= g.n1;
     const float B = (float)g.B, cut2 = (float)g.cut2, cap = (float)g.maxpair;
-    // factors for a partner of species 1 (A) or species 2 (Bp)
+    // factors for a partner of species 1 (A) or species 2 (B)
     const float sigA = s2i ? (float)g.s12 : 1.0f, prefA = s2i ? (float)g.e12f : 1.0f;
     const float sigB = s2i ? (float)g.s22 : (float)g.s12, prefB = s2i ? (float)g.e22f : (float)g.e12f;
+    const float yb = ri.y + B, ym = ri.y, ya = ri.y - B;
     float fsx = 0, fsy = 0, fcx = 0, fcy = 0;
-    // candidates q0..qe-1; oy = B below m1, 0 below m2, -B from m2 on
-    auto range = [&](int q0, int qe, int m1, int m2, float ox) {
-        unsigned ent = list[q0];
-        float2 rj = rf[q0];
-        for (int q = q0; q < qe; ++q) {
-            const unsigned e = ent;
-            const float2 r = rj;
-            ent = list[q + 1];  // q + 1 <= N: padding entry
-            rj = rf[q + 1];
-            const float oy = q < m1 ? B : (q < m2 ? 0.0f : -B);
-            const float dx = (ri.x - r.x) + ox, dy = (ri.y - r.y) + oy;
-            const float r2 = dx * dx + dy * dy;
-            if (UNIT) {
-                if (r2 <= cut2 && q != qi) {
-                    const float inv = __builtin_amdgcn_rcpf(r2);
-                    const float i2 = inv * inv, i4 = i2 * i2;
-                    const float absF = fminf(24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
-                    fsx += absF * dx;
-                    fsy += absF * dy;
-                }
-            } else {
-                const bool s2j = (e >> 31) != 0;
-                const float sig = s2j ? sigB : sigA;
-                const float r2s = r2 * sig * sig;
-                if (r2s <= cut2 && q != qi) {
-                    const float inv = __builtin_amdgcn_rcpf(r2s);
-                    const float i2 = inv * inv, i4 = i2 * i2;
-                    const float absF = fminf((s2j ? prefB : prefA) * 24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
-                    const float fx = absF * dx * sig, fy = absF * dy * sig;
-                    if (s2i == s2j) {
-                        fsx += fx;
-                        fsy += fy;
-                    } else {
-                        fcx += fx;
-                        fcy += fy;
-                    }
-                }
-            }
-        }
-    };
+#define BDK_WCA_ROW(SELF, Q0, QE, M1, M2)                                                                    \
+    wca_f32_row<UNIT, SELF>(list, rf, qi, s2i, Q0, QE, M1, M2, xo, yb, ym, ya, cut2, cap, sigA, prefA, sigB, \
+                            prefB, fsx, fsy, fcx, fcy)
     const bool inner = cy >= 1 && cy <= nc - 2;
     for (int da = -1; da <= 1; ++da) {
         const int row = wrap1(cx + da, nc) * nc;
-        const float ox = -da * B;
+        const float xo = ri.x - da * B;
         if (inner) {
             // cells cy-1, cy, cy+1 of a row are adjacent in the list
             const int *st = d.start + row + cy - 1;
-            range(st[0], st[3], st[1], st[2], ox);
+            const int s0 = st[0], s1 = st[1], s2 = st[2], s3 = st[3];
+            if (da == 0)
+                BDK_WCA_ROW(true, s0, s3, s1, s2);
+            else
+                BDK_WCA_ROW(false, s0, s3, s1, s2);
         } else {
             for (int db = -1; db <= 1; ++db) {
                 const int cc = row + wrap1(cy + db, nc);
                 const int q0 = d.start[cc], qe = d.start[cc + 1];
-                range(q0, qe, db < 0 ? qe : q0, db > 0 ? q0 : qe, ox);
+                const int m1 = db < 0 ? qe : q0, m2 = db > 0 ? q0 : qe;
+                if (da == 0 && db == 0)
+                    BDK_WCA_ROW(true, q0, qe, m1, m2);
+                else
+                    BDK_WCA_ROW(false, q0, qe, m1, m2);
             }
         }
     }
+#undef BDK_WCA_ROW
     float2 o;
     o.x = fsx + fcx;
     o.y = fsy + fcy;
