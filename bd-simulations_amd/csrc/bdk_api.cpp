@@ -109,7 +109,7 @@ Footprint footprint_of(const bd_params &p) {
     Footprint f;
     // cell, rel, pid, fwca, fkap, noise_up, noise_out, rank, list
     f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4 + 8);
-    f.cells = ncell * 12 + 4 + 4096 * 32 * 4;
+    f.cells = ncell * 16 + 8 + 4096 * 32 * 4 + 64 * 16 * 8;
     // rho, ht, gt, fmesh, tables
     f.mesh = Z * Z * R + (Z / 2) * Z * 2 * R + (Z / 2) * Z * 4 * R + Z * Z * 2 * R + Z * (2 * R + 3 * 8 + 3 * 4);
     f.total = f.particles + f.cells + f.mesh + sizeof(DevScalars);
@@ -119,15 +119,16 @@ Footprint footprint_of(const bd_params &p) {
 void release_streams(bdk_t *h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->ev_copy) (void)hipEventDestroy(h->ev_copy);
     if (h->aux) (void)hipStreamDestroy(h->aux);
     if (h->stream) (void)hipStreamDestroy(h->stream);
-    h->ev_fork = h->ev_join = nullptr;
+    h->ev_fork = h->ev_join = h->ev_copy = nullptr;
     h->aux = h->stream = nullptr;
 }
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -243,6 +244,7 @@ int upload(bdk_t *h) {
     HIPCHK(h, bdkx::launch_bin(h));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->dev_valid = true;
+    h->forces_ready = false;
     return BDK_OK;
 }
 
@@ -252,6 +254,16 @@ int ensure(bdk_t *h) {
     if (!h->pos_set[0] || (h->p.n2 > 0 && !h->pos_set[1]))
         return fail(h, BDK_ERR_STATE, "positions of both species must be set first (bdk_set_positions)");
     return upload(h);
+}
+
+// uploaded state whose forces are on the device (steps start from them)
+int ensure_forces(bdk_t *h) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (h->forces_ready) return BDK_OK;
+    HIPCHK(h, bdkx::enqueue_forces(h));
+    h->forces_ready = true;
+    return BDK_OK;
 }
 
 // list-ordered device array of real2 -> per-species [n][2] doubles
@@ -294,7 +306,7 @@ void drop_graph(bdk_t *h) {
 int capture_steps(bdk_t *h, int nsteps, hipGraph_t *graph, hipGraphExec_t *exec) {
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = bdkx::enqueue_step(h, nullptr, nullptr);
+    for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = bdkx::enqueue_step(h, nullptr, nullptr);  // forces ready
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess) return hipfail(h, e, "enqueue_step (capture)");
@@ -324,8 +336,8 @@ double dt_from_bits(unsigned long long b) {
 struct bdk_batch {
     std::vector<bdk_t *> mem;
     hipStream_t stream = nullptr;
-    hipStream_t aux = nullptr;  // WCA stage beside the mesh stages
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t aux = nullptr;  // mesh stages beside the list rebuild
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_copy = nullptr;
     int dev = 0;
     bdkx::BatchPlan bp;
     int ept_min[3] = {8, 8, 8};
@@ -377,13 +389,24 @@ struct bdk_batch {
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
         for (int i = 0; i < nsteps && e == hipSuccess; ++i)
-            e = bdkx::batch_enqueue_step(bp, stream, aux, ev_fork, ev_join);
+            e = bdkx::batch_enqueue_step(bp, stream, aux, ev_fork, ev_join, ev_copy);
         hipGraph_t gg = nullptr;
         hipError_t e2 = hipStreamEndCapture(stream, &gg);
         if (e == hipSuccess) e = e2;
         if (e != hipSuccess) return e;
         *g = gg;
         return hipGraphInstantiate(x, gg, nullptr, nullptr, 0);
+    }
+    // members uploaded since their last step get their forces first (all
+    // members: the others' forces come out the same)
+    hipError_t ensure_forces() {
+        bool all = true;
+        for (bdk_t *h : mem) all = all && h->forces_ready;
+        if (all) return hipSuccess;
+        hipError_t e = bdkx::batch_enqueue_forces(bp, stream, aux, ev_fork, ev_join);
+        if (e == hipSuccess)
+            for (bdk_t *h : mem) h->forces_ready = true;
+        return e;
     }
     hipError_t ensure_graphs() {
         if (gexec && gexec_multi) return hipSuccess;
@@ -525,6 +548,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     ALLOC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     ALLOC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    ALLOC(hipEventCreateWithFlags(&h->ev_copy, hipEventDisableTiming));
     ALLOC(dalloc(d.cell, N));
     ALLOC(dalloc(d.rel, N));
     ALLOC(dalloc(d.pid, N));
@@ -536,6 +560,8 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.count, ncell));
     ALLOC(dalloc(d.arr, ncell));
     ALLOC(dalloc(d.start, ncell + 1));
+    ALLOC(dalloc(d.start_old, ncell + 1));
+    ALLOC(dalloc(d.exo, 64 * 16));
     // list-ordered arrays carry one padding entry (read ahead by k_wca_f32)
     ALLOC(dalloc(d.list, N + 1));
     ALLOC(dalloc(d.lcell, N + 1));
@@ -569,6 +595,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     std::memset(&sc, 0, sizeof(sc));
     e = hipMemcpy(d.sc, &sc, sizeof(sc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(d.count, 0, ncell * 4);
+    if (e == hipSuccess) e = hipMemset(d.exo, 0, 64 * 16 * 8);
     if (e == hipSuccess) e = hipMemset(d.arr, 0, ncell * 4);
     if (e == hipSuccess) e = hipMemset(d.list, 0, (N + 1) * 4);
     if (e == hipSuccess) e = hipMemset(d.lrelf, 0, (N + 1) * 8);
@@ -737,7 +764,7 @@ int bdk_step(bdk_t *h, double tmax, double *dt_out) {
         if (dt_out) *dt_out = 1.0e10;
         return fail(h, BDK_ERR_ARG, "Fehler: Negative Zeitschrittweite gefordert (tmax <= 0)");
     }
-    int rc = ensure(h);
+    int rc = ensure_forces(h);
     if (rc) return rc;
     if (h->g.idgas) {
         HIPCHK(h, hipMemsetAsync(h->d.fmesh, 0, (size_t)h->g.Z * h->g.Z * 2 * real_size(h->p), h->stream));
@@ -758,7 +785,7 @@ int bdk_step(bdk_t *h, double tmax, double *dt_out) {
 int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out) {
     if (!h || !out) return fail(h, BDK_ERR_ARG, "null argument");
     if (!(tmax > 0.0)) return fail(h, BDK_ERR_ARG, "Fehler: Negative Zeitschrittweite gefordert (tmax <= 0)");
-    int rc = ensure(h);
+    int rc = ensure_forces(h);
     if (rc) return rc;
     bdkx::StepInfoDev *dinfo = nullptr;
     HIPCHK(h, hipMalloc((void **)&dinfo, sizeof(bdkx::StepInfoDev)));
@@ -798,7 +825,7 @@ int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out) {
 }
 
 int bdk_run_steps(bdk_t *h, int64_t k, double *t_out) {
-    int rc = ensure(h);
+    int rc = ensure_forces(h);
     if (rc) return rc;
     if (k < 0) return fail(h, BDK_ERR_ARG, "k must be >= 0");
     long double t = 0.0L;
@@ -864,6 +891,7 @@ int bdk_refresh(bdk_t *h) {
     if (rc) return rc;
     HIPCHK(h, bdkx::launch_refresh(h));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->forces_ready = false;
     return check_nonfinite(h);
 }
 
@@ -926,7 +954,7 @@ int bdk_get_mesh(bdk_t *h, int which, double *out) {
 
 int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, int32_t names_cap,
                       int32_t *n_out) {
-    int rc = ensure(h);
+    int rc = ensure_forces(h);
     if (rc) return rc;
     if (h->p.noise_mode == BDK_NOISE_UPLOAD) return fail(h, BDK_ERR_STATE, "profiling needs a Philox noise mode");
     std::vector<double> acc;
@@ -960,7 +988,7 @@ int bdk_profile_steps(bdk_t *h, int64_t k, int32_t nk, double *ms, char *names, 
 }
 
 int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch) {
-    int rc = ensure(h);
+    int rc = ensure_forces(h);
     if (rc) return rc;
     if (reps < 1 || kernel < 0 || kernel > 7 || !ms_per_launch) return fail(h, BDK_ERR_ARG, "bad profile request");
     // k_update changes positions, cells, ranks, counts and scalars: snapshot them
@@ -975,7 +1003,8 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
         Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.pid, N * 4},
                        {nullptr, h->d.rank, N * 4},
                        {nullptr, h->d.count, nc2 * 4}, {nullptr, h->d.arr, nc2 * 4},
-                       {nullptr, h->d.bcount, 4096 * 32 * 4}, {nullptr, h->d.sc, sizeof(DevScalars)}};
+                       {nullptr, h->d.bcount, 4096 * 32 * 4}, {nullptr, h->d.exo, 64 * 16 * 8},
+                       {nullptr, h->d.sc, sizeof(DevScalars)}};
         for (Snap sn : list) {
             HIPCHK(h, hipMalloc(&sn.p, sn.n));
             HIPCHK(h, hipMemcpyAsync(sn.p, sn.src, sn.n, hipMemcpyDeviceToDevice, h->stream));
@@ -1399,6 +1428,7 @@ int bdk_load_state(bdk_t *h, const char *path) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->pos_set[0] = h->pos_set[1] = true;
     h->dev_valid = true;
+    h->forces_ready = false;
     h->host_step = hs;
     return BDK_OK;
 }
@@ -1533,6 +1563,7 @@ int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_copy, hipEventDisableTiming);
     int ept[3];
     for (int k = 0; k < 3; ++k) ept[k] = bdkx::batch_min_ept(k, zmax);
     if (e == hipSuccess) e = b->rebuild(ept);
@@ -1554,6 +1585,7 @@ void bdk_batch_destroy(bdk_batch_t *b) {
     b->free_tables();
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev_copy) (void)hipEventDestroy(b->ev_copy);
     if (b->aux) (void)hipStreamDestroy(b->aux);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1565,6 +1597,7 @@ int bdk_batch_run_steps(bdk_batch_t *b, int64_t k) {
     if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
     if (k < 0) return b->fail(BDK_ERR_ARG, "k must be >= 0");
     hipError_t e = hipSetDevice(b->dev);
+    if (e == hipSuccess) e = b->ensure_forces();
     if (e == hipSuccess) e = b->ensure_graphs();
     int64_t i = 0;
     for (; e == hipSuccess && i + kGraphSteps <= k; i += kGraphSteps) e = hipGraphLaunch(b->gexec_multi, b->stream);
@@ -1596,6 +1629,7 @@ int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_pe
     if (stage < 0 || stage > 7 || stage == 5 || stage == 6 || reps < 1 || !ms_per_launch)
         return b->fail(BDK_ERR_ARG, "bad profile request (stages 0-4 and 7)");
     hipError_t e = hipSetDevice(b->dev);
+    if (e == hipSuccess) e = b->ensure_forces();
     if (e == hipSuccess) e = b->ensure_graphs();
     auto launch = [&]() -> hipError_t {
         if (stage == 7) return hipGraphLaunch(b->gexec, b->stream);
@@ -1646,6 +1680,10 @@ int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *ept_out) {
     }
     if (e == hipSuccess) e = b->rebuild(best);
     if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_tune: ") + hipGetErrorString(e));
+    // the timed FFT stages overwrote the members' mesh forces with other
+    // tilings (whose twiddle roundings can differ): recompute them with the
+    // chosen one before the next step
+    for (bdk_t *h : b->mem) h->forces_ready = false;
     if (ept_out)
         for (int k = 0; k < 3; ++k) ept_out[k] = best[k];
     return BDK_OK;

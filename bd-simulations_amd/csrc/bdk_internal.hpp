@@ -69,6 +69,8 @@ struct DevBufs {
     int *count = nullptr;     // [ncell] particles per cell (kept current across steps)
     int *arr = nullptr;       // [ncell] arrivals of the current step (zero between steps)
     int *start = nullptr;     // [ncell+1]
+    int *start_old = nullptr; // [ncell+1] previous starts: the slot arrays' grouping during a rebuild
+    unsigned long long *exo = nullptr; // [64 x 16] max distance of a new position outside its previous cell (double bits)
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
     // the state in list order (k_fill): cell, rel and particle id of list entry q
     int *lcell = nullptr;
@@ -101,6 +103,7 @@ struct DepArgs {
     double iscale_hi, iscale_lo;    // 2^-S1, 2^-S2
     int reset;     // re-arm the per-step scalars (block 0)
     double q1, q2; // species charges (1 and f2_f1 for the capillary force)
+    int slot;      // positions from the slot arrays grouped by start_old (during a rebuild), bound from exo
 };
 
 // One member of a batch, as the batched kernels read it (device memory).
@@ -134,11 +137,12 @@ struct bdk {
     bdkx::Geo g;
     bdkx::DevBufs d;
     hipStream_t stream = nullptr;
-    hipStream_t aux = nullptr;            // WCA runs here, beside the mesh chain
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t aux = nullptr;            // second stream of a step (mesh chain beside the list rebuild)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_copy = nullptr;
     int dev = 0;
     bool pos_set[2] = {false, false};
     bool dev_valid = false;       // device particle state uploaded
+    bool forces_ready = false;    // fwca / fmesh hold the forces of the current state
     std::vector<int32_t> hgit[2];
     std::vector<double> hrel[2];
     int scan_blocks = 0;
@@ -154,11 +158,12 @@ namespace bdkx {
 
 // Launchers implemented in bdk_kernels.hip.  All enqueue on h->stream.
 hipError_t launch_wca(bdk_t *h, hipStream_t st = nullptr);  // WCA forces -> fwca (default: h->stream)
-hipError_t launch_deposit(bdk_t *h, int reset);   // charges -> rho (reset: re-arm per-step scalars)
+hipError_t launch_deposit(bdk_t *h, int reset, int slot = 0, hipStream_t st = nullptr);  // charges -> rho
+                                                  // (reset: re-arm per-step scalars; slot: during a rebuild)
 hipError_t launch_rearm(bdk_t *h);                // per-step scalars when no deposit runs
 size_t deposit_lds_bytes(const bdk_t *h);          // LDS of one deposit block
 hipError_t launch_fft(bdk_t *h);                  // rho -> fmesh (r2c, multiply, c2r)
-hipError_t launch_fft_pass(bdk_t *h, int pass);   // 0 r2c rows, 1 columns, 2 c2r rows
+hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st = nullptr);  // 0 r2c rows, 1 columns, 2 c2r rows
 hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)
 hipError_t launch_update(bdk_t *h, int fused);    // noise, Euler step, refresh, counting
 hipError_t launch_bin(bdk_t *h);                  // counting after an upload/refresh
@@ -180,8 +185,13 @@ int batch_min_ept(int stage, int zmax);
 hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
                        std::vector<int2> maps[BS_COUNT]);
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st);
+// one batched step of members whose forces are ready; with aux and the three
+// events the mesh stages run beside the list rebuild
 hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux = nullptr,
-                              hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
+                              hipEvent_t fork = nullptr, hipEvent_t join = nullptr, hipEvent_t copy = nullptr);
+// the forces of the members' current states (after an upload)
+hipError_t batch_enqueue_forces(const BatchPlan &bp, hipStream_t st, hipStream_t aux = nullptr,
+                                hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 
 // Counters of one step (TimestepInfo, signaturen2.h:71-145), device side.
 struct StepInfoDev {
@@ -195,6 +205,11 @@ struct StepInfoDev {
 
 // one full step as a sequence of launches; names (if non-null) get the
 // kernel labels in launch order; info (if non-null) receives the counters
+// Forces of the current state (WCA beside the deposit and the FFT passes),
+// needed once after the state was set from the host; every step leaves the
+// forces of its new state behind (h->forces_ready).
+hipError_t enqueue_forces(bdk_t *h, std::vector<std::string> *names = nullptr, std::vector<hipEvent_t> *ev = nullptr);
+// One step from a state whose forces are ready.
 hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev,
                         StepInfoDev *info = nullptr);
 

@@ -542,11 +542,9 @@ __global__ void k_own(Geo g, int *own) {
 // far below FP32 resolution) and summed in 64 bits, so the sums stay exact
 // and independent of the order of the atomics.
 template <int SCHEME>
-__device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs &d, const DepArgs &a, int q,
-                                                     int gx, int gy, int kw0, int lw0, int nwx, int nwy,
-                                                     unsigned long long *acc) {
-    const bool s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
-    const double2 rr = d.lrel[q];
+__device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs &d, const DepArgs &a, int gx,
+                                                     int gy, double2 rr, bool s2, int kw0, int lw0, int nwx,
+                                                     int nwy, unsigned long long *acc) {
     const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
     int kp, lp;
     float wx[3], wy[3];
@@ -575,11 +573,9 @@ __device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs
 }
 
 template <typename R, int SCHEME>
-__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int q,
-                                                 int gx, int gy, int klo, int khi, int llo, int lhi,
+__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int gx, int gy,
+                                                 double2 rr, bool s2, int klo, int khi, int llo, int lhi,
                                                  unsigned long long *acc_hi, unsigned long long *acc_lo) {
-    const bool s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
-    const double2 rr = d.lrel[q];
     const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
     int kp, lp;
     R wx[3], wy[3];
@@ -644,7 +640,21 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     const int cx0 = (lb % tx) * tc, cy0 = (lb / tx) * tc;
     const int nx = min(tc, nc - cx0), ny = min(tc, nc - cy0);
     const unsigned long long step = d.sc->step;
-    const double e = dbits(d.sc->exc_bits[step & 1]) + 1e-9 * g.L;
+    // bound on how far a staged particle lies outside its staging cell: the
+    // excursion of rel outside [0, B] (list order), or during a rebuild the
+    // distance of the new positions outside their previous cells
+    __shared__ double e_sh;
+    if (a.slot) {
+        if (tid < 64) {
+            double m = dbits(d.exo[tid * 16]);
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+            if (tid == 0) e_sh = m;
+        }
+    } else if (tid == 0) {
+        e_sh = dbits(d.sc->exc_bits[step & 1]);
+    }
+    const int *st = a.slot ? d.start_old : d.start;
     if (a.reset && lb == 0 && tid == 0) {
         // per-step scalars for the kernels that follow this one
         d.sc->dt_bits = bitsd(g.dt_max);
@@ -653,6 +663,7 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     if (tid < 4) info[tid] = d.own[tid == 0 ? cx0 : tid == 1 ? cx0 + nx : tid == 2 ? cy0 : cy0 + ny];
     __syncthreads();
     const int klo = info[0], khi = info[1], llo = info[2], lhi = info[3];
+    const double e = e_sh + 1e-9 * g.L;
     int hx = (int)floor((1.5 * g.h + e) / g.B) + 1;
     if (hx < 1) hx = 1;
     // staged cells: the tile plus hx halo cells, each distinct cell once (a
@@ -693,9 +704,9 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
                 ya = part == 0 ? sy0 : 0;
                 yb = part == 0 ? nc : sy0 + SY - nc;
             }
-            const int qa = d.start[row + ya];
+            const int qa = st[row + ya];
             seg_q[i] = qa;
-            seg_pre[i] = max(d.start[row + yb] - qa, 0);
+            seg_pre[i] = max(st[row + yb] - qa, 0);
             seg_row[i] = gx;
         }
         __syncthreads();
@@ -710,23 +721,49 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
                 else hi = mid - 1;
             }
             const int q = seg_q[lo] + (t - seg_pre[lo]);
-            const int gx = seg_row[lo];
+            int gx, gy;
+            double2 rr;
+            bool s2;
+            if (a.slot) {  // the particle's new cell, not its staging row
+                const int c = d.cell[q];
+                gx = div_nc(c, g);
+                gy = c - gx * nc;
+                rr = d.rel[q];
+                s2 = d.pid[q] >= g.n1;
+            } else {
+                gx = seg_row[lo];
+                gy = d.lcell[q] - gx * nc;
+                rr = d.lrel[q];
+                s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
+            }
             if (win)
-                deposit_particle_win<SCHEME>(g, d, a, q, gx, d.lcell[q] - gx * nc, kw0, lw0, nwx, nwy, acc_hi);
+                deposit_particle_win<SCHEME>(g, d, a, gx, gy, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
             else
-                deposit_particle<R, SCHEME>(g, d, a, q, gx, d.lcell[q] - gx * nc, klo, khi, llo, lhi, acc_hi,
-                                            acc_lo);
+                deposit_particle<R, SCHEME>(g, d, a, gx, gy, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
         }
     } else {
         for (int idx = tid; idx < SX * SY; idx += nt) {
             const int ia = idx / SY, ib = idx - ia * SY;
             const int gx = wrap1(sx0 + ia, nc), gy = wrap1(sy0 + ib, nc);
             const int c = gx * nc + gy;
-            for (int q = d.start[c]; q < d.start[c + 1]; ++q) {
+            for (int q = st[c]; q < st[c + 1]; ++q) {
+                int px = gx, py = gy;
+                double2 rr;
+                bool s2;
+                if (a.slot) {
+                    const int cq = d.cell[q];
+                    px = div_nc(cq, g);
+                    py = cq - px * nc;
+                    rr = d.rel[q];
+                    s2 = d.pid[q] >= g.n1;
+                } else {
+                    rr = d.lrel[q];
+                    s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
+                }
                 if (win)
-                    deposit_particle_win<SCHEME>(g, d, a, q, gx, gy, kw0, lw0, nwx, nwy, acc_hi);
+                    deposit_particle_win<SCHEME>(g, d, a, px, py, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
                 else
-                    deposit_particle<R, SCHEME>(g, d, a, q, gx, gy, klo, khi, llo, lhi, acc_hi, acc_lo);
+                    deposit_particle<R, SCHEME>(g, d, a, px, py, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
             }
         }
     }
@@ -1222,13 +1259,13 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
     }
 }
 
-// Euler step (dm:116-157) + refresh (dm:216-325) + cell counting.
+// Euler step (dm:116-157) + refresh (dm:216-325) + cell counting of list
+// position s (written to slot s).  Returns how far the new position lies
+// outside the particle's previous cell (<= 0 inside).
 // FUSED: capillary force gathered here (fixed dt); else read from fkap.
 template <typename R, int SCHEME, bool FUSED>
-__device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int blk) {
+__device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int s) {
     using R2 = typename V2<R>::T;
-    const int s = blk * blockDim.x + threadIdx.x;  // list position in, slot out
-    if (s >= g.N) return;
     const unsigned long long step = d.sc->step;
     const int c = d.lcell[s];
     int gx = div_nc(c, g), gy = c - gx * g.nc;
@@ -1269,11 +1306,14 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     }
     const double B = g.B;
     int nc_new = c;
-    if (!(fabs(rn.x) < 1e9 && fabs(rn.y) < 1e9)) {
+    const bool finite = fabs(rn.x) < 1e9 && fabs(rn.y) < 1e9;
+    if (!finite) {
         // non-finite or runaway position: keep the old state, raise the flag
         atomicOr(&d.sc->nonfinite, 1);
         rn = r;
-    } else if (rn.x < 0.0 || rn.x > B || rn.y < 0.0 || rn.y > B) {
+    }
+    const double ex_prev = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
+    if (finite && (rn.x < 0.0 || rn.x > B || rn.y < 0.0 || rn.y > B)) {
         int ic = (int)floor((gx * B + rn.x) / B);
         int jc = (int)floor((gy * B + rn.y) / B);
         ic = wrapc(ic + g.nc, g.nc);
@@ -1302,6 +1342,28 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
     if (ex > 0.0) atomicMax(&d.sc->exc_bits[(step + 1) & 1], bitsd(ex));
     if (s == 0) d.sc->dt_last = bitsd(dt);
+    return ex_prev;
+}
+
+// The update of one block of list positions.  The largest distance of a new
+// position outside its previous cell bounds where the next deposit finds the
+// particles when it stages them by their previous cells (during the list
+// rebuild): reduced per block, then one atomicMax per block into one of 64
+// slots (128 bytes apart), so no single address takes every block's atomic.
+template <typename R, int SCHEME, bool FUSED>
+__device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int blk) {
+    __shared__ double wm[16];
+    const int s = blk * blockDim.x + threadIdx.x;  // list position in, slot out
+    double m = 0.0;
+    if (s < g.N) m = update_one<R, SCHEME, FUSED>(g, d, s);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, wm[w]);
+        if (m > 0.0) atomicMax(&d.exo[(blk & 63) * 16], bitsd(m));
+    }
 }
 
 template <typename R, int SCHEME, bool FUSED>
@@ -1549,6 +1611,31 @@ __device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, 
 
 __global__ void k_sort_cells(Geo g, DevBufs d, int moved_ranks) { sort_cells_body(g, d, moved_ranks, blockIdx.x); }
 
+// The starts that group the slot arrays after an update (the previous list
+// order), kept for the deposit that runs beside the rebuild.
+__device__ __forceinline__ void keep_starts_body(const Geo &g, const DevBufs &d, int blk) {
+    const int c = blk * blockDim.x + threadIdx.x;
+    if (c < g.ncell) d.start_old[c] = d.start[c];
+    if (c == 0) d.start_old[g.ncell] = d.start[g.ncell];
+}
+
+__global__ void k_keep_starts(Geo g, DevBufs d) { keep_starts_body(g, d, blockIdx.x); }
+
+// End of a step: step counter + 1, and the per-step scalars of the next
+// update re-armed (dt minimum, excursion slot, displacement bound).
+__device__ __forceinline__ void step_end_body(const Geo &g, const DevBufs &d) {
+    const int t = threadIdx.x;
+    if (t < 64) d.exo[t * 16] = 0ull;
+    if (t == 0) {
+        const unsigned long long s = d.sc->step;
+        d.sc->dt_bits = bitsd(g.dt_max);
+        d.sc->exc_bits[s & 1] = 0ull;  // filled by the update of step s + 1
+        d.sc->step = s + 1;
+    }
+}
+
+__global__ void k_step_end(Geo g, DevBufs d) { step_end_body(g, d); }
+
 // ------------------------------------------------------------------ batches
 // A batch advances many independent simulations (members) with one launch
 // per stage.  A batched kernel maps its block through the stage's table to
@@ -1571,13 +1658,18 @@ __global__ __launch_bounds__(256) void kb_wca(const RepDev *m, const int2 *map) 
 }
 
 template <typename R>
-__global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const int2 *map) {
+__global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const int2 *map, int slot) {
     const int2 t = map[blockIdx.x];
     const RepDev &r = m[t.x];
+    DepArgs a = r.dep;
+    if (slot) {  // beside the rebuild: the step's end re-arms the scalars
+        a.slot = 1;
+        a.reset = 0;
+    }
     switch (r.g.scheme) {
-    case 0: deposit_body<R, 0>(r.g, r.d, r.dep, t.y, r.dep_tx); break;
-    case 1: deposit_body<R, 1>(r.g, r.d, r.dep, t.y, r.dep_tx); break;
-    default: deposit_body<R, 2>(r.g, r.d, r.dep, t.y, r.dep_tx); break;
+    case 0: deposit_body<R, 0>(r.g, r.d, a, t.y, r.dep_tx); break;
+    case 1: deposit_body<R, 1>(r.g, r.d, a, t.y, r.dep_tx); break;
+    default: deposit_body<R, 2>(r.g, r.d, a, t.y, r.dep_tx); break;
     }
 }
 
@@ -1628,7 +1720,7 @@ __global__ void kb_rank(const RepDev *m, const int2 *map) {
 __global__ __launch_bounds__(SCAN_THREADS) void kb_scan(const RepDev *m, const int2 *map) {
     const int2 t = map[blockIdx.x];
     const RepDev &r = m[t.x];
-    scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 1, t.y);
+    scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
 }
 
 __global__ void kb_fill(const RepDev *m, const int2 *map) {
@@ -1640,6 +1732,14 @@ __global__ void kb_sort_cells(const RepDev *m, const int2 *map) {
     const int2 t = map[blockIdx.x];
     sort_cells_body(m[t.x].g, m[t.x].d, 1, t.y);
 }
+
+__global__ void kb_keep_starts(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    keep_starts_body(m[t.x].g, m[t.x].d, t.y);
+}
+
+// one block of 64 threads per member
+__global__ void kb_step_end(const RepDev *m) { step_end_body(m[blockIdx.x].g, m[blockIdx.x].d); }
 
 // ------------------------------------------------------------------ launchers
 static inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1797,6 +1897,7 @@ static DepPlan dep_plan(const bdk_t *h, int reset) {
     pl.a.reset = reset;
     pl.a.q1 = 1.0;  // species charges of the capillary deposit (gr:31-547)
     pl.a.q2 = g.f2_f1;
+    pl.a.slot = 0;
     // FP32 stencil window, sized for halos of up to 2 cells and excursions
     // up to B/4 (larger ones take the bounds-checked path): the staged span
     // plus the excursion on both sides and the index margins
@@ -1817,17 +1918,19 @@ static DepPlan dep_plan(const bdk_t *h, int reset) {
 
 size_t deposit_lds_bytes(const bdk_t *h) { return dep_plan(h, 0).lds; }
 
-template <typename R, int SCHEME> static void launch_deposit_t(bdk_t *h, const DepPlan &pl) {
-    hipLaunchKernelGGL((k_deposit<R, SCHEME>), pl.grid, dim3(DP_THREADS), pl.lds, h->stream, h->g, h->d, pl.a);
+template <typename R, int SCHEME> static void launch_deposit_t(bdk_t *h, const DepPlan &pl, hipStream_t st) {
+    hipLaunchKernelGGL((k_deposit<R, SCHEME>), pl.grid, dim3(DP_THREADS), pl.lds, st, h->g, h->d, pl.a);
 }
 
-hipError_t launch_deposit(bdk_t *h, int reset) {
-    const DepPlan pl = dep_plan(h, reset);
+hipError_t launch_deposit(bdk_t *h, int reset, int slot, hipStream_t st) {
+    DepPlan pl = dep_plan(h, reset);
+    pl.a.slot = slot;
+    if (!st) st = h->stream;
     const bool f32 = h->p.precision == BDK_FP32;
     switch (h->g.scheme) {
-    case 0: f32 ? launch_deposit_t<float, 0>(h, pl) : launch_deposit_t<double, 0>(h, pl); break;
-    case 1: f32 ? launch_deposit_t<float, 1>(h, pl) : launch_deposit_t<double, 1>(h, pl); break;
-    default: f32 ? launch_deposit_t<float, 2>(h, pl) : launch_deposit_t<double, 2>(h, pl); break;
+    case 0: f32 ? launch_deposit_t<float, 0>(h, pl, st) : launch_deposit_t<double, 0>(h, pl, st); break;
+    case 1: f32 ? launch_deposit_t<float, 1>(h, pl, st) : launch_deposit_t<double, 1>(h, pl, st); break;
+    default: f32 ? launch_deposit_t<float, 2>(h, pl, st) : launch_deposit_t<double, 2>(h, pl, st); break;
     }
     return hipGetLastError();
 }
@@ -1844,7 +1947,8 @@ hipError_t launch_rearm(bdk_t *h) {
     default: CALL(32); break;   \
     }
 
-template <typename R> static void launch_fft_t(bdk_t *h, int only = -1) {
+template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStream_t st = nullptr) {
+    if (!st) st = h->stream;
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     const int Z = h->g.Z, logZ = h->g.logZ;
@@ -1852,13 +1956,13 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1) {
     const R *sq = std::is_same<R, float>::value ? (const R *)h->d.sqf : (const R *)h->d.sq;
     const R *lap = std::is_same<R, float>::value ? (const R *)h->d.lapf : (const R *)h->d.lap;
 #define R2C(E)                                                                                              \
-    hipLaunchKernelGGL((k_r2c_rows<R, E>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, h->stream,             \
+    hipLaunchKernelGGL((k_r2c_rows<R, E>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,             \
                        (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1, h->g.xcd)
 #define COL(E)                                                                                              \
-    hipLaunchKernelGGL((k_col<R, E>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, h->stream,            \
+    hipLaunchKernelGGL((k_col<R, E>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,            \
                        (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC)
 #define C2R(E)                                                                                              \
-    hipLaunchKernelGGL((k_c2r_rows<R, E>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, h->stream,            \
+    hipLaunchKernelGGL((k_c2r_rows<R, E>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,            \
                        (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2, h->g.xcd)
     if (only < 0 || only == 0) BDK_EPT_SWITCH(c.e1, R2C)
     if (only < 0 || only == 1) BDK_EPT_SWITCH(c.e2, COL)
@@ -1868,11 +1972,11 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1) {
 #undef C2R
 }
 
-hipError_t launch_fft_pass(bdk_t *h, int pass) {
+hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st) {
     if (h->p.precision == BDK_FP32)
-        launch_fft_t<float>(h, pass);
+        launch_fft_t<float>(h, pass, st);
     else
-        launch_fft_t<double>(h, pass);
+        launch_fft_t<double>(h, pass, st);
     return hipGetLastError();
 }
 
@@ -2030,11 +2134,12 @@ hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *par
     DepPlan pl = dep_plan(h, 0);
     pl.a.q1 = species == 1 ? 1.0 : 0.0;
     pl.a.q2 = species == 2 ? 1.0 : 0.0;
+    hipStream_t st = h->stream;
     const bool f32 = h->p.precision == BDK_FP32;
     switch (h->g.scheme) {
-    case 0: f32 ? launch_deposit_t<float, 0>(h, pl) : launch_deposit_t<double, 0>(h, pl); break;
-    case 1: f32 ? launch_deposit_t<float, 1>(h, pl) : launch_deposit_t<double, 1>(h, pl); break;
-    default: f32 ? launch_deposit_t<float, 2>(h, pl) : launch_deposit_t<double, 2>(h, pl); break;
+    case 0: f32 ? launch_deposit_t<float, 0>(h, pl, st) : launch_deposit_t<double, 0>(h, pl, st); break;
+    case 1: f32 ? launch_deposit_t<float, 1>(h, pl, st) : launch_deposit_t<double, 1>(h, pl, st); break;
+    default: f32 ? launch_deposit_t<float, 2>(h, pl, st) : launch_deposit_t<double, 2>(h, pl, st); break;
     }
     launch_fft_pass(h, 0);
     if (f32)
@@ -2291,56 +2396,80 @@ __global__ void k_info_moves(Geo g, DevBufs d, StepInfoDev *info) {
     if (d.cell[q] != d.lcell[q]) atomicAdd(&info->moves[d.lpid[q] >= g.n1 ? 1 : 0], 1);
 }
 
-// One step.  With ev != nullptr an event is recorded after every launch
-// (profiling); names receives the labels.
-hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev,
-                        StepInfoDev *info) {
+// Event marks for per-kernel timing: with ev != nullptr an event is
+// recorded on the main stream after every launch and everything runs on the
+// main stream in order; names receives the labels.
+struct Marks {
+    bdk_t *h;
+    std::vector<std::string> *names;
+    std::vector<hipEvent_t> *ev;
     hipError_t e = hipSuccess;
-    auto mark = [&](const char *nm) {
+    void chk(hipError_t r) {
+        if (e == hipSuccess && r != hipSuccess) e = r;
+    }
+    void mark(const char *nm) {
         if (names) names->push_back(nm);
         if (ev) {
             hipEvent_t x;
             if (hipEventCreate(&x) == hipSuccess) {
-                if (hipEventRecord(x, h->stream) != hipSuccess && e == hipSuccess) e = hipErrorUnknown;
+                chk(hipEventRecord(x, h->stream));
                 ev->push_back(x);
             }
         }
-    };
-    auto chk = [&](hipError_t r) {
-        if (e == hipSuccess && r != hipSuccess) e = r;
-    };
-    if (ev) mark("begin");
-    const bool kap = !h->g.idgas;
-    // the WCA forces and the mesh chain (deposit, FFTs) only read the
-    // particle state: the WCA runs on a second stream beside the chain
-    // (fork and join by events; in a captured graph they are branches)
-    const bool fork = kap && !ev && h->aux;
-    if (kap) {
-        if (fork) {
-            chk(hipEventRecord(h->ev_fork, h->stream));
-            chk(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
-            chk(launch_wca(h, h->aux));
-            chk(hipEventRecord(h->ev_join, h->aux));
-        } else {
-            chk(launch_wca(h));
-            mark("k_wca");
-        }
-        chk(launch_deposit(h, 1));
-        mark("k_deposit");
-        chk(launch_fft_pass(h, 0));
-        mark("k_r2c_rows");
-        chk(launch_fft_pass(h, 1));
-        mark("k_col");
-        chk(launch_fft_pass(h, 2));
-        mark("k_c2r_rows");
-        if (fork) chk(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+    }
+};
+
+// Forces of the current state.  The WCA forces and the mesh chain (deposit
+// from the list order, FFTs) only read the particle state: the WCA runs on
+// the second stream beside the chain (fork and join by events).
+hipError_t enqueue_forces(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev) {
+    Marks m{h, names, ev};
+    if (ev) m.mark("begin");
+    if (h->g.idgas) return hipSuccess;  // no forces
+    const bool fork = !ev && h->aux;
+    if (fork) {
+        m.chk(hipEventRecord(h->ev_fork, h->stream));
+        m.chk(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
+        m.chk(launch_wca(h, h->aux));
+        m.chk(hipEventRecord(h->ev_join, h->aux));
     } else {
-        chk(launch_rearm(h));
-        mark("k_rearm");
+        m.chk(launch_wca(h));
+        m.mark("k_wca");
+    }
+    m.chk(launch_deposit(h, 1));
+    m.mark("k_deposit");
+    for (int p = 0; p < 3; ++p) {
+        m.chk(launch_fft_pass(h, p));
+        m.mark(p == 0 ? "k_r2c_rows" : p == 1 ? "k_col" : "k_c2r_rows");
+    }
+    if (fork) m.chk(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+    m.chk(hipGetLastError());
+    return m.e;
+}
+
+// One step from a state whose forces are ready: the update (gather, noise,
+// Euler step, refresh), then the two halves that only need its output side
+// by side:
+//   main stream: the list rebuild (rank, scan, fill, sort), then the WCA
+//                forces of the new lists;
+//   second stream: the starts of the previous lists kept, the deposit from
+//                the slot arrays (grouped by those starts, halo widened by
+//                the update's displacement bound), the FFT passes;
+// joined, and the step's end (counter, per-step scalars).  The new state
+// leaves with its forces ready.
+hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev,
+                        StepInfoDev *info) {
+    Marks m{h, names, ev};
+    if (ev) m.mark("begin");
+    const bool kap = !h->g.idgas;
+    const int ncell = h->g.ncell, nb = (int)nblk(ncell, SCAN_BLOCK);
+    if (!kap) {
+        m.chk(launch_rearm(h));
+        m.mark("k_rearm");
     }
     if (info) {
         // capillary forces on their own (the step fuses them into the update)
-        if (kap && !h->g.adaptive) chk(launch_gather(h, 0));
+        if (kap && !h->g.adaptive) m.chk(launch_gather(h, 0));
         const bool f32 = h->p.precision == BDK_FP32;
         if (f32) {
             hipLaunchKernelGGL(k_info_forces<float>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
@@ -2351,28 +2480,52 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         }
     }
     if (h->g.adaptive) {
-        chk(launch_gather(h, 1));
-        mark("k_gather");
-        chk(launch_update(h, 0));
+        m.chk(launch_gather(h, 1));
+        m.mark("k_gather");
+        m.chk(launch_update(h, 0));
     } else {
-        chk(launch_update(h, kap ? 1 : 0));
+        m.chk(launch_update(h, kap ? 1 : 0));
     }
-    mark("k_update");
+    m.mark("k_update");
     if (info) hipLaunchKernelGGL(k_info_moves, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
+    const bool fork = kap && !ev && h->aux;
+    hipStream_t side = fork ? h->aux : h->stream;
+    if (kap) {
+        if (fork) {
+            m.chk(hipEventRecord(h->ev_fork, h->stream));
+            m.chk(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
+        }
+        hipLaunchKernelGGL(k_keep_starts, dim3(nblk(ncell, 256)), dim3(256), 0, side, h->g, h->d);
+        m.mark("k_keep_starts");
+        if (fork) m.chk(hipEventRecord(h->ev_copy, side));
+        m.chk(launch_deposit(h, 0, 1, side));
+        m.mark("k_deposit");
+        for (int p = 0; p < 3; ++p) {
+            m.chk(launch_fft_pass(h, p, side));
+            m.mark(p == 0 ? "k_r2c_rows" : p == 1 ? "k_col" : "k_c2r_rows");
+        }
+        if (fork) m.chk(hipEventRecord(h->ev_join, side));
+    }
     hipLaunchKernelGGL(k_rank, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
-    mark("k_rank");
-    const int ncell = h->g.ncell;
-    const int nb = (int)nblk(ncell, SCAN_BLOCK);
+    m.mark("k_rank");
+    if (fork) m.chk(hipStreamWaitEvent(h->stream, h->ev_copy, 0));  // the previous starts are kept
     // bcount is current (the update adjusted it with the counts)
     hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start, ncell,
-                       nb, h->g.N, h->d.sc, 1);
-    mark("k_scan");
+                       nb, h->g.N, h->d.sc, 0);
+    m.mark("k_scan");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
-    mark("k_fill");
+    m.mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
-    mark("k_sort_cells");
-    chk(hipGetLastError());
-    return e;
+    m.mark("k_sort_cells");
+    if (kap) {
+        m.chk(launch_wca(h));
+        m.mark("k_wca");
+    }
+    if (fork) m.chk(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
+    m.mark("k_step_end");
+    m.chk(hipGetLastError());
+    return m.e;
 }
 
 // ------------------------------------------------------------------ batch plans
@@ -2439,8 +2592,10 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
     case 0:
         hipLaunchKernelGGL(kb_wca<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         break;
-    case 1:
-        hipLaunchKernelGGL(kb_deposit<R>, dim3(bp.nblk[BS_DEP]), dim3(DP_THREADS), bp.dep_lds, st, m, bp.map[BS_DEP]);
+    case 1:  // from the list order
+    case 8:  // from the slot arrays, beside the rebuild
+        hipLaunchKernelGGL(kb_deposit<R>, dim3(bp.nblk[BS_DEP]), dim3(DP_THREADS), bp.dep_lds, st, m, bp.map[BS_DEP],
+                           stage == 8 ? 1 : 0);
         break;
     case 2: {
 #define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E>), dim3(bp.nblk[BS_R2C]), dim3(FFT_THREADS), bp.l1, st, m, bp.map[BS_R2C])
@@ -2463,8 +2618,16 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
     case 5:
         hipLaunchKernelGGL(kb_update<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         break;
-    default: {
+    case 6:
         hipLaunchKernelGGL(kb_rank, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        break;
+    case 9:
+        hipLaunchKernelGGL(kb_keep_starts, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
+        break;
+    case 10:
+        hipLaunchKernelGGL(kb_step_end, dim3(bp.nmem), dim3(64), 0, st, m);
+        break;
+    default: {  // 11: scan, fill, sort
         hipLaunchKernelGGL(kb_scan, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
         hipLaunchKernelGGL(kb_fill, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         hipLaunchKernelGGL(kb_sort_cells, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
@@ -2473,7 +2636,9 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
     }
 }
 
-// stage 0 WCA, 1 deposit, 2 r2c, 3 col, 4 c2r, 5 update, 6 cell-list rebuild
+// stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update, 6 rank,
+// 8 deposit (slot arrays), 9 previous starts kept, 10 step end,
+// 11 scan + fill + sort
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)
         batch_stage_t<float>(bp, stage, st);
@@ -2482,10 +2647,9 @@ hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     return hipGetLastError();
 }
 
-// with aux (and the two events): the WCA stage runs on aux beside the mesh
-// stages, as in the single-simulation step
-hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
-                              hipEvent_t join) {
+// The members' forces: the WCA stage on aux beside the mesh stages.
+hipError_t batch_enqueue_forces(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
+                                hipEvent_t join) {
     hipError_t e = hipSuccess;
     if (aux) {
         e = hipEventRecord(fork, st);
@@ -2497,7 +2661,29 @@ hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t a
     }
     for (int k = 1; k <= 4 && e == hipSuccess; ++k) e = batch_stage(bp, k, st);
     if (aux && e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
-    for (int k = 5; k <= 6 && e == hipSuccess; ++k) e = batch_stage(bp, k, st);
+    return e;
+}
+
+// One batched step, laid out like the single-simulation step: update, then
+// the rebuild and the WCA on st beside the kept starts, the slot-order
+// deposit and the FFT stages on aux, joined, step end.
+hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
+                              hipEvent_t join, hipEvent_t copy) {
+    hipError_t e = batch_stage(bp, 5, st);
+    hipStream_t side = aux ? aux : st;
+    if (aux && e == hipSuccess) e = hipEventRecord(fork, st);
+    if (aux && e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
+    if (e == hipSuccess) e = batch_stage(bp, 9, side);
+    if (aux && e == hipSuccess) e = hipEventRecord(copy, side);
+    if (e == hipSuccess) e = batch_stage(bp, 8, side);
+    for (int k = 2; k <= 4 && e == hipSuccess; ++k) e = batch_stage(bp, k, side);
+    if (aux && e == hipSuccess) e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = batch_stage(bp, 6, st);
+    if (aux && e == hipSuccess) e = hipStreamWaitEvent(st, copy, 0);
+    if (e == hipSuccess) e = batch_stage(bp, 11, st);
+    if (e == hipSuccess) e = batch_stage(bp, 0, st);
+    if (aux && e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+    if (e == hipSuccess) e = batch_stage(bp, 10, st);
     return e;
 }
 
