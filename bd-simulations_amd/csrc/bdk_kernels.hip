@@ -1348,21 +1348,19 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
 // The update of one block of list positions.  The largest distance of a new
 // position outside its previous cell bounds where the next deposit finds the
 // particles when it stages them by their previous cells (during the list
-// rebuild): reduced per block, then one atomicMax per block into one of 64
-// slots (128 bytes apart), so no single address takes every block's atomic.
+// rebuild): reduced per wave, then one atomicMax per wave into one of 64
+// slots (128 bytes apart), skipped when the slot already holds as much.
 template <typename R, int SCHEME, bool FUSED>
 __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int blk) {
-    __shared__ double wm[16];
     const int s = blk * blockDim.x + threadIdx.x;  // list position in, slot out
     double m = 0.0;
     if (s < g.N) m = update_one<R, SCHEME, FUSED>(g, d, s);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, wm[w]);
-        if (m > 0.0) atomicMax(&d.exo[(blk & 63) * 16], bitsd(m));
+    if ((threadIdx.x & 63) == 0 && m > 0.0) {
+        unsigned long long *slot = &d.exo[((s >> 6) & 63) * 16];
+        const unsigned long long b = bitsd(m);
+        if (b > *(volatile unsigned long long *)slot) atomicMax(slot, b);
     }
 }
 
