@@ -596,15 +596,12 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
             if (l < llo || l >= lhi) continue;
             const int w = (k - klo) * nl + (l - llo);
             if (std::is_same<R, float>::value) {
-                float cf;
-                if (SCHEME == 0)
-                    cf = (float)(g.drho * (s2 ? a.q2 : a.q1));
-                else {
-                    cf = (float)wx[ia] * (float)wy[ib] * (float)g.drho;
-                    cf = cf * (float)(s2 ? a.q2 : a.q1);
-                }
-                const long long X = __double2ll_rn((double)cf * a.scale_hi);
-                atomicAdd(&acc_hi[w], (unsigned long long)X);
+                // the charge and its fixed point exactly as in the window path,
+                // so the mesh does not depend on which path a tile takes
+                const float dq = (float)g.drho * (float)(s2 ? a.q2 : a.q1);
+                const float cf = SCHEME == 0 ? dq : ((float)wx[ia] * (float)wy[ib]) * dq;
+                const int X = __float2int_rn(cf * a.scale_f);
+                atomicAdd(&acc_hi[w], (unsigned long long)(long long)X);
             } else {
                 double cd;
                 if (SCHEME == 0)
@@ -775,9 +772,10 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
         double v;
         if (win) {
             v = (double)(long long)acc_hi[((k - kw0) & (g.Z - 1)) * nwy + ((l - lw0) & (g.Z - 1))] * a.iscale_f;
+        } else if (std::is_same<R, float>::value) {
+            v = (double)(long long)acc_hi[w] * a.iscale_f;
         } else {
-            v = (double)(long long)acc_hi[w] * a.iscale_hi;
-            if (std::is_same<R, double>::value) v += (double)(long long)acc_lo[w] * a.iscale_lo;
+            v = (double)(long long)acc_hi[w] * a.iscale_hi + (double)(long long)acc_lo[w] * a.iscale_lo;
         }
         rho[(size_t)k * g.Z + l] = (R)v;
     }

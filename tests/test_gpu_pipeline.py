@@ -17,7 +17,7 @@ CASES = [
     # n1, n2, L, Z, scheme, prec, T, dt, steps
     (2000, 1500, 80.0, 128, bdk.TSC, bdk.FP32, 1.0, 1e-4, 30),
     (2000, 1500, 80.0, 128, bdk.CIC, bdk.FP64, 1.0, 1e-4, 30),
-    (150, 100, 16.0, 32, bdk.TSC, bdk.FP32, 1.0, 1e-4, 20),
+    (150, 100, 22.0, 32, bdk.TSC, bdk.FP32, 1.0, 1e-4, 20),
     (800, 800, 60.0, 64, bdk.NGP, bdk.FP32, 1.0, 1e-4, 20),
     (1500, 1000, 70.0, 128, bdk.TSC, bdk.FP32, 400.0, 2e-3, 10),
     (1500, 1000, 70.0, 128, bdk.CIC, bdk.FP64, 400.0, 2e-3, 10),
