@@ -119,10 +119,9 @@ Footprint footprint_of(const bd_params &p) {
 void release_streams(bdk_t *h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-    if (h->ev_copy) (void)hipEventDestroy(h->ev_copy);
     if (h->aux) (void)hipStreamDestroy(h->aux);
     if (h->stream) (void)hipStreamDestroy(h->stream);
-    h->ev_fork = h->ev_join = h->ev_copy = nullptr;
+    h->ev_fork = h->ev_join = nullptr;
     h->aux = h->stream = nullptr;
 }
 
@@ -337,7 +336,7 @@ struct bdk_batch {
     std::vector<bdk_t *> mem;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;  // mesh stages beside the list rebuild
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_copy = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int dev = 0;
     bdkx::BatchPlan bp;
     int ept_min[3] = {8, 8, 8};
@@ -389,7 +388,7 @@ struct bdk_batch {
         hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
         for (int i = 0; i < nsteps && e == hipSuccess; ++i)
-            e = bdkx::batch_enqueue_step(bp, stream, aux, ev_fork, ev_join, ev_copy);
+            e = bdkx::batch_enqueue_step(bp, stream, aux, ev_fork, ev_join);
         hipGraph_t gg = nullptr;
         hipError_t e2 = hipStreamEndCapture(stream, &gg);
         if (e == hipSuccess) e = e2;
@@ -548,7 +547,6 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
     ALLOC(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     ALLOC(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
-    ALLOC(hipEventCreateWithFlags(&h->ev_copy, hipEventDisableTiming));
     ALLOC(dalloc(d.cell, N));
     ALLOC(dalloc(d.rel, N));
     ALLOC(dalloc(d.pid, N));
@@ -1563,7 +1561,6 @@ int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_copy, hipEventDisableTiming);
     int ept[3];
     for (int k = 0; k < 3; ++k) ept[k] = bdkx::batch_min_ept(k, zmax);
     if (e == hipSuccess) e = b->rebuild(ept);
@@ -1585,7 +1582,6 @@ void bdk_batch_destroy(bdk_batch_t *b) {
     b->free_tables();
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-    if (b->ev_copy) (void)hipEventDestroy(b->ev_copy);
     if (b->aux) (void)hipStreamDestroy(b->aux);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -138,7 +138,7 @@ struct bdk {
     bdkx::DevBufs d;
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;            // second stream of a step (mesh chain beside the list rebuild)
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_copy = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int dev = 0;
     bool pos_set[2] = {false, false};
     bool dev_valid = false;       // device particle state uploaded
@@ -163,7 +163,8 @@ hipError_t launch_deposit(bdk_t *h, int reset, int slot = 0, hipStream_t st = nu
 hipError_t launch_rearm(bdk_t *h);                // per-step scalars when no deposit runs
 size_t deposit_lds_bytes(const bdk_t *h);          // LDS of one deposit block
 hipError_t launch_fft(bdk_t *h);                  // rho -> fmesh (r2c, multiply, c2r)
-hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st = nullptr);  // 0 r2c rows, 1 columns, 2 c2r rows
+// 0 r2c rows, 1 columns, 2 c2r rows (end_step: the c2r pass also ends the step)
+hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st = nullptr, int end_step = 0);
 hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)
 hipError_t launch_update(bdk_t *h, int fused);    // noise, Euler step, refresh, counting
 hipError_t launch_bin(bdk_t *h);                  // counting after an upload/refresh
@@ -185,10 +186,10 @@ int batch_min_ept(int stage, int zmax);
 hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
                        std::vector<int2> maps[BS_COUNT]);
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st);
-// one batched step of members whose forces are ready; with aux and the three
+// one batched step of members whose forces are ready; with aux and the two
 // events the mesh stages run beside the list rebuild
 hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux = nullptr,
-                              hipEvent_t fork = nullptr, hipEvent_t join = nullptr, hipEvent_t copy = nullptr);
+                              hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 // the forces of the members' current states (after an upload)
 hipError_t batch_enqueue_forces(const BatchPlan &bp, hipStream_t st, hipStream_t aux = nullptr,
                                 hipEvent_t fork = nullptr, hipEvent_t join = nullptr);

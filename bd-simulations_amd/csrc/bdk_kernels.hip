@@ -787,6 +787,21 @@ __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArg
     deposit_body<R, SCHEME>(g, d, a, blockIdx.x + blockIdx.y * gridDim.x, gridDim.x);
 }
 
+// End of a step: step counter + 1, and the per-step scalars of the next
+// update re-armed (dt minimum, excursion slot, displacement bound).  Run by
+// threads 0..63 of one block of the step's last mesh pass (k_c2r_rows), which
+// follows every reader of these scalars in the step.
+__device__ __forceinline__ void step_end_body(const Geo &g, const DevBufs &d) {
+    const int t = threadIdx.x;
+    if (t < 64) d.exo[t * 16] = 0ull;
+    if (t == 0) {
+        const unsigned long long s = d.sc->step;
+        d.sc->dt_bits = bitsd(g.dt_max);
+        d.sc->exc_bits[s & 1] = 0ull;  // filled by the update of step s + 1
+        d.sc->step = s + 1;
+    }
+}
+
 // per-step scalar re-arm when no deposit runs (ideal-gas runs)
 __global__ void k_rearm(Geo g, DevScalars *sc) {
     const unsigned long long step = sc->step;
@@ -1144,7 +1159,9 @@ template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
                                                           typename V2<R>::T *__restrict__ fm,
                                                           const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
-                                                          int RB2, int stride, int remap) {
+                                                          int RB2, int stride, int remap, Geo g, DevBufs d,
+                                                          int end_step) {
+    if (end_step && blockIdx.x == 0) step_end_body(g, d);
     c2r_rows_body<R, EPT>(gt, fm, tw, Z, logZ, RB2, stride, xcd_block(remap));
 }
 
@@ -1351,6 +1368,10 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
 template <typename R, int SCHEME, bool FUSED>
 __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int blk) {
     const int s = blk * blockDim.x + threadIdx.x;  // list position in, slot out
+    // the starts that group the slot arrays after this update (the previous
+    // list order), kept for the deposit beside the rebuild
+    const int stride = ((g.N + blockDim.x - 1) / blockDim.x) * blockDim.x;
+    for (int c = s; c <= g.ncell; c += stride) d.start_old[c] = d.start[c];
     double m = 0.0;
     if (s < g.N) m = update_one<R, SCHEME, FUSED>(g, d, s);
 #pragma unroll
@@ -1607,30 +1628,6 @@ __device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, 
 
 __global__ void k_sort_cells(Geo g, DevBufs d, int moved_ranks) { sort_cells_body(g, d, moved_ranks, blockIdx.x); }
 
-// The starts that group the slot arrays after an update (the previous list
-// order), kept for the deposit that runs beside the rebuild.
-__device__ __forceinline__ void keep_starts_body(const Geo &g, const DevBufs &d, int blk) {
-    const int c = blk * blockDim.x + threadIdx.x;
-    if (c < g.ncell) d.start_old[c] = d.start[c];
-    if (c == 0) d.start_old[g.ncell] = d.start[g.ncell];
-}
-
-__global__ void k_keep_starts(Geo g, DevBufs d) { keep_starts_body(g, d, blockIdx.x); }
-
-// End of a step: step counter + 1, and the per-step scalars of the next
-// update re-armed (dt minimum, excursion slot, displacement bound).
-__device__ __forceinline__ void step_end_body(const Geo &g, const DevBufs &d) {
-    const int t = threadIdx.x;
-    if (t < 64) d.exo[t * 16] = 0ull;
-    if (t == 0) {
-        const unsigned long long s = d.sc->step;
-        d.sc->dt_bits = bitsd(g.dt_max);
-        d.sc->exc_bits[s & 1] = 0ull;  // filled by the update of step s + 1
-        d.sc->step = s + 1;
-    }
-}
-
-__global__ void k_step_end(Geo g, DevBufs d) { step_end_body(g, d); }
 
 // ------------------------------------------------------------------ batches
 // A batch advances many independent simulations (members) with one launch
@@ -1689,11 +1686,12 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int
 }
 
 template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map) {
+__global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map, int end_step) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     const int2 t = map[blockIdx.x];
     const RepDev &r = m[t.x];
+    if (end_step && t.y == 0) step_end_body(r.g, r.d);
     c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb2, r.sR2, t.y);
 }
 
@@ -1729,13 +1727,6 @@ __global__ void kb_sort_cells(const RepDev *m, const int2 *map) {
     sort_cells_body(m[t.x].g, m[t.x].d, 1, t.y);
 }
 
-__global__ void kb_keep_starts(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    keep_starts_body(m[t.x].g, m[t.x].d, t.y);
-}
-
-// one block of 64 threads per member
-__global__ void kb_step_end(const RepDev *m) { step_end_body(m[blockIdx.x].g, m[blockIdx.x].d); }
 
 // ------------------------------------------------------------------ launchers
 static inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1943,7 +1934,7 @@ hipError_t launch_rearm(bdk_t *h) {
     default: CALL(32); break;   \
     }
 
-template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStream_t st = nullptr) {
+template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStream_t st = nullptr, int end_step = 0) {
     if (!st) st = h->stream;
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
@@ -1959,7 +1950,8 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStrea
                        (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC)
 #define C2R(E)                                                                                              \
     hipLaunchKernelGGL((k_c2r_rows<R, E>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,            \
-                       (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2, h->g.xcd)
+                       (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2, h->g.xcd, h->g, h->d,  \
+                       end_step)
     if (only < 0 || only == 0) BDK_EPT_SWITCH(c.e1, R2C)
     if (only < 0 || only == 1) BDK_EPT_SWITCH(c.e2, COL)
     if (only < 0 || only == 2) BDK_EPT_SWITCH(c.e3, C2R)
@@ -1968,11 +1960,11 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStrea
 #undef C2R
 }
 
-hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st) {
+hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st, int end_step) {
     if (h->p.precision == BDK_FP32)
-        launch_fft_t<float>(h, pass, st);
+        launch_fft_t<float>(h, pass, st, end_step);
     else
-        launch_fft_t<double>(h, pass, st);
+        launch_fft_t<double>(h, pass, st, end_step);
     return hipGetLastError();
 }
 
@@ -2448,11 +2440,12 @@ hipError_t enqueue_forces(bdk_t *h, std::vector<std::string> *names, std::vector
 // by side:
 //   main stream: the list rebuild (rank, scan, fill, sort), then the WCA
 //                forces of the new lists;
-//   second stream: the starts of the previous lists kept, the deposit from
-//                the slot arrays (grouped by those starts, halo widened by
-//                the update's displacement bound), the FFT passes;
-// joined, and the step's end (counter, per-step scalars).  The new state
-// leaves with its forces ready.
+//   second stream: the deposit from the slot arrays (grouped by the previous
+//                starts, which the update kept; halo widened by the
+//                update's displacement bound), the FFT passes, the last of
+//                which ends the step (counter, per-step scalars);
+// joined.  The new state leaves with its forces ready.  Without forces (an
+// ideal gas) the scan ends the step.
 hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<hipEvent_t> *ev,
                         StepInfoDev *info) {
     Marks m{h, names, ev};
@@ -2491,23 +2484,20 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
             m.chk(hipEventRecord(h->ev_fork, h->stream));
             m.chk(hipStreamWaitEvent(h->aux, h->ev_fork, 0));
         }
-        hipLaunchKernelGGL(k_keep_starts, dim3(nblk(ncell, 256)), dim3(256), 0, side, h->g, h->d);
-        m.mark("k_keep_starts");
-        if (fork) m.chk(hipEventRecord(h->ev_copy, side));
         m.chk(launch_deposit(h, 0, 1, side));
         m.mark("k_deposit");
         for (int p = 0; p < 3; ++p) {
-            m.chk(launch_fft_pass(h, p, side));
+            m.chk(launch_fft_pass(h, p, side, p == 2));
             m.mark(p == 0 ? "k_r2c_rows" : p == 1 ? "k_col" : "k_c2r_rows");
         }
         if (fork) m.chk(hipEventRecord(h->ev_join, side));
     }
     hipLaunchKernelGGL(k_rank, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
     m.mark("k_rank");
-    if (fork) m.chk(hipStreamWaitEvent(h->stream, h->ev_copy, 0));  // the previous starts are kept
-    // bcount is current (the update adjusted it with the counts)
+    // bcount is current (the update adjusted it with the counts); the
+    // update also kept the previous starts for the deposit
     hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start, ncell,
-                       nb, h->g.N, h->d.sc, 0);
+                       nb, h->g.N, h->d.sc, !kap);
     m.mark("k_scan");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     m.mark("k_fill");
@@ -2518,8 +2508,6 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         m.mark("k_wca");
     }
     if (fork) m.chk(hipStreamWaitEvent(h->stream, h->ev_join, 0));
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
-    m.mark("k_step_end");
     m.chk(hipGetLastError());
     return m.e;
 }
@@ -2605,8 +2593,10 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
 #undef COL
         break;
     }
-    case 4: {
-#define C2R(E) hipLaunchKernelGGL((kb_c2r<R, E>), dim3(bp.nblk[BS_C2R]), dim3(FFT_THREADS), bp.l2, st, m, bp.map[BS_C2R])
+    case 4:
+    case 12: {  // 12: also the end of the step
+#define C2R(E) hipLaunchKernelGGL((kb_c2r<R, E>), dim3(bp.nblk[BS_C2R]), dim3(FFT_THREADS), bp.l2, st, m, bp.map[BS_C2R], \
+                              stage == 12 ? 1 : 0)
         BDK_EPT_SWITCH2(bp.ept[2], C2R)
 #undef C2R
         break;
@@ -2616,12 +2606,6 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
         break;
     case 6:
         hipLaunchKernelGGL(kb_rank, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
-        break;
-    case 9:
-        hipLaunchKernelGGL(kb_keep_starts, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
-        break;
-    case 10:
-        hipLaunchKernelGGL(kb_step_end, dim3(bp.nmem), dim3(64), 0, st, m);
         break;
     default: {  // 11: scan, fill, sort
         hipLaunchKernelGGL(kb_scan, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
@@ -2633,8 +2617,7 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
 }
 
 // stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update, 6 rank,
-// 8 deposit (slot arrays), 9 previous starts kept, 10 step end,
-// 11 scan + fill + sort
+// 8 deposit (slot arrays), 11 scan + fill + sort, 12 c2r + step end
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)
         batch_stage_t<float>(bp, stage, st);
@@ -2661,25 +2644,22 @@ hipError_t batch_enqueue_forces(const BatchPlan &bp, hipStream_t st, hipStream_t
 }
 
 // One batched step, laid out like the single-simulation step: update, then
-// the rebuild and the WCA on st beside the kept starts, the slot-order
-// deposit and the FFT stages on aux, joined, step end.
+// the rebuild and the WCA on st beside the slot-order deposit and the FFT
+// stages on aux (the last of which ends the step), joined.
 hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
-                              hipEvent_t join, hipEvent_t copy) {
+                              hipEvent_t join) {
     hipError_t e = batch_stage(bp, 5, st);
     hipStream_t side = aux ? aux : st;
     if (aux && e == hipSuccess) e = hipEventRecord(fork, st);
     if (aux && e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);
-    if (e == hipSuccess) e = batch_stage(bp, 9, side);
-    if (aux && e == hipSuccess) e = hipEventRecord(copy, side);
     if (e == hipSuccess) e = batch_stage(bp, 8, side);
-    for (int k = 2; k <= 4 && e == hipSuccess; ++k) e = batch_stage(bp, k, side);
+    for (int k = 2; k <= 3 && e == hipSuccess; ++k) e = batch_stage(bp, k, side);
+    if (e == hipSuccess) e = batch_stage(bp, 12, side);
     if (aux && e == hipSuccess) e = hipEventRecord(join, side);
     if (e == hipSuccess) e = batch_stage(bp, 6, st);
-    if (aux && e == hipSuccess) e = hipStreamWaitEvent(st, copy, 0);
     if (e == hipSuccess) e = batch_stage(bp, 11, st);
     if (e == hipSuccess) e = batch_stage(bp, 0, st);
     if (aux && e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
-    if (e == hipSuccess) e = batch_stage(bp, 10, st);
     return e;
 }
 
