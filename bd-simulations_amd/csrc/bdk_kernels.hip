@@ -1,21 +1,25 @@
 // HIP kernels of the MI355X Brownian-dynamics timestep (gfx950).
 //
-// One step (dynamik_methoden.cpp:74-163) is this launch sequence:
-//   k_wca       WCA forces (dm:617-1059), one thread per particle;
-//   k_deposit   charge deposit (gr:31-547) per tile of list cells into the
-//               tile's own mesh points, integer LDS accumulation;
-//   k_r2c_rows  row FFTs of the real mesh, transposed half-spectrum out;
-//   k_col       column FFT, Green's function x gradient (dm:1210-1216), two
-//               inverse column FFTs (Fx, Fy);
-//   k_c2r_rows  inverse row FFT of (Fx + i Fy) -> real Fx, Fy mesh;
+// One step (dynamik_methoden.cpp:74-163) starts from a state whose forces
+// are on the device and leaves the next state with its forces:
 //   k_update    gather (gr:81-668), noise (dm:1317-1339), Euler step
 //               (dm:116-157), refresh (dm:216-325); the cell counts are
 //               updated by the particles that change cell;
-//   k_rank      rank of each particle that stayed in its cell (no atomics);
-//   k_scan_*    exclusive scan of the cell counts;
+// then on two streams side by side:
+//   k_rank_scan rank of each particle that stayed in its cell (no atomics)
+//               beside the exclusive scan of the cell counts;
 //   k_fill      cell list = slots grouped by cell, species in bit 31, and
 //               the particle state in list order (lcell, lrel, lpid, lrelf);
-//   k_sort_cells  the arrivals of each cell in slot order.
+//   k_sort_cells  the arrivals of each cell in slot order;
+//   k_wca       WCA forces (dm:617-1059) of the new lists;
+// and
+//   k_deposit   charge deposit (gr:31-547) per tile of cells into the tile's
+//               own mesh points, integer LDS accumulation, here from the
+//               slot arrays (grouped by the previous cells);
+//   k_r2c_rows  row FFTs of the real mesh, transposed half-spectrum out;
+//   k_col       column FFT, Green's function x gradient (dm:1210-1216), two
+//               inverse column FFTs (Fx, Fy);
+//   k_c2r_rows  inverse row FFT of (Fx + i Fy) -> real Fx, Fy mesh.
 // Every force, gather and update kernel runs over list positions q and
 // reads the list-ordered state; k_update writes the new state to slot q.
 // The slots are therefore re-sorted by cell every step: consecutive threads
@@ -1545,19 +1549,27 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *__restrict__ c
 
 // Rank of a particle that stayed in its cell during the step: its position
 // among the cell's previous entries, minus the entries before it that left.
-// Runs before the scan overwrites the previous starts.
+// Reads the previous starts from the update's copy, so it runs beside the
+// scan that overwrites start.
 __device__ __forceinline__ void rank_body(const Geo &g, const DevBufs &d, int blk) {
     const int q = blk * blockDim.x + threadIdx.x;
     if (q >= g.N) return;
     const int c = d.lcell[q];
     if (d.cell[q] != c) return;  // moved: ranked among the arrivals
-    const int b = d.start[c];
+    const int b = d.start_old[c];
     int r = q - b;
     for (int p = b; p < q; ++p) r -= d.cell[p] != d.lcell[p];
     d.rank[q] = r;
 }
 
-__global__ void k_rank(Geo g, DevBufs d) { rank_body(g, d, blockIdx.x); }
+// the ranks and the scan of the new counts as one launch: blocks [0, nrank)
+// rank, the others scan
+__global__ __launch_bounds__(256) void k_rank_scan(Geo g, DevBufs d, int nrank, int nb, int inc_step) {
+    if ((int)blockIdx.x < nrank)
+        rank_body(g, d, blockIdx.x);
+    else
+        scan_body(d.count, d.bcount, d.start, g.ncell, nb, g.N, d.sc, inc_step, blockIdx.x - nrank);
+}
 
 // List entries and the list-ordered state from the ranks.  After a step
 // (moved_ranks) a cell holds its stayers first (ascending slot, from
@@ -1706,16 +1718,17 @@ __global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *ma
     }
 }
 
-__global__ void kb_rank(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    rank_body(m[t.x].g, m[t.x].d, t.y);
+__global__ __launch_bounds__(256) void kb_rank_scan(const RepDev *m, const int2 *mp, int nrank, const int2 *ms) {
+    if ((int)blockIdx.x < nrank) {
+        const int2 t = mp[blockIdx.x];
+        rank_body(m[t.x].g, m[t.x].d, t.y);
+    } else {
+        const int2 t = ms[blockIdx.x - nrank];
+        const RepDev &r = m[t.x];
+        scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
+    }
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
-}
 
 __global__ void kb_fill(const RepDev *m, const int2 *map) {
     const int2 t = map[blockIdx.x];
@@ -2492,13 +2505,11 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         }
         if (fork) m.chk(hipEventRecord(h->ev_join, side));
     }
-    hipLaunchKernelGGL(k_rank, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d);
-    m.mark("k_rank");
-    // bcount is current (the update adjusted it with the counts); the
-    // update also kept the previous starts for the deposit
-    hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start, ncell,
-                       nb, h->g.N, h->d.sc, !kap);
-    m.mark("k_scan");
+    // bcount is current (the update adjusted it with the counts); the ranks
+    // read the previous starts that the update kept
+    const int nrank = (int)nblk(h->g.N, 256);
+    hipLaunchKernelGGL(k_rank_scan, dim3(nrank + nb), dim3(256), 0, h->stream, h->g, h->d, nrank, nb, !kap);
+    m.mark("k_rank_scan");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     m.mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
@@ -2605,10 +2616,10 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
         hipLaunchKernelGGL(kb_update<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         break;
     case 6:
-        hipLaunchKernelGGL(kb_rank, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        hipLaunchKernelGGL(kb_rank_scan, dim3(bp.nblk[BS_PART] + bp.nblk[BS_SCAN]), dim3(256), 0, st, m,
+                           bp.map[BS_PART], bp.nblk[BS_PART], bp.map[BS_SCAN]);
         break;
-    default: {  // 11: scan, fill, sort
-        hipLaunchKernelGGL(kb_scan, dim3(bp.nblk[BS_SCAN]), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN]);
+    default: {  // 11: fill, sort
         hipLaunchKernelGGL(kb_fill, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         hipLaunchKernelGGL(kb_sort_cells, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
         break;
@@ -2616,8 +2627,8 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
     }
 }
 
-// stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update, 6 rank,
-// 8 deposit (slot arrays), 11 scan + fill + sort, 12 c2r + step end
+// stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update,
+// 6 rank + scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)
         batch_stage_t<float>(bp, stage, st);
