@@ -132,6 +132,8 @@ void free_dev(bdk_t *h) {
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    if (h->info_dev) (void)hipFree(h->info_dev);
+    h->info_dev = nullptr;
     h->d = bdkx::DevBufs();
 }
 
@@ -785,8 +787,8 @@ int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out) {
     if (!(tmax > 0.0)) return fail(h, BDK_ERR_ARG, "Fehler: Negative Zeitschrittweite gefordert (tmax <= 0)");
     int rc = ensure_forces(h);
     if (rc) return rc;
-    bdkx::StepInfoDev *dinfo = nullptr;
-    HIPCHK(h, hipMalloc((void **)&dinfo, sizeof(bdkx::StepInfoDev)));
+    if (!h->info_dev) HIPCHK(h, hipMalloc((void **)&h->info_dev, sizeof(bdkx::StepInfoDev)));
+    bdkx::StepInfoDev *dinfo = h->info_dev;
     bdkx::StepInfoDev hi;
     hipError_t e = hipMemsetAsync(dinfo, 0, sizeof hi, h->stream);
     if (e == hipSuccess && h->g.idgas)
@@ -796,7 +798,6 @@ int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out) {
     if (e == hipSuccess) e = hipMemcpyAsync(&hi, dinfo, sizeof hi, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(&sc, h->d.sc, sizeof sc, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(dinfo);
     if (e != hipSuccess) return hipfail(h, e, "bdk_step_info");
     h->host_step++;
     const int n1 = h->p.n1;
@@ -989,13 +990,19 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
     int rc = ensure_forces(h);
     if (rc) return rc;
     if (reps < 1 || kernel < 0 || kernel > 7 || !ms_per_launch) return fail(h, BDK_ERR_ARG, "bad profile request");
-    // k_update changes positions, cells, ranks, counts and scalars: snapshot them
+    // k_update changes positions, cells, ranks, counts and scalars: snapshot
+    // them (the copies are freed on every return path)
     struct Snap {
         void *p;
         void *src;
         size_t n;
     };
-    std::vector<Snap> snaps;
+    struct Snaps {
+        std::vector<Snap> v;
+        ~Snaps() {
+            for (Snap &sn : v) (void)hipFree(sn.p);
+        }
+    } snaps;
     if (kernel == BDK_K_UPDATE) {
         const size_t N = (size_t)h->g.N, nc2 = (size_t)h->g.ncell;
         Snap list[] = {{nullptr, h->d.cell, N * 4}, {nullptr, h->d.rel, N * 16}, {nullptr, h->d.pid, N * 4},
@@ -1005,8 +1012,8 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
                        {nullptr, h->d.sc, sizeof(DevScalars)}};
         for (Snap sn : list) {
             HIPCHK(h, hipMalloc(&sn.p, sn.n));
+            snaps.v.push_back(sn);
             HIPCHK(h, hipMemcpyAsync(sn.p, sn.src, sn.n, hipMemcpyDeviceToDevice, h->stream));
-            snaps.push_back(sn);
         }
         // forces of the current state for the update to consume
         HIPCHK(h, bdkx::launch_wca(h));
@@ -1031,24 +1038,25 @@ int bdk_profile_kernel(bdk_t *h, int kernel, int32_t reps, double *ms_per_launch
             return bdkx::enqueue_step(h, nullptr, nullptr);
         }
     };
+    struct Events {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
     HIPCHK(h, launch());  // warm-up
-    hipEvent_t e0, e1;
-    HIPCHK(h, hipEventCreate(&e0));
-    HIPCHK(h, hipEventCreate(&e1));
-    HIPCHK(h, hipEventRecord(e0, h->stream));
+    HIPCHK(h, hipEventCreate(&ev.e0));
+    HIPCHK(h, hipEventCreate(&ev.e1));
+    HIPCHK(h, hipEventRecord(ev.e0, h->stream));
     for (int i = 0; i < reps; ++i) HIPCHK(h, launch());
-    HIPCHK(h, hipEventRecord(e1, h->stream));
-    HIPCHK(h, hipEventSynchronize(e1));
+    HIPCHK(h, hipEventRecord(ev.e1, h->stream));
+    HIPCHK(h, hipEventSynchronize(ev.e1));
     float ms = 0.f;
-    HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(h, hipEventElapsedTime(&ms, ev.e0, ev.e1));
     if (kernel == BDK_K_STEP) h->host_step += reps + 1;
-    for (Snap &sn : snaps) {
-        HIPCHK(h, hipMemcpyAsync(sn.src, sn.p, sn.n, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        (void)hipFree(sn.p);
-    }
+    for (Snap &sn : snaps.v) HIPCHK(h, hipMemcpyAsync(sn.src, sn.p, sn.n, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     *ms_per_launch = ms / reps;
     return check_nonfinite(h);
 }

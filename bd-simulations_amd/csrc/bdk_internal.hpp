@@ -132,6 +132,10 @@ struct BatchPlan {
 
 }  // namespace bdkx
 
+namespace bdkx {
+struct StepInfoDev;
+}
+
 struct bdk {
     bd_params p;
     bdkx::Geo g;
@@ -152,6 +156,7 @@ struct bdk {
     hipGraphExec_t gexec_multi = nullptr;
     std::string err;
     int64_t host_step = 0;
+    bdkx::StepInfoDev *info_dev = nullptr;  // step counters (bdk_step_info), allocated on first use
 };
 
 namespace bdkx {
