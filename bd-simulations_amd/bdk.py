@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbdk_hip.so")
 
 NGP, CIC, TSC = 0, 1, 2
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_POS32, FP32_POS32K = 0, 1, 2, 3
 NOISE_DISC, NOISE_GAUSS, NOISE_UPLOAD = 0, 1, 2
 GREEN_DISCRETE, GREEN_CONTINUOUS = 0, 1
 MESH_RHO, MESH_FX, MESH_FY = 0, 1, 2

@@ -83,7 +83,8 @@ std::string validate(const bd_params &p) {
     if (!(p.T >= 0.0)) return "T must be >= 0";
     if (!(p.dt_max > 0.0)) return "dt_max must be positive";
     if (!(p.max_travel > 0.0)) return "max_travel must be positive (+inf for fixed dt)";
-    if (p.precision != BDK_FP64 && p.precision != BDK_FP32) return "precision must be BDK_FP64 or BDK_FP32";
+    if (p.precision < BDK_FP64 || p.precision > BDK_FP32_POS32K)
+        return "precision must be BDK_FP64, BDK_FP32, BDK_FP32_POS32 or BDK_FP32_POS32K";
     if (p.noise_mode < 0 || p.noise_mode > 2) return "noise_mode must be DISC, GAUSS or UPLOAD";
     if (p.green_mode < 0 || p.green_mode > 1) return "green_mode must be DISCRETE or CONTINUOUS";
     if (p.noise_mode == BDK_NOISE_GAUSS && !(p.noise_var >= 0.0)) return "noise_var must be >= 0";
@@ -96,7 +97,7 @@ std::string validate(const bd_params &p) {
     return "";
 }
 
-size_t real_size(const bd_params &p) { return p.precision == BDK_FP32 ? 4 : 8; }
+size_t real_size(const bd_params &p) { return p.precision != BDK_FP64 ? 4 : 8; }
 
 struct Footprint {
     size_t particles, cells, mesh, total;
@@ -108,7 +109,8 @@ Footprint footprint_of(const bd_params &p) {
     const size_t ncell = (size_t)d.nc * d.nc, Z = (size_t)p.Z;
     Footprint f;
     // cell, rel, pid, fwca, fkap, noise_up, noise_out, rank, list
-    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4 + 8);
+    f.particles = N * (4 + 16 + 4 + 2 * R + 2 * R + 16 + 16 + 4 + 4 + 4 + 16 + 4 + 8) +
+                  (p.precision == BDK_FP32_POS32K ? 2 * (N + 1) * 8 : 0);  // Kahan compensation
     f.cells = ncell * 16 + 8 + 4096 * 32 * 4 + 64 * 16 * 8;
     // rho, ht, gt, fmesh, tables
     f.mesh = Z * Z * R + (Z / 2) * Z * 2 * R + (Z / 2) * Z * 4 * R + Z * Z * 2 * R + Z * (2 * R + 3 * 8 + 3 * 4);
@@ -127,7 +129,7 @@ void release_streams(bdk_t *h) {
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.comp, h->d.lcomp, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -172,7 +174,7 @@ int build_tables_host(bdk_t *h) {
     HIPCHK(h, hipMemcpy(h->d.lapf, lapf.data(), Z * 4, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d.qvf, qvf.data(), Z * 4, hipMemcpyHostToDevice));
     // twiddles exp(-2 pi i k / Z)
-    if (p.precision == BDK_FP32) {
+    if (p.precision != BDK_FP64) {
         std::vector<float> tw(2 * Z);
         for (int k = 0; k < Z; ++k) {
             double a = 2.0 * M_PI * k / Z;
@@ -222,6 +224,10 @@ int upload(bdk_t *h) {
                 return fail(h, BDK_ERR_NONFINITE, "non-finite rel for species " + std::to_string(sp + 1) +
                                                       " particle " + std::to_string(i));
             cell[off + i] = gx * nc + gy;
+            if (h->g.pos_mode) {  // FP32 storage of the in-cell positions
+                rx = (double)(float)rx;
+                ry = (double)(float)ry;
+            }
             rel[2 * (size_t)(off + i)] = rx;
             rel[2 * (size_t)(off + i) + 1] = ry;
         }
@@ -241,6 +247,7 @@ int upload(bdk_t *h) {
     HIPCHK(h, hipMemcpyAsync(h->d.cell, scell.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d.pid, spid.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d.rel, srel.data(), (size_t)N * 16, hipMemcpyHostToDevice, h->stream));
+    if (h->d.comp) HIPCHK(h, hipMemsetAsync(h->d.comp, 0, (size_t)N * 8, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d.count, 0, (size_t)h->g.ncell * 4, h->stream));
     HIPCHK(h, bdkx::launch_bin(h));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -287,7 +294,7 @@ int download_r2(bdk_t *h, const void *src, double *f1, double *f2) {
 }
 
 int download_forces(bdk_t *h, const void *src, double *f1, double *f2) {
-    return h->p.precision == BDK_FP32 ? download_r2<float>(h, src, f1, f2) : download_r2<double>(h, src, f1, f2);
+    return h->p.precision != BDK_FP64 ? download_r2<float>(h, src, f1, f2) : download_r2<double>(h, src, f1, f2);
 }
 
 // steps per replay of the multi-step graph
@@ -527,8 +534,9 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     g.kap = p->kap;
     g.zero_mode_g = p->lambda > 0.2 * p->L;
     g.ncinv = 1.0f / (float)g.nc;
-    g.fast_wca = (p->precision == BDK_FP32 && g.nc >= 6) ? 1 : 0;
-    g.f32noise = p->precision == BDK_FP32 ? 1 : 0;
+    g.fast_wca = (p->precision != BDK_FP64 && g.nc >= 6) ? 1 : 0;
+    g.f32noise = p->precision != BDK_FP64 ? 1 : 0;
+    g.pos_mode = p->precision == BDK_FP32_POS32 ? 1 : (p->precision == BDK_FP32_POS32K ? 2 : 0);
     g.hinv = 1.0 / g.h;
     {
         // BDK_XCD_REMAP=0 keeps the hardware workgroup order (for comparison)
@@ -561,6 +569,10 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     ALLOC(dalloc(d.arr, ncell));
     ALLOC(dalloc(d.start, ncell + 1));
     ALLOC(dalloc(d.start_old, ncell + 1));
+    if (h->g.pos_mode == 2) {
+        ALLOC(dalloc(d.comp, N + 1));
+        ALLOC(dalloc(d.lcomp, N + 1));
+    }
     ALLOC(dalloc(d.exo, 64 * 16));
     // list-ordered arrays carry one padding entry (read ahead by k_wca_f32)
     ALLOC(dalloc(d.list, N + 1));
@@ -596,6 +608,8 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     e = hipMemcpy(d.sc, &sc, sizeof(sc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(d.count, 0, ncell * 4);
     if (e == hipSuccess) e = hipMemset(d.exo, 0, 64 * 16 * 8);
+    if (e == hipSuccess && d.comp) e = hipMemset(d.comp, 0, (N + 1) * 8);
+    if (e == hipSuccess && d.lcomp) e = hipMemset(d.lcomp, 0, (N + 1) * 8);
     if (e == hipSuccess) e = hipMemset(d.arr, 0, ncell * 4);
     if (e == hipSuccess) e = hipMemset(d.list, 0, (N + 1) * 4);
     if (e == hipSuccess) e = hipMemset(d.lrelf, 0, (N + 1) * 8);
@@ -923,7 +937,7 @@ int bdk_get_mesh(bdk_t *h, int which, double *out) {
     if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
     if (!out || which < 0 || which > 2) return fail(h, BDK_ERR_ARG, "bad mesh request");
     const size_t Z2 = (size_t)h->g.Z * h->g.Z;
-    const bool f32 = h->p.precision == BDK_FP32;
+    const bool f32 = h->p.precision != BDK_FP64;
     if (which == BDK_MESH_RHO) {
         if (f32) {
             std::vector<float> b(Z2);
@@ -1364,6 +1378,9 @@ int bdk_save_state(bdk_t *h, const char *path) {
     HIPCHK(h, hipMemcpyAsync(cell.data(), h->d.lcell, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(pid.data(), h->d.lpid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(rel.data(), h->d.lrel, (size_t)N * 16, hipMemcpyDeviceToHost, h->stream));
+    std::vector<float> comp(h->d.lcomp ? 2 * (size_t)N : 0);  // Kahan compensation (POS32K rung)
+    if (h->d.lcomp)
+        HIPCHK(h, hipMemcpyAsync(comp.data(), h->d.lcomp, (size_t)N * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     FILE *f = std::fopen(path, "wb");
@@ -1375,7 +1392,8 @@ int bdk_save_state(bdk_t *h, const char *path) {
     bool ok = std::fwrite(kStateMagic, 1, 8, f) == 8 && std::fwrite(&p, sizeof p, 1, f) == 1 &&
               std::fwrite(&step, 8, 1, f) == 1 && std::fwrite(&hs, 8, 1, f) == 1 &&
               std::fwrite(cell.data(), 4, N, f) == (size_t)N && std::fwrite(rel.data(), 16, N, f) == (size_t)N &&
-              std::fwrite(pid.data(), 4, N, f) == (size_t)N;
+              std::fwrite(pid.data(), 4, N, f) == (size_t)N &&
+              (comp.empty() || std::fwrite(comp.data(), 8, N, f) == (size_t)N);
     ok = (std::fclose(f) == 0) && ok;
     return ok ? BDK_OK : fail(h, BDK_ERR_ARG, std::string("write failed: ") + path);
 }
@@ -1391,6 +1409,7 @@ int bdk_load_state(bdk_t *h, const char *path) {
     const int N = h->g.N;
     std::vector<int32_t> cell(N), pid(N);
     std::vector<double> rel(2 * (size_t)N);
+    std::vector<float> comp;
     bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kStateMagic, 8) == 0 &&
               std::fread(&p, sizeof p, 1, f) == 1 && std::fread(&step, 8, 1, f) == 1 && std::fread(&hs, 8, 1, f) == 1;
     if (ok) {
@@ -1403,6 +1422,10 @@ int bdk_load_state(bdk_t *h, const char *path) {
         }
         ok = std::fread(cell.data(), 4, N, f) == (size_t)N && std::fread(rel.data(), 16, N, f) == (size_t)N &&
              std::fread(pid.data(), 4, N, f) == (size_t)N;
+        if (ok && h->d.comp) {
+            comp.resize(2 * (size_t)N);
+            ok = std::fread(comp.data(), 8, N, f) == (size_t)N;
+        }
     }
     std::fclose(f);
     if (!ok) return fail(h, BDK_ERR_ARG, std::string("not a readable state file: ") + path);
@@ -1428,6 +1451,8 @@ int bdk_load_state(bdk_t *h, const char *path) {
     HIPCHK(h, hipMemcpyAsync(h->d.cell, cell.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d.pid, pid.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d.rel, rel.data(), (size_t)N * 16, hipMemcpyHostToDevice, h->stream));
+    if (h->d.comp)  // slot order = the saved list order, like the state
+        HIPCHK(h, hipMemcpyAsync(h->d.comp, comp.data(), (size_t)N * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d.sc, &step, 8, hipMemcpyHostToDevice, h->stream));  // DevScalars::step
     HIPCHK(h, hipMemsetAsync(h->d.count, 0, (size_t)h->g.ncell * 4, h->stream));
     HIPCHK(h, bdkx::launch_bin(h));

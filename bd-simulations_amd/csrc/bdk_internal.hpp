@@ -40,6 +40,7 @@ struct Geo {
     float ncinv;     // 1/nc (fast exact division of cell indices)
     int fast_wca;    // FP32 forces and nc >= 6: k_wca_f32 (reads lrelf)
     int f32noise;    // FP32 forces: device noise evaluated in FP32
+    int pos_mode;    // in-cell positions: 0 FP64, 1 FP32, 2 FP32 + Kahan-compensated increments
     double hinv;     // 1/h
     unsigned long long seed;
 };
@@ -70,6 +71,8 @@ struct DevBufs {
     int *arr = nullptr;       // [ncell] arrivals of the current step (zero between steps)
     int *start = nullptr;     // [ncell+1]
     int *start_old = nullptr; // [ncell+1] previous starts: the slot arrays' grouping during a rebuild
+    float2 *comp = nullptr;   // pos_mode 2: Kahan compensation by slot (written by k_update)
+    float2 *lcomp = nullptr;  // pos_mode 2: the same in list order (written by k_fill)
     unsigned long long *exo = nullptr; // [64 x 16] max distance of a new position outside its previous cell (double bits)
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
     // the state in list order (k_fill): cell, rel and particle id of list entry q

@@ -1313,8 +1313,28 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
     const double dx = dt * (fwx + fkx) + sq * n.x;
     const double dy = dt * (fwy + fky) + sq * n.y;
     double2 rn;
-    rn.x = r.x + dx;
-    rn.y = r.y + dy;
+    float2 kc = make_float2(0.f, 0.f);  // Kahan compensation (pos_mode 2)
+    if (g.pos_mode == 0) {
+        rn.x = r.x + dx;
+        rn.y = r.y + dy;
+    } else {
+        // FP32 in-cell positions: the increment rounded to FP32 and added in
+        // FP32 (pos_mode 2: Kahan summation, c = (t - r) - y carries the
+        // low-order part lost by the addition into the next step)
+        const float rx = (float)r.x, ry = (float)r.y;
+        if (g.pos_mode == 1) {
+            rn.x = (double)(rx + (float)dx);
+            rn.y = (double)(ry + (float)dy);
+        } else {
+            const float2 c = d.lcomp[s];
+            const float yx = (float)dx - c.x, yy = (float)dy - c.y;
+            const float tx = rx + yx, ty = ry + yy;
+            kc.x = (tx - rx) - yx;
+            kc.y = (ty - ry) - yy;
+            rn.x = (double)tx;
+            rn.y = (double)ty;
+        }
+    }
     if (s2 && g.restrict_radial) {  // dm:138-155
         const double mitte = 0.5 * g.L;
         const int i2 = pid - g.n1;
@@ -1345,10 +1365,20 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
         gx = ic;
         gy = jc;
         nc_new = gx * g.nc + gy;
+        if (g.pos_mode) {
+            // back to FP32 storage; the rounding of the shifted value joins
+            // the compensation (the position is t - c)
+            const float wx = (float)rn.x, wy = (float)rn.y;
+            kc.x += (float)((double)wx - rn.x);
+            kc.y += (float)((double)wy - rn.y);
+            rn.x = (double)wx;
+            rn.y = (double)wy;
+        }
     }
     d.cell[s] = nc_new;
     d.rel[s] = rn;
     d.pid[s] = pid;
+    if (g.pos_mode == 2) d.comp[s] = kc;
     if (nc_new != c) {
         // a cell change (a few percent of the particles per step): the
         // counts follow, and the rank among the new cell's arrivals comes
@@ -1597,6 +1627,7 @@ __device__ __forceinline__ void fill_body(const Geo &g, const DevBufs &d, int mo
     d.lrel[pos] = r;
     d.lpid[pos] = pid;
     if (g.fast_wca) d.lrelf[pos] = make_float2((float)(r.x - 0.5 * g.B), (float)(r.y - 0.5 * g.B));
+    if (g.pos_mode == 2) d.lcomp[pos] = d.comp[s];
 }
 
 __global__ void k_fill(Geo g, DevBufs d, int moved_ranks) { fill_body(g, d, moved_ranks, blockIdx.x); }
@@ -1620,24 +1651,28 @@ __device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, 
     double2 *rl = d.lrel + b;
     int *pl = d.lpid + b;
     float2 *fl = d.lrelf + b;
-    const bool f = g.fast_wca;
+    const bool f = g.fast_wca, kah = g.pos_mode == 2;
+    float2 *cl = kah ? d.lcomp + b : nullptr;
     for (int i = lo + 1; i < n; ++i) {
         const unsigned v = l[i];
         const double2 vr = rl[i];
         const int vp = pl[i];
         const float2 vf = f ? fl[i] : make_float2(0.f, 0.f);
+        const float2 vc = kah ? cl[i] : make_float2(0.f, 0.f);
         int j = i - 1;
         while (j >= lo && (l[j] & LIST_SLOT) > (v & LIST_SLOT)) {
             l[j + 1] = l[j];
             rl[j + 1] = rl[j];
             pl[j + 1] = pl[j];
             if (f) fl[j + 1] = fl[j];
+            if (kah) cl[j + 1] = cl[j];
             --j;
         }
         l[j + 1] = v;
         rl[j + 1] = vr;
         pl[j + 1] = vp;
         if (f) fl[j + 1] = vf;
+        if (kah) cl[j + 1] = vc;
     }
 }
 
@@ -1859,7 +1894,7 @@ hipError_t launch_wca(bdk_t *h, hipStream_t st) {
             hipLaunchKernelGGL(k_wca_f32<true>, grid, blk, 0, st, h->g, h->d);
         else
             hipLaunchKernelGGL(k_wca_f32<false>, grid, blk, 0, st, h->g, h->d);
-    } else if (h->p.precision == BDK_FP32) {
+    } else if (h->p.precision != BDK_FP64) {
         hipLaunchKernelGGL(k_wca<float>, grid, blk, 0, st, h->g, h->d);
     } else {
         hipLaunchKernelGGL(k_wca<double>, grid, blk, 0, st, h->g, h->d);
@@ -1929,7 +1964,7 @@ hipError_t launch_deposit(bdk_t *h, int reset, int slot, hipStream_t st) {
     DepPlan pl = dep_plan(h, reset);
     pl.a.slot = slot;
     if (!st) st = h->stream;
-    const bool f32 = h->p.precision == BDK_FP32;
+    const bool f32 = h->p.precision != BDK_FP64;
     switch (h->g.scheme) {
     case 0: f32 ? launch_deposit_t<float, 0>(h, pl, st) : launch_deposit_t<double, 0>(h, pl, st); break;
     case 1: f32 ? launch_deposit_t<float, 1>(h, pl, st) : launch_deposit_t<double, 1>(h, pl, st); break;
@@ -1977,7 +2012,7 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStrea
 }
 
 hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st, int end_step) {
-    if (h->p.precision == BDK_FP32)
+    if (h->p.precision != BDK_FP64)
         launch_fft_t<float>(h, pass, st, end_step);
     else
         launch_fft_t<double>(h, pass, st, end_step);
@@ -1985,7 +2020,7 @@ hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st, int end_step) {
 }
 
 hipError_t launch_fft(bdk_t *h) {
-    if (h->p.precision == BDK_FP32)
+    if (h->p.precision != BDK_FP64)
         launch_fft_t<float>(h);
     else
         launch_fft_t<double>(h);
@@ -1998,7 +2033,7 @@ template <typename R, int SCHEME> static void launch_gather_t(bdk_t *h, int with
 }
 
 hipError_t launch_gather(bdk_t *h, int with_dt) {
-    const bool f32 = h->p.precision == BDK_FP32;
+    const bool f32 = h->p.precision != BDK_FP64;
     switch (h->g.scheme) {
     case 0: f32 ? launch_gather_t<float, 0>(h, with_dt) : launch_gather_t<double, 0>(h, with_dt); break;
     case 1: f32 ? launch_gather_t<float, 1>(h, with_dt) : launch_gather_t<double, 1>(h, with_dt); break;
@@ -2017,7 +2052,7 @@ template <typename R, int SCHEME> static void launch_update_t(bdk_t *h, int fuse
 }
 
 hipError_t launch_update(bdk_t *h, int fused) {
-    const bool f32 = h->p.precision == BDK_FP32;
+    const bool f32 = h->p.precision != BDK_FP64;
     switch (h->g.scheme) {
     case 0: f32 ? launch_update_t<float, 0>(h, fused) : launch_update_t<double, 0>(h, fused); break;
     case 1: f32 ? launch_update_t<float, 1>(h, fused) : launch_update_t<double, 1>(h, fused); break;
@@ -2139,7 +2174,7 @@ hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *par
     pl.a.q1 = species == 1 ? 1.0 : 0.0;
     pl.a.q2 = species == 2 ? 1.0 : 0.0;
     hipStream_t st = h->stream;
-    const bool f32 = h->p.precision == BDK_FP32;
+    const bool f32 = h->p.precision != BDK_FP64;
     switch (h->g.scheme) {
     case 0: f32 ? launch_deposit_t<float, 0>(h, pl, st) : launch_deposit_t<double, 0>(h, pl, st); break;
     case 1: f32 ? launch_deposit_t<float, 1>(h, pl, st) : launch_deposit_t<double, 1>(h, pl, st); break;
@@ -2475,7 +2510,7 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     if (info) {
         // capillary forces on their own (the step fuses them into the update)
         if (kap && !h->g.adaptive) m.chk(launch_gather(h, 0));
-        const bool f32 = h->p.precision == BDK_FP32;
+        const bool f32 = h->p.precision != BDK_FP64;
         if (f32) {
             hipLaunchKernelGGL(k_info_forces<float>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
             hipLaunchKernelGGL(k_info_fetch<float>, dim3(1), dim3(1), 0, h->stream, h->g, h->d, info);
@@ -2545,7 +2580,7 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchP
     reps.assign(n, RepDev());
     for (int k = 0; k < BS_COUNT; ++k) maps[k].clear();
     bp.nmem = n;
-    bp.f32 = mem.empty() ? 1 : (mem[0]->p.precision == BDK_FP32);
+    bp.f32 = mem.empty() ? 1 : (mem[0]->p.precision != BDK_FP64);
     for (int k = 0; k < 3; ++k) bp.ept[k] = ept[k];
     bp.dep_lds = bp.l1 = bp.lc = bp.l2 = 0;
     const size_t csz = bp.f32 ? 8 : 16;

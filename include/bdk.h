@@ -34,7 +34,13 @@ enum bdk_status {
 };
 
 enum bdk_scheme { BDK_NGP = 0, BDK_CIC = 1, BDK_TSC = 2 };     /* densGrid_Schema, ph:22 */
-enum bdk_precision { BDK_FP64 = 0, BDK_FP32 = 1 };
+/* Precision ladder.  FP64: forces, mesh and FFT in FP64 (the reference's
+   arithmetic).  FP32: forces, mesh and FFT in FP32, positions FP64.  The two
+   further rungs keep FP32 forces and store the in-cell positions (rel) in
+   FP32: POS32 rounds every new position to FP32; POS32K adds each step's
+   increment with Kahan compensation (a per-particle FP32 compensation term
+   that follows the particle). */
+enum bdk_precision { BDK_FP64 = 0, BDK_FP32 = 1, BDK_FP32_POS32 = 2, BDK_FP32_POS32K = 3 };
 enum bdk_noise {
     BDK_NOISE_DISC = 0,    /* uniform on the disc of radius sqrt 5 (dm:1317-1339), Philox */
     BDK_NOISE_GAUSS = 1,   /* Gaussian of variance noise_var, Philox */
@@ -64,7 +70,7 @@ typedef struct bd_params {
     int32_t no_wca;          /* noWCA (ph:70): switches the noise off (dm:90-101) */
     int32_t restrict_radial; /* restrictRadial (ph:76) */
     int32_t ngp_fix;         /* 0: reference NGP species-2 y force (gr:123, ~0); 1: use Fy */
-    int32_t precision;       /* BDK_FP64 or BDK_FP32 (forces, mesh, FFT; positions stay FP64) */
+    int32_t precision;       /* bdk_precision (forces, mesh, FFT; positions FP64 unless a POS32 rung) */
     int32_t noise_mode;      /* bdk_noise */
     int32_t green_mode;      /* bdk_green */
     int32_t device;          /* HIP device ordinal */
