@@ -1,13 +1,14 @@
-"""Precision ladder: FP32 forces against FP64 on a dense system.
+"""Precision ladder: FP32 forces and FP32 positions against FP64 on a dense
+system.
 
 L = 2000, N1 = N2 = 2,000,000 (density 1.0), Z = 4096, for NGP, CIC and
 TSC: per-component relative RMS and maximum error of the WCA and the
 capillary forces in FP32 mode against FP64 mode (whose arithmetic is the
 reference's, pinned against the FP64 CPU oracle by the parity tests) on
-identical seeded positions, and the trajectory drift of FP32 against FP64
-over 500 steps at kT = 0 and kT = 1 (the same Philox noise in both).
-Positions are FP64 in both modes; FP32 positions with compensated
-increments are not built (FP64 positions bound that rung from above).
+identical seeded positions, and the trajectory drift against FP64 over 500
+steps at kT = 0 and kT = 1 (the same Philox noise in all) of three rungs:
+FP32 forces with FP64 positions, FP32 forces with FP32 in-cell positions,
+and FP32 forces with FP32 positions whose increments are Kahan-compensated.
 
 python bd-simulations_amd/precision_ladder.py [--steps 500] [--n 2000000] [--json out.json]
 """
@@ -21,6 +22,7 @@ import numpy as np
 import bdk
 
 SCHEMES = {"NGP": bdk.NGP, "CIC": bdk.CIC, "TSC": bdk.TSC}
+RUNGS = {"fp64": bdk.FP64, "fp32": bdk.FP32, "fp32_pos32": bdk.FP32_POS32, "fp32_pos32k": bdk.FP32_POS32K}
 
 
 def rel_err(a, b):
@@ -61,14 +63,17 @@ def run(n=2_000_000, L=2000.0, Z=4096, steps=500, seed=7, log=print):
             s.close()
         for T in (0.0, 1.0):
             tr = {}
-            for prec in (bdk.FP64, bdk.FP32):
+            for rung, prec in RUNGS.items():
                 s = bdk.RunDynamik(bdk.make_params(precision=prec, scheme=sch, T=T, **base))
                 s.set_positions(git, rel)
+                s.run_steps(2)  # forces, graphs
+                s.sync()
                 t0 = time.perf_counter()
                 s.run_steps(steps)
-                tr[prec] = (s, time.perf_counter() - t0)
-            r["drift_T%g" % T] = drift(tr[bdk.FP32][0], tr[bdk.FP64][0], L)
-            r["ms_per_step_T%g" % T] = {"fp64": 1e3 * tr[bdk.FP64][1] / steps, "fp32": 1e3 * tr[bdk.FP32][1] / steps}
+                s.sync()
+                tr[rung] = (s, time.perf_counter() - t0)
+            r["drift_T%g" % T] = {rung: drift(tr[rung][0], tr["fp64"][0], L) for rung in RUNGS if rung != "fp64"}
+            r["ms_per_step_T%g" % T] = {rung: 1e3 * t / steps for rung, (_, t) in tr.items()}
             for s, _ in tr.values():
                 s.close()
         log("ladder %s: %s" % (name, json.dumps(r)))
