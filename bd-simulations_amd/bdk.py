@@ -99,6 +99,8 @@ def lib():
             "bdk_init_uniform": (C.c_int, [vp, u64, dbl, vp, vp]),
             "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_write_pos_async": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_write_wait": (C.c_int, [vp]),
             "bdk_step_info": (C.c_int, [vp, dbl, vp]),
             "bdk_read_startpos": (C.c_int, [vp, C.c_char_p, vp, vp]),
             "bdk_save_state": (C.c_int, [vp, C.c_char_p]),
@@ -137,7 +139,7 @@ def exported_symbols():
             "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_profile_kernel",
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
-            "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos",
+            "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos", "bdk_write_pos_async", "bdk_write_wait",
             "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
             "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
@@ -437,6 +439,15 @@ class RunDynamik:
 
     def write_pos(self, path1, path2, nr, obs_dt):
         self._chk(self.L.bdk_write_pos(self.h, path1.encode(), path2.encode() if path2 else None, nr, obs_dt))
+
+    def write_pos_async(self, path1, path2, nr, obs_dt):
+        """The checkpoint of write_pos, formatted and written by a host thread
+        while later steps run; write_wait() returns its status."""
+        self._chk(self.L.bdk_write_pos_async(self.h, path1.encode(), path2.encode() if path2 else None, nr,
+                                             obs_dt))
+
+    def write_wait(self):
+        self._chk(self.L.bdk_write_wait(self.h))
 
 
 class Batch:

@@ -405,6 +405,16 @@ struct bdk_batch {
         *g = gg;
         return hipGraphInstantiate(x, gg, nullptr, nullptr, 0);
     }
+    // a member's asynchronous pos write copies on the member's stream: the
+    // batch's steps wait for those copies
+    hipError_t wait_writers() {
+        for (bdk_t *h : mem)
+            if (h->writer.pending) {
+                hipError_t e = hipStreamWaitEvent(stream, h->writer.ev, 0);
+                if (e != hipSuccess) return e;
+            }
+        return hipSuccess;
+    }
     // members uploaded since their last step get their forces first (all
     // members: the others' forces come out the same)
     hipError_t ensure_forces() {
@@ -642,11 +652,14 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     return BDK_OK;
 }
 
+static void free_writer(bdk_t *h);
+
 void bdk_destroy(bdk_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->aux) (void)hipStreamSynchronize(h->aux);
+    free_writer(h);
     drop_graph(h);
     free_dev(h);
     release_streams(h);
@@ -1284,35 +1297,144 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
     return BDK_OK;
 }
 
-int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt) {
-    if (!h || !path1) return fail(h, BDK_ERR_ARG, "null argument");
-    const int n1 = h->p.n1, n2 = h->p.n2;
-    std::vector<int32_t> g1(2 * (size_t)n1), g2(2 * (size_t)n2);
-    std::vector<double> r1(2 * (size_t)n1), r2(2 * (size_t)n2);
-    int rc = bdk_get_positions(h, 1, n1, g1.data(), r1.data());
-    if (rc) return rc;
-    if (n2) {
-        rc = bdk_get_positions(h, 2, n2, g2.data(), r2.data());
-        if (rc) return rc;
-    }
-    const double B = h->g.B;
+// One checkpoint of pos1/pos2 (au:57-69, au:631-648): on nr == 0 the file is
+// truncated and gets the header, then one line "t x y" per particle of the
+// species in id order and a blank line.  g/r: [n][2] of each species.
+// Returns an empty string or the error.
+static std::string append_pos_files(const char *path1, const char *path2, int n1, int n2, double B, int32_t nr,
+                                    double obs_dt, const int32_t *g1, const double *r1, const int32_t *g2,
+                                    const double *r2) {
     for (int sp = 0; sp < (n2 ? 2 : 1); ++sp) {
         const char *path = sp == 0 ? path1 : path2;
         if (!path) continue;
         FILE *f = std::fopen(path, nr == 0 ? "w" : "a");
-        if (!f) return fail(h, BDK_ERR_ARG, std::string("cannot open ") + path);
+        if (!f) return std::string("cannot open ") + path;
         if (nr == 0) std::fprintf(f, "# Format: t TAB x TAB y \n\n");  // au:60
         const int n = sp == 0 ? n1 : n2;
-        const std::vector<int32_t> &g = sp == 0 ? g1 : g2;
-        const std::vector<double> &r = sp == 0 ? r1 : r2;
+        const int32_t *g = sp == 0 ? g1 : g2;
+        const double *r = sp == 0 ? r1 : r2;
         for (int i = 0; i < n; ++i) {  // au:633-637
             double x = r[2 * (size_t)i] + B * g[2 * (size_t)i];
             double y = r[2 * (size_t)i + 1] + B * g[2 * (size_t)i + 1];
             std::fprintf(f, "%g \t %g \t %g \n", nr * obs_dt, x, y);
         }
         std::fprintf(f, "\n");
-        std::fclose(f);
+        if (std::fclose(f) != 0) return std::string("cannot write ") + path;
     }
+    return std::string();
+}
+
+int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt) {
+    if (!h || !path1) return fail(h, BDK_ERR_ARG, "null argument");
+    int rc = bdk_write_wait(h);  // an earlier asynchronous write of the same files comes first
+    if (rc) return rc;
+    const int n1 = h->p.n1, n2 = h->p.n2;
+    std::vector<int32_t> g1(2 * (size_t)n1), g2(2 * (size_t)n2);
+    std::vector<double> r1(2 * (size_t)n1), r2(2 * (size_t)n2);
+    rc = bdk_get_positions(h, 1, n1, g1.data(), r1.data());
+    if (rc) return rc;
+    if (n2) {
+        rc = bdk_get_positions(h, 2, n2, g2.data(), r2.data());
+        if (rc) return rc;
+    }
+    std::string e = append_pos_files(path1, path2, n1, n2, h->g.B, nr, obs_dt, g1.data(), r1.data(), g2.data(),
+                                     r2.data());
+    return e.empty() ? BDK_OK : fail(h, BDK_ERR_ARG, e);
+}
+
+static void free_writer(bdk_t *h) {
+    bdkx::PosWriter &w = h->writer;
+    if (w.th.joinable()) w.th.join();
+    w.pending = false;
+    if (w.cell) (void)hipHostFree(w.cell);
+    if (w.pid) (void)hipHostFree(w.pid);
+    if (w.rel) (void)hipHostFree(w.rel);
+    if (w.sc) (void)hipHostFree(w.sc);
+    if (w.ev) (void)hipEventDestroy(w.ev);
+    w.cell = w.pid = nullptr;
+    w.rel = nullptr;
+    w.sc = nullptr;
+    w.ev = nullptr;
+    w.cap = 0;
+}
+
+int bdk_write_pos_async(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt) {
+    if (!h || !path1) return fail(h, BDK_ERR_ARG, "null argument");
+    int rc = bdk_write_wait(h);
+    if (rc) return rc;
+    rc = ensure(h);
+    if (rc) return rc;
+    bdkx::PosWriter &w = h->writer;
+    const int N = h->g.N;
+    if (w.cap < N || !w.ev) {
+        free_writer(h);
+        HIPCHK(h, hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+        HIPCHK(h, hipHostMalloc((void **)&w.cell, (size_t)std::max(N, 1) * 4, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void **)&w.pid, (size_t)std::max(N, 1) * 4, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void **)&w.rel, (size_t)std::max(N, 1) * 16, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void **)&w.sc, sizeof(DevScalars), hipHostMallocDefault));
+        w.cap = N;
+    }
+    // ordered on the handle's stream: later steps cannot overwrite the state
+    // before it is copied
+    HIPCHK(h, hipMemcpyAsync(w.cell, h->d.cell, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(w.pid, h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(w.rel, h->d.rel, (size_t)N * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(w.sc, h->d.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipEventRecord(w.ev, h->stream));
+    w.pending = true;
+    w.status = BDK_OK;
+    w.err.clear();
+    const std::string p1 = path1, p2 = path2 ? path2 : "";
+    const bool has2 = path2 != nullptr;
+    const int n1 = h->p.n1, n2 = h->p.n2, nc = h->g.nc;
+    const double B = h->g.B;
+    bdkx::PosWriter *wp = &w;
+    w.th = std::thread([=]() {
+        hipError_t e = hipEventSynchronize(wp->ev);
+        if (e != hipSuccess) {
+            wp->status = BDK_ERR_HIP;
+            wp->err = std::string("bdk_write_pos_async: ") + hipGetErrorString(e);
+            return;
+        }
+        if (wp->sc->nonfinite & 2) {
+            wp->status = BDK_ERR_HIP;
+            wp->err = "internal error: inconsistent cell lists (cell counts do not sum to N)";
+            return;
+        }
+        if (wp->sc->nonfinite) {
+            wp->status = BDK_ERR_NONFINITE;
+            wp->err = "a particle position became non-finite or ran away (|rel| >= 1e9); it was held in place";
+            return;
+        }
+        std::vector<int32_t> g(2 * (size_t)N);
+        std::vector<double> r(2 * (size_t)N);
+        for (int s = 0; s < N; ++s) {  // slot -> particle id order, species 1 first
+            const int q = wp->pid[s], c = wp->cell[s];
+            g[2 * (size_t)q] = c / nc;
+            g[2 * (size_t)q + 1] = c % nc;
+            r[2 * (size_t)q] = wp->rel[2 * (size_t)s];
+            r[2 * (size_t)q + 1] = wp->rel[2 * (size_t)s + 1];
+        }
+        std::string msg = append_pos_files(p1.c_str(), has2 ? p2.c_str() : nullptr, n1, n2, B, nr, obs_dt,
+                                           g.data(), r.data(), g.data() + 2 * (size_t)n1,
+                                           r.data() + 2 * (size_t)n1);
+        if (!msg.empty()) {
+            wp->status = BDK_ERR_ARG;
+            wp->err = msg;
+        }
+    });
+    return BDK_OK;
+}
+
+int bdk_write_wait(bdk_t *h) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    bdkx::PosWriter &w = h->writer;
+    if (w.th.joinable()) w.th.join();
+    w.pending = false;
+    const int st = w.status;
+    w.status = BDK_OK;
+    if (st) return fail(h, st, w.err);
     return BDK_OK;
 }
 
@@ -1628,6 +1750,7 @@ int bdk_batch_run_steps(bdk_batch_t *b, int64_t k) {
     hipError_t e = hipSetDevice(b->dev);
     if (e == hipSuccess) e = b->ensure_forces();
     if (e == hipSuccess) e = b->ensure_graphs();
+    if (e == hipSuccess) e = b->wait_writers();
     int64_t i = 0;
     for (; e == hipSuccess && i + kGraphSteps <= k; i += kGraphSteps) e = hipGraphLaunch(b->gexec_multi, b->stream);
     for (; e == hipSuccess && i < k; ++i) e = hipGraphLaunch(b->gexec, b->stream);
@@ -1661,7 +1784,10 @@ int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_pe
     if (e == hipSuccess) e = b->ensure_forces();
     if (e == hipSuccess) e = b->ensure_graphs();
     auto launch = [&]() -> hipError_t {
-        if (stage == 7) return hipGraphLaunch(b->gexec, b->stream);
+        if (stage == 7) {
+            hipError_t w = b->wait_writers();
+            return w != hipSuccess ? w : hipGraphLaunch(b->gexec, b->stream);
+        }
         return bdkx::batch_stage(b->bp, stage, b->stream);
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;

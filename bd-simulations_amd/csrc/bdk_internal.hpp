@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bdk.h"
@@ -54,6 +55,22 @@ struct DevScalars {
     int nonfinite;                // a non-finite position appeared
     int fallback;                 // tiles that ran the unstaged WCA/deposit path
     int pad[4];
+};
+
+// Asynchronous pos writer (bdk_write_pos_async): pinned host copies of the
+// slot arrays and scalars, the event recorded after the copies, and the host
+// thread that formats and appends the files.  At most one write is in flight.
+struct PosWriter {
+    int *cell = nullptr;
+    int *pid = nullptr;
+    double *rel = nullptr;        // [cap][2]
+    DevScalars *sc = nullptr;
+    int cap = 0;
+    hipEvent_t ev = nullptr;
+    std::thread th;
+    bool pending = false;         // copies enqueued, thread not yet joined
+    int status = 0;
+    std::string err;
 };
 
 struct DevBufs {
@@ -160,6 +177,7 @@ struct bdk {
     std::string err;
     int64_t host_step = 0;
     bdkx::StepInfoDev *info_dev = nullptr;  // step counters (bdk_step_info), allocated on first use
+    bdkx::PosWriter writer;
 };
 
 namespace bdkx {

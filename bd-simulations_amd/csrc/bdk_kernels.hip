@@ -2180,7 +2180,8 @@ hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *par
     case 1: f32 ? launch_deposit_t<float, 1>(h, pl, st) : launch_deposit_t<double, 1>(h, pl, st); break;
     default: f32 ? launch_deposit_t<float, 2>(h, pl, st) : launch_deposit_t<double, 2>(h, pl, st); break;
     }
-    launch_fft_pass(h, 0);
+    hipError_t e = launch_fft_pass(h, 0);
+    if (e != hipSuccess) return e;
     if (f32)
         launch_rhok_t<float>(h, dqr, nbins, part, out);
     else

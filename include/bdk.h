@@ -204,6 +204,17 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
 /* pos1_<run+1>.txt / pos2_<run+1>.txt writer (au:57-69, au:631-648):
    header on nr == 0 with truncate, then one block per checkpoint. */
 int bdk_write_pos(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
+/* The same checkpoint written in the background (schreibe_pos, au:631-648,
+   which blocks the reference's run loop): the current state is copied to
+   pinned host memory on the handle's stream, ordered before any later step,
+   and a host thread formats and appends the files while the device keeps
+   stepping.  The files' bytes equal bdk_write_pos's.  At most one write is in
+   flight per handle: a new call (or bdk_write_pos) first waits for the
+   previous one and returns its error, if any. */
+int bdk_write_pos_async(bdk_t *h, const char *path1, const char *path2, int32_t nr, double obs_dt);
+/* Wait for the handle's asynchronous write; returns its status (BDK_OK when
+   none is in flight).  bdk_destroy also waits. */
+int bdk_write_wait(bdk_t *h);
 
 /* One step like bdk_step that also fills the counters of TimestepInfo
    (signaturen2.h:71-145, zeitschritt_debug dm:168-211): the largest WCA,
