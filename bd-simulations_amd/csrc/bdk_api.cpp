@@ -17,6 +17,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bdk_internal.hpp"
@@ -129,7 +130,7 @@ void release_streams(bdk_t *h) {
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.comp, h->d.lcomp, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.dts, h->d.comp, h->d.lcomp, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -584,6 +585,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
         ALLOC(dalloc(d.lcomp, N + 1));
     }
     ALLOC(dalloc(d.exo, 64 * 16));
+    ALLOC(dalloc(d.dts, 64 * 16));
     // list-ordered arrays carry one padding entry (read ahead by k_wca_f32)
     ALLOC(dalloc(d.list, N + 1));
     ALLOC(dalloc(d.lcell, N + 1));
@@ -618,6 +620,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     e = hipMemcpy(d.sc, &sc, sizeof(sc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(d.count, 0, ncell * 4);
     if (e == hipSuccess) e = hipMemset(d.exo, 0, 64 * 16 * 8);
+    if (e == hipSuccess) e = hipMemset(d.dts, 0xff, 64 * 16 * 8);
     if (e == hipSuccess && d.comp) e = hipMemset(d.comp, 0, (N + 1) * 8);
     if (e == hipSuccess && d.lcomp) e = hipMemset(d.lcomp, 0, (N + 1) * 8);
     if (e == hipSuccess) e = hipMemset(d.arr, 0, ncell * 4);
@@ -1297,29 +1300,53 @@ int bdk_init_blob(const bd_params *p, uint64_t seed, double width, double backgr
     return BDK_OK;
 }
 
+// The lines "t x y" of particles [lo, hi) of one species (au:633-637) into out.
+static void format_pos_lines(std::string &out, int lo, int hi, double B, double t, const int32_t *g,
+                             const double *r) {
+    out.clear();
+    out.reserve((size_t)(hi - lo) * 40);
+    char line[128];
+    for (int i = lo; i < hi; ++i) {
+        const double x = r[2 * (size_t)i] + B * g[2 * (size_t)i];
+        const double y = r[2 * (size_t)i + 1] + B * g[2 * (size_t)i + 1];
+        const int k = std::snprintf(line, sizeof line, "%g \t %g \t %g \n", t, x, y);
+        if (k > 0) out.append(line, (size_t)std::min(k, (int)sizeof line - 1));
+    }
+}
+
 // One checkpoint of pos1/pos2 (au:57-69, au:631-648): on nr == 0 the file is
 // truncated and gets the header, then one line "t x y" per particle of the
-// species in id order and a blank line.  g/r: [n][2] of each species.
-// Returns an empty string or the error.
+// species in id order and a blank line.  g/r: [n][2] of each species.  The
+// lines are formatted by up to 16 host threads (BDK_WRITER_THREADS sets the
+// limit, 1-64) in contiguous chunks and written in order, so the bytes equal
+// one thread's.  Returns an empty string or the error.
 static std::string append_pos_files(const char *path1, const char *path2, int n1, int n2, double B, int32_t nr,
                                     double obs_dt, const int32_t *g1, const double *r1, const int32_t *g2,
                                     const double *r2) {
+    const double t = nr * obs_dt;
+    int hw = (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (const char *e = std::getenv("BDK_WRITER_THREADS")) hw = std::max(1, std::min(64, std::atoi(e)));
     for (int sp = 0; sp < (n2 ? 2 : 1); ++sp) {
         const char *path = sp == 0 ? path1 : path2;
         if (!path) continue;
-        FILE *f = std::fopen(path, nr == 0 ? "w" : "a");
-        if (!f) return std::string("cannot open ") + path;
-        if (nr == 0) std::fprintf(f, "# Format: t TAB x TAB y \n\n");  // au:60
         const int n = sp == 0 ? n1 : n2;
         const int32_t *g = sp == 0 ? g1 : g2;
         const double *r = sp == 0 ? r1 : r2;
-        for (int i = 0; i < n; ++i) {  // au:633-637
-            double x = r[2 * (size_t)i] + B * g[2 * (size_t)i];
-            double y = r[2 * (size_t)i + 1] + B * g[2 * (size_t)i + 1];
-            std::fprintf(f, "%g \t %g \t %g \n", nr * obs_dt, x, y);
-        }
-        std::fprintf(f, "\n");
-        if (std::fclose(f) != 0) return std::string("cannot write ") + path;
+        const int nt = std::min(hw, 1 + n / 32768);
+        std::vector<std::string> part((size_t)nt);
+        std::vector<std::thread> th;
+        for (int k = 1; k < nt; ++k)
+            th.emplace_back(format_pos_lines, std::ref(part[k]), (int)((long)n * k / nt),
+                            (int)((long)n * (k + 1) / nt), B, t, g, r);
+        format_pos_lines(part[0], 0, (int)((long)n / nt), B, t, g, r);
+        for (std::thread &x : th) x.join();
+        FILE *f = std::fopen(path, nr == 0 ? "w" : "a");
+        if (!f) return std::string("cannot open ") + path;
+        bool ok = true;
+        if (nr == 0) ok = std::fputs("# Format: t TAB x TAB y \n\n", f) >= 0;  // au:60
+        for (const std::string &q : part) ok = ok && std::fwrite(q.data(), 1, q.size(), f) == q.size();
+        ok = std::fputs("\n", f) >= 0 && ok;
+        if (std::fclose(f) != 0 || !ok) return std::string("cannot write ") + path;
     }
     return std::string();
 }
@@ -1351,6 +1378,10 @@ static void free_writer(bdk_t *h) {
     if (w.rel) (void)hipHostFree(w.rel);
     if (w.sc) (void)hipHostFree(w.sc);
     if (w.ev) (void)hipEventDestroy(w.ev);
+    if (w.ready) (void)hipEventDestroy(w.ready);
+    if (w.st) (void)hipStreamDestroy(w.st);
+    w.st = nullptr;
+    w.ready = nullptr;
     w.cell = w.pid = nullptr;
     w.rel = nullptr;
     w.sc = nullptr;
@@ -1368,6 +1399,8 @@ int bdk_write_pos_async(bdk_t *h, const char *path1, const char *path2, int32_t 
     const int N = h->g.N;
     if (w.cap < N || !w.ev) {
         free_writer(h);
+        HIPCHK(h, hipStreamCreateWithFlags(&w.st, hipStreamNonBlocking));
+        HIPCHK(h, hipEventCreateWithFlags(&w.ready, hipEventDisableTiming));
         HIPCHK(h, hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
         HIPCHK(h, hipHostMalloc((void **)&w.cell, (size_t)std::max(N, 1) * 4, hipHostMallocDefault));
         HIPCHK(h, hipHostMalloc((void **)&w.pid, (size_t)std::max(N, 1) * 4, hipHostMallocDefault));
@@ -1375,13 +1408,16 @@ int bdk_write_pos_async(bdk_t *h, const char *path1, const char *path2, int32_t 
         HIPCHK(h, hipHostMalloc((void **)&w.sc, sizeof(DevScalars), hipHostMallocDefault));
         w.cap = N;
     }
-    // ordered on the handle's stream: later steps cannot overwrite the state
-    // before it is copied
-    HIPCHK(h, hipMemcpyAsync(w.cell, h->d.cell, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(w.pid, h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(w.rel, h->d.rel, (size_t)N * 16, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(w.sc, h->d.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipEventRecord(w.ev, h->stream));
+    // the copies start when the state is there, and later work on the
+    // handle's stream waits for them, so no step overwrites the state first
+    HIPCHK(h, hipEventRecord(w.ready, h->stream));
+    HIPCHK(h, hipStreamWaitEvent(w.st, w.ready, 0));
+    HIPCHK(h, hipMemcpyAsync(w.cell, h->d.cell, (size_t)N * 4, hipMemcpyDeviceToHost, w.st));
+    HIPCHK(h, hipMemcpyAsync(w.pid, h->d.pid, (size_t)N * 4, hipMemcpyDeviceToHost, w.st));
+    HIPCHK(h, hipMemcpyAsync(w.rel, h->d.rel, (size_t)N * 16, hipMemcpyDeviceToHost, w.st));
+    HIPCHK(h, hipMemcpyAsync(w.sc, h->d.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, w.st));
+    HIPCHK(h, hipEventRecord(w.ev, w.st));
+    HIPCHK(h, hipStreamWaitEvent(h->stream, w.ev, 0));
     w.pending = true;
     w.status = BDK_OK;
     w.err.clear();

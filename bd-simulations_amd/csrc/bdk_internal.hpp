@@ -66,7 +66,10 @@ struct PosWriter {
     double *rel = nullptr;        // [cap][2]
     DevScalars *sc = nullptr;
     int cap = 0;
-    hipEvent_t ev = nullptr;
+    hipStream_t st = nullptr;     // copies run here (the thread never waits on the handle's stream,
+                                  // which may be under graph capture meanwhile)
+    hipEvent_t ready = nullptr;   // on the handle's stream: the state to write
+    hipEvent_t ev = nullptr;      // on st: copies done
     std::thread th;
     bool pending = false;         // copies enqueued, thread not yet joined
     int status = 0;
@@ -91,6 +94,8 @@ struct DevBufs {
     float2 *comp = nullptr;   // pos_mode 2: Kahan compensation by slot (written by k_update)
     float2 *lcomp = nullptr;  // pos_mode 2: the same in list order (written by k_fill)
     unsigned long long *exo = nullptr; // [64 x 16] max distance of a new position outside its previous cell (double bits)
+    unsigned long long *dts = nullptr; // [64 x 16] adaptive dt: minima of the candidates by block (double bits;
+                                       // all ones = none), folded into sc->dt_bits by k_dt_fold
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
     // the state in list order (k_fill): cell, rel and particle id of list entry q
     int *lcell = nullptr;

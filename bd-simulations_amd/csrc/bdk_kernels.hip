@@ -1271,13 +1271,44 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
         }
     }
     if (with_dt) {
-        // NaN candidates never win (std::min semantics of dm:1420-1491)
+        // NaN candidates never win (std::min semantics of dm:1420-1491).
+        // Minimum over the block, then one atomicMin into one of 64 slots
+        // (128 bytes apart), skipped when the slot already holds less.
+        __shared__ double wmin[4];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
             double v = __shfl_xor(cand, o, 64);
             if (v < cand) cand = v;
         }
-        if ((threadIdx.x & 63) == 0 && cand < INFINITY && cand >= 0.0) atomicMin(&d.sc->dt_bits, bitsd(cand));
+        if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = cand;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+                if (wmin[w] < cand) cand = wmin[w];
+            if (cand < INFINITY && cand >= 0.0) {
+                unsigned long long *slot = &d.dts[(blockIdx.x & 63) * 16];
+                const unsigned long long b = bitsd(cand);
+                if (b < *(volatile unsigned long long *)slot) atomicMin(slot, b);
+            }
+        }
+    }
+}
+
+// dt of the step: dt_max and the 64 slot minima of k_gather folded into
+// sc->dt_bits; the slots are re-armed (all ones) for the next reduction.
+// One wave.
+__global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) {
+    const int t = threadIdx.x;
+    unsigned long long m = d.dts[t * 16];
+    d.dts[t * 16] = ~0ull;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long v = __shfl_xor(m, o, 64);
+        if (v < m) m = v;
+    }
+    if (t == 0) {
+        const unsigned long long dm = bitsd(g.dt_max);
+        d.sc->dt_bits = m < dm ? m : dm;
     }
 }
 
@@ -2030,6 +2061,7 @@ hipError_t launch_fft(bdk_t *h) {
 template <typename R, int SCHEME> static void launch_gather_t(bdk_t *h, int with_dt) {
     hipLaunchKernelGGL((k_gather<R, SCHEME>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d,
                        with_dt);
+    if (with_dt) hipLaunchKernelGGL(k_dt_fold, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
 }
 
 hipError_t launch_gather(bdk_t *h, int with_dt) {

@@ -98,3 +98,36 @@ def test_async_write_error_reported_by_wait():
         s.write_wait()
         assert read(good).startswith(b"# Format")
     s.close()
+
+
+def test_many_chunk_writer_equals_oracle_writer(tmp_path):
+    """Enough particles that both writers format the lines on several host
+    threads: the files still equal the oracle's single-threaded writer."""
+    print("case: multi-thread formatting, 150000 + 70000 particles", flush=True)
+    import oracle as O
+    sp = dict(n1=150000, n2=70000, L=700.0, Z=64, scheme=bdk.CIC, lam=20.0, T=1.0, dt_max=1e-4, seed=4)
+    bp, op = params(**sp)
+    B = bdk.derived(bp)["B"]
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.0, sp["L"], (sp["n1"] + sp["n2"], 2))
+    git = np.minimum(np.floor(x / B), bdk.derived(bp)["nc"] - 1).astype(np.int32)
+    rel = x - git * B
+    s = bdk.RunDynamik(bp)
+    s.set_positions(git, rel)
+    g, r = s.positions()
+    o = O.Oracle(op)
+    o.set_all(g, r)
+    p = {k: str(tmp_path / k) for k in ("a1", "a2", "b1", "b2", "o1", "o2")}
+    for nr in range(2):
+        s.write_pos(p["a1"], p["a2"], nr, 0.25)
+        s.write_pos_async(p["b1"], p["b2"], nr, 0.25)
+        s.write_wait()
+        if nr == 0:
+            o.write_pos_header(p["o1"])
+            o.write_pos_header(p["o2"])
+        o.append_pos(1, p["o1"], nr, 0.25)
+        o.append_pos(2, p["o2"], nr, 0.25)
+    for a, b, c in (("a1", "b1", "o1"), ("a2", "b2", "o2")):
+        assert read(p[a]) == read(p[c]) and read(p[b]) == read(p[c])
+    o.close()
+    s.close()
