@@ -208,6 +208,25 @@ def ensemble_figures(log_fn, cpu=False):
     return batched, served
 
 
+def adaptive_figure(bdk, w, git, rel, steps, warmup, N):
+    """The same workload with the reference's default travel cap
+    (max_reisedistanz = 0.24, parameter.h:43): dt per step from the device
+    minimum of optimaler_zeitschritt's candidates (dm:1420-1491), dt_max 1e-4."""
+    log("bench: adaptive dt (max_travel 0.24), %d steps" % steps)
+    p = bdk.make_params(n1=w["n1"], n2=w["n2"], L=w["L"], Z=w["Z"], scheme=w["scheme"], lam=w["lam"],
+                        kap=w["kap"], f2_f1=w["f2_f1"], T=w["T"], dt_max=w["dt"], precision=bdk.FP32,
+                        noise_mode=bdk.NOISE_DISC, seed=w["seed"], max_travel=0.24)
+    sim = bdk.RunDynamik(p)
+    sim.set_positions(git, rel)
+    sim.run_steps(warmup)
+    t0 = time.perf_counter()
+    t_sim = sim.run_steps(steps)
+    elapsed = time.perf_counter() - t0
+    sim.close()
+    return {"max_travel": 0.24, "steps": steps, "ms_per_step": elapsed / steps * 1e3,
+            "particle_steps_per_s": N * steps / elapsed, "mean_dt": t_sim / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +301,7 @@ def main():
         "kernels_ms": kernels,
     }
     sim.close()
+    out["adaptive_dt"] = adaptive_figure(bdk, w, git, rel, args.steps, args.warmup, N)
     if not args.no_ensemble:
         out["batched"], out["served"] = ensemble_figures(log, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
