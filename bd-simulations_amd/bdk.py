@@ -100,6 +100,8 @@ def lib():
             "bdk_init_blob": (C.c_int, [vp, u64, dbl, dbl, dbl, vp, vp, vp]),
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
             "bdk_write_pos_async": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
+            "bdk_density_grid": (C.c_int, [vp, i32, vp, vp]),
+            "bdk_density_stats": (C.c_int, [vp, i32, i32, dbl, vp, vp, vp, vp]),
             "bdk_write_wait": (C.c_int, [vp]),
             "bdk_step_info": (C.c_int, [vp, dbl, vp]),
             "bdk_read_startpos": (C.c_int, [vp, C.c_char_p, vp, vp]),
@@ -141,7 +143,7 @@ def exported_symbols():
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos", "bdk_write_pos_async", "bdk_write_wait",
             "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
-            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
+            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_density_grid", "bdk_density_stats", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
@@ -420,6 +422,21 @@ class RunDynamik:
         prof = np.zeros((6, nbins))
         self._chk(self.L.bdk_radial_profile(self.h, nbins, dr, _p(cen), _p(prof)))
         return cen, prof
+
+    def density_grid(self, m):
+        """rho1, rho2 [m, m] of the snapshot grid (main_auswertung.cpp:214-320)."""
+        r1, r2 = np.zeros((m, m)), np.zeros((m, m))
+        self._chk(self.L.bdk_density_grid(self.h, m, _p(r1), _p(r2)))
+        return r1, r2
+
+    def density_stats(self, m, nbins=100, dbin=0.01, rho_thresh=(0.1, 0.2, 0.5), drho_thresh=(0.1, 0.2, 0.5, 0.99)):
+        """hist [4, nbins] (rho1, rho2, rho1+rho2, rho1-rho2) and over [7]
+        (main_auswertung.cpp:324-385), as fractions of the m*m cells."""
+        hist, over = np.zeros((4, nbins)), np.zeros(7)
+        rt = np.asarray(rho_thresh, dtype=np.float64)
+        dt = np.asarray(drho_thresh, dtype=np.float64)
+        self._chk(self.L.bdk_density_stats(self.h, m, nbins, dbin, _p(rt), _p(dt), _p(hist), _p(over)))
+        return hist, over
 
     def pair_correlation(self, nbins, dr):
         """g11, g12, g22 (auswertung.cpp:110-188)."""

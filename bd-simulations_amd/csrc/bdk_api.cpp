@@ -1683,6 +1683,64 @@ int bdk_pair_correlation(bdk_t *h, int32_t nbins, double dr, double *g11, double
     return BDK_OK;
 }
 
+// Snapshot density grid and its statistics (main_auswertung.cpp:214-385).
+static int density_snapshot(bdk_t *h, int32_t m, double *rho1, double *rho2, int32_t nbins, double dbin,
+                            const double *rho_thresh, const double *drho_thresh, double *hist, double *over,
+                            const char *what) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if (m < 1 || m > 16384) return fail(h, BDK_ERR_ARG, std::string(what) + ": need 1 <= m <= 16384");
+    const int stats = hist || over ? 1 : 0;
+    if (stats && (nbins < 1 || nbins > 4096 || !(dbin > 0.0) || !rho_thresh || !drho_thresh))
+        return fail(h, BDK_ERR_ARG, std::string(what) + ": need 1 <= nbins <= 4096, dbin > 0, thresholds");
+    const size_t mm = (size_t)m * m;
+    bdkx::SnapThresh th{};
+    if (stats) {
+        for (int t = 0; t < 3; ++t) th.rho[t] = rho_thresh[t];
+        for (int t = 0; t < 4; ++t) th.drho[t] = drho_thresh[t];
+    }
+    unsigned long long *acc = nullptr;
+    double *rho = nullptr;
+    unsigned int *cnt = nullptr;
+    const size_t nc = stats ? (size_t)4 * nbins + 7 : 1;
+    hipError_t e = hipMalloc((void **)&acc, 2 * mm * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void **)&rho, 2 * mm * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&cnt, nc * sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMemsetAsync(acc, 0, 2 * mm * sizeof(unsigned long long), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, nc * sizeof(unsigned int), h->stream);
+    if (e == hipSuccess) e = bdkx::launch_snap(h, m, acc, rho, stats, nbins, dbin, th, cnt);
+    std::vector<unsigned int> c(nc);
+    if (e == hipSuccess && rho1) e = hipMemcpyAsync(rho1, rho, mm * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && rho2) e = hipMemcpyAsync(rho2, rho + mm, mm * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && stats)
+        e = hipMemcpyAsync(c.data(), cnt, nc * sizeof(unsigned int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (acc) (void)hipFree(acc);
+    if (rho) (void)hipFree(rho);
+    if (cnt) (void)hipFree(cnt);
+    if (e != hipSuccess) return hipfail(h, e, what);
+    if (stats) {
+        const double pre = 1.0 / (double)mm;  // main_auswertung.cpp:328
+        if (hist)
+            for (size_t i = 0; i < (size_t)4 * nbins; ++i) hist[i] = pre * c[i];
+        if (over)
+            for (int t = 0; t < 7; ++t) over[t] = pre * c[(size_t)4 * nbins + t];
+    }
+    return BDK_OK;
+}
+
+int bdk_density_grid(bdk_t *h, int32_t m, double *rho1, double *rho2) {
+    if (!h || (!rho1 && !rho2)) return fail(h, BDK_ERR_ARG, "null argument");
+    return density_snapshot(h, m, rho1, rho2, 0, 0.0, nullptr, nullptr, nullptr, nullptr, "bdk_density_grid");
+}
+
+int bdk_density_stats(bdk_t *h, int32_t m, int32_t nbins, double dbin, const double *rho_thresh,
+                      const double *drho_thresh, double *hist, double *over) {
+    if (!h || !hist || !over) return fail(h, BDK_ERR_ARG, "null argument");
+    return density_snapshot(h, m, nullptr, nullptr, nbins, dbin, rho_thresh, drho_thresh, hist, over,
+                            "bdk_density_stats");
+}
+
 int bdk_rho_k(bdk_t *h, int32_t species, double dq_ratio, int32_t nbins, double *re, double *im) {
     int rc = ensure(h);
     if (rc) return rc;

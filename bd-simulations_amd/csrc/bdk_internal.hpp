@@ -209,6 +209,14 @@ hipError_t launch_pair_hist(bdk_t *h, int nbins, double dr, unsigned long long *
 hipError_t launch_rhok(bdk_t *h, int species, double dqr, int nbins, double *part, double *out);
 // radial density profiles: part [ceil(N/256)*8] scratch, centers[6], hist[6*nbins]
 hipError_t launch_radial(bdk_t *h, int nbins, double dr, double *part, double *centers, int *hist);
+// snapshot density grid (main_auswertung.cpp:214-375): acc [2][m*m] zeroed by
+// the caller, rho [2][m*m]; stats: cnt [4*nbins + 7] zeroed by the caller
+struct SnapThresh {
+    double rho[3];
+    double drho[4];
+};
+hipError_t launch_snap(bdk_t *h, int m, unsigned long long *acc, double *rho, int stats, int nbins, double dbin,
+                       SnapThresh th, unsigned int *cnt);
 hipError_t set_kernel_attributes();
 hipError_t build_tables(bdk_t *h);                // spectral tables in device memory
 

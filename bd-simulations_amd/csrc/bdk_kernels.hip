@@ -2385,6 +2385,105 @@ hipError_t launch_radial(bdk_t *h, int nbins, double dr, double *part, double *c
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ snapshot grid
+// Local densities on an m x m grid of width w = L/m (main_auswertung.cpp:
+// 214-320): each particle adds its TSC weights times 1/w^2 to the nine cells
+// around cell (int)(x/w) mod m of its species' grid.  The weight products are
+// summed as 2^-40 fixed point in 64-bit integers, so the grid does not depend
+// on the order of the additions.  The offset d from the cell centre uses the
+// unwrapped cell index (the reference wraps first, which only differs when
+// x/w rounds up to m).
+constexpr double SNAP_SCALE = 1099511627776.0;  // 2^40
+
+__device__ __forceinline__ void snap_tsc(double x, double w, int m, int &k, double wt[3]) {
+    const int k0 = (int)(x / w);
+    k = k0 % m;
+    const double dd = x - (k0 + 0.5) * w;
+    wt[1] = 0.75 - dd * dd / (w * w);
+    double t = fabs((dd + w) / w);
+    wt[0] = 0.5 * (1.5 - t) * (1.5 - t);
+    t = fabs((dd - w) / w);
+    wt[2] = 0.5 * (1.5 - t) * (1.5 - t);
+}
+
+__global__ __launch_bounds__(256) void k_snap_grid(Geo g, DevBufs d, int m, double w, unsigned long long *acc) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.N) return;
+    const int c = d.lcell[q];
+    const int gx = div_nc(c, g), gy = c - gx * g.nc;
+    const double2 r = d.lrel[q];
+    int k, l;
+    double wx[3], wy[3];
+    snap_tsc(gx * g.B + r.x, w, m, k, wx);
+    snap_tsc(gy * g.B + r.y, w, m, l, wy);
+    unsigned long long *a = acc + (d.lpid[q] >= g.n1 ? (size_t)m * m : 0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const size_t row = (size_t)((k - 1 + i + m) % m) * m;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const long long v = __double2ll_rn(wx[i] * wy[j] * SNAP_SCALE);
+            atomicAdd(&a[row + (l - 1 + j + m) % m], (unsigned long long)v);
+        }
+    }
+}
+
+// fixed point -> densities; with stats: the histograms of rho1, rho2,
+// rho = rho1 + rho2 and rho1 - rho2 and the cells over the thresholds
+// (main_auswertung.cpp:324-375), counted per block in LDS.
+// cnt: [4][nbins] histograms, then 7 threshold counts.
+__device__ __forceinline__ int snap_bin(double t, int nbins) {
+    if (!(t > -1.0)) return 0;  // (int) truncates toward zero; below 0 clamps to 0
+    if (t >= (double)nbins) return nbins - 1;
+    const int b = (int)t;
+    return b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
+}
+
+__global__ __launch_bounds__(256) void k_snap_final(int m, double drho, const unsigned long long *acc, double *rho,
+                                                    int stats, int nbins, double dbin, SnapThresh th,
+                                                    unsigned int *cnt) {
+    extern __shared__ unsigned int sh[];  // [4 * nbins + 7]
+    const int nsh = 4 * nbins + 7;
+    if (stats)
+        for (int i = threadIdx.x; i < nsh; i += blockDim.x) sh[i] = 0u;
+    __syncthreads();
+    const size_t mm = (size_t)m * m;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < mm; i += stride) {
+        const double r1 = ((double)acc[i] / SNAP_SCALE) * drho;
+        const double r2 = ((double)acc[mm + i] / SNAP_SCALE) * drho;
+        rho[i] = r1;
+        rho[mm + i] = r2;
+        if (!stats) continue;
+        const double rs = r1 + r2, dr = r1 - r2;
+        for (int t = 0; t < 3; ++t)
+            if (rs > th.rho[t]) atomicAdd(&sh[4 * nbins + t], 1u);
+        const double f = rs > 0.0 ? fabs(dr) / rs : 0.0;  // 0/0 passes no threshold
+        for (int t = 0; t < 4; ++t)
+            if (rs > 0.0 && f > th.drho[t]) atomicAdd(&sh[4 * nbins + 3 + t], 1u);
+        atomicAdd(&sh[snap_bin(r1 / dbin - 0.5, nbins)], 1u);
+        atomicAdd(&sh[nbins + snap_bin(r2 / dbin - 0.5, nbins)], 1u);
+        atomicAdd(&sh[2 * nbins + snap_bin(rs / dbin - 0.5, nbins)], 1u);
+        atomicAdd(&sh[3 * nbins + snap_bin(dr / dbin - 0.5 + (double)(nbins / 2), nbins)], 1u);
+    }
+    if (!stats) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nsh; i += blockDim.x)
+        if (sh[i]) atomicAdd(&cnt[i], sh[i]);
+}
+
+hipError_t launch_snap(bdk_t *h, int m, unsigned long long *acc, double *rho, int stats, int nbins, double dbin,
+                       SnapThresh th, unsigned int *cnt) {
+    const double w = h->p.L / m;
+    hipLaunchKernelGGL(k_snap_grid, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, m, w, acc);
+    const size_t mm = (size_t)m * m;
+    const int nb = (int)std::min<size_t>(nblk(mm, 256), 4096);
+    const size_t lds = stats ? (size_t)(4 * nbins + 7) * sizeof(unsigned int) : 0;
+    hipLaunchKernelGGL(k_snap_final, dim3(nb), dim3(256), lds, h->stream, m, 1.0 / (w * w), acc, rho, stats, nbins,
+                       dbin, th, cnt);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ step counters
 // TimestepInfo (signaturen2.h:71-145, filled by zeitschritt_debug,
 // dm:168-211): the largest force of each kind by |F|^2 with its particle,

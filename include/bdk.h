@@ -284,6 +284,24 @@ int bdk_pair_correlation(bdk_t *h, int32_t nbins, double dr, double *g11, double
    the full plane (zero here; rounding-size in the reference). */
 int bdk_rho_k(bdk_t *h, int32_t species, double dq_ratio, int32_t nbins, double *re, double *im);
 
+/* Local densities of the snapshot grid (main_auswertung.cpp:214-320) on the
+   current positions: an m x m grid of width w = L/m (the reference uses
+   m = 2 (int)(L/(2 snapgrid_crude) + 0.5), parameter.h:161-176); each particle
+   adds its TSC weights (centre 3/4 - d^2/w^2, sides (3/2 - |d +- w|/w)^2 / 2,
+   d from the centre of cell (int)(x/w)) times 1/w^2 to the nine cells around
+   it on its species' grid.  rho1/rho2 [m][m], x index first; either may be
+   NULL.  Weight products are summed in fixed point (2^-40), so the grid is
+   independent of summation order; each contribution differs from its FP64
+   value by at most 2^-41 / w^2. */
+int bdk_density_grid(bdk_t *h, int32_t m, double *rho1, double *rho2);
+/* Statistics of that grid (main_auswertung.cpp:324-385): hist[4][nbins] =
+   fractions of the m*m cells per bin of rho1, rho2, rho = rho1 + rho2 (bin =
+   (int)(x/dbin - 0.5), clamped to [0, nbins-1]) and rho1 - rho2 (bin shifted
+   by nbins/2, integer division); over[7] = fractions with rho >
+   rho_thresh[0..2] and |rho1 - rho2|/rho > drho_thresh[0..3]. */
+int bdk_density_stats(bdk_t *h, int32_t m, int32_t nbins, double dbin, const double *rho_thresh,
+                      const double *drho_thresh, double *hist, double *over);
+
 /* ---------------------------------------------------------------- batches
    Many independent simulations (the reference's `runs` and parameter sweeps
    over N, L, lambda and scheme) advanced together with one kernel launch per

@@ -226,3 +226,66 @@ def pair_correlation_ref(x1, x2, L, nbins, dr):
     tri1 = np.tril(np.ones((n1, n1), bool), -1)
     tri2 = np.tril(np.ones((n2, n2), bool), -1)
     return hist(a11, v11, tri1), hist(a12, v12, np.ones((n1, n2), bool)), hist(a22, v22, tri2)
+
+
+def tsc_snapshot_ref(x1, x2, L, m):
+    """Snapshot grid (main_auswertung.cpp:214-300), vectorised: per particle
+    k = (int)(x/w) mod m, d = x - (k+1/2) w, centre 3/4 - d^2/w^2, sides
+    (3/2 - |d +- w|/w)^2 / 2, times 1/w^2 into the 3x3 cells (x index
+    first).  Returns rho1, rho2 [m, m] (FP64 sums)."""
+    w = L / m
+    out = []
+    for x in (x1, x2):
+        g = np.zeros((m, m))
+        if len(x):
+            k0 = np.floor(x / w).astype(np.int64)  # x >= 0: (int) truncation
+            d = x - (k0 + 0.5) * w
+            t_l, t_r = np.abs((d + w) / w), np.abs((d - w) / w)
+            wt = np.stack([0.5 * (1.5 - t_l) ** 2, 0.75 - d * d / (w * w), 0.5 * (1.5 - t_r) ** 2])  # [3, n, 2]
+            for a in range(3):
+                for b in range(3):
+                    np.add.at(g, ((k0[:, 0] - 1 + a) % m, (k0[:, 1] - 1 + b) % m),
+                              wt[a, :, 0] * wt[b, :, 1] / (w * w))
+        out.append(g)
+    return out[0], out[1]
+
+
+def tsc_snapshot_loops(x1, x2, L, m):
+    """The same grid with the reference's per-particle loops, for tiny inputs."""
+    w = L / m
+    grids = []
+    for x in (x1, x2):
+        g = [[0.0] * m for _ in range(m)]
+        for px, py in x:
+            k, l = int(px / w) % m, int(py / w) % m
+            ws = []
+            for c, kk in ((px, k), (py, l)):
+                d = c - (kk + 0.5) * w
+                ws.append((0.5 * (1.5 - abs((d + w) / w)) ** 2, 0.75 - d * d / (w * w),
+                           0.5 * (1.5 - abs((d - w) / w)) ** 2))
+            for a, kk in enumerate(((k - 1 + m) % m, k, (k + 1) % m)):
+                for b, ll in enumerate(((l - 1 + m) % m, l, (l + 1) % m)):
+                    g[kk][ll] += ws[0][a] * ws[1][b] / (w * w)
+        grids.append(np.array(g))
+    return grids[0], grids[1]
+
+
+def density_stats_ref(r1, r2, nbins, dbin, rho_thresh, drho_thresh):
+    """Histograms and threshold fractions of a snapshot grid
+    (main_auswertung.cpp:324-375): rows rho1, rho2, rho1+rho2, rho1-rho2;
+    bin (int)(x/dbin - 0.5) clamped to [0, nbins-1], the difference shifted
+    by nbins // 2; over = fractions with rho > t and |drho|/rho > t."""
+    rs, dr = r1 + r2, r1 - r2
+    pre = 1.0 / r1.size
+
+    def bins(t):
+        b = np.trunc(np.clip(t, -1.0, nbins)).astype(np.int64)
+        return np.clip(b, 0, nbins - 1)
+
+    ts = [r1 / dbin - 0.5, r2 / dbin - 0.5, rs / dbin - 0.5, dr / dbin - 0.5 + nbins // 2]
+    hist = np.stack([np.bincount(bins(t).ravel(), minlength=nbins) for t in ts]) * pre
+    with np.errstate(invalid="ignore", divide="ignore"):
+        f = np.abs(dr) / rs
+    over = [np.count_nonzero(rs > t) * pre for t in rho_thresh]
+    over += [np.count_nonzero((rs > 0) & (f > t)) * pre for t in drho_thresh]
+    return hist, np.array(over)

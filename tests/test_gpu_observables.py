@@ -101,3 +101,42 @@ def test_pair_correlation_matches_brute_force(nbins, dr):
         assert np.allclose(got, ref, rtol=1e-12, atol=0)
     if dr == 0.25:
         assert g[0][:3].max() == 0.0  # nothing closer than 0.75 sigma
+
+
+@pytest.mark.parametrize("m", [None, 37])
+def test_snapshot_density_grid_and_stats(m):
+    """The snapshot grid (main_auswertung.cpp:214-320) against the numpy
+    restatement on the device positions, at the reference's grid size
+    m = 2 (int)(L/(2 * 0.5) + 0.5) and at an odd m; each species integrates
+    to its particle count; the histograms and threshold fractions
+    (main_auswertung.cpp:324-375) equal the restatement's on the device grid."""
+    from helpers import density_stats_ref, tsc_snapshot_ref
+    L = 60.0
+    m = m or 2 * int(L / (2 * 0.5) + 0.5)
+    print("case: snapshot grid m=%d" % m, flush=True)
+    bp, _ = params(n1=1800, n2=1200, L=L, Z=64, scheme=bdk.TSC, lam=20.0, precision=bdk.FP32, seed=71)
+    git, rel, _ = bdk.init_blob(bp, 71, L / 5, 0.2, 0.95)
+    s = bdk.RunDynamik(bp)
+    s.set_positions(git, rel)
+    s.run_steps(30)
+    r1, r2 = s.density_grid(m)
+    x = s.absolute()
+    w1, w2 = tsc_snapshot_ref(x[:1800], x[1800:], L, m)
+    # each of a cell's contributions is rounded to 2^-41 of a weight (fixed
+    # point), times 1/w^2; the FP64 sum adds rounding of its own
+    w = L / m
+    for got, ref, xs in ((r1, w1, x[:1800]), (r2, w2, x[1800:])):
+        k = np.floor(xs / w).astype(np.int64)
+        cnt = np.zeros((m, m))
+        for a in (-1, 0, 1):
+            for b in (-1, 0, 1):
+                np.add.at(cnt, ((k[:, 0] + a) % m, (k[:, 1] + b) % m), 1.0)
+        bound = cnt * (2.0 ** -41 + 1e-15) / (w * w)
+        assert np.all(np.abs(got - ref) <= bound)
+    assert abs(r1.sum() * w * w - 1800) < 1e-8 and abs(r2.sum() * w * w - 1200) < 1e-8
+    th, dth = (0.1, 0.2, 0.5), (0.1, 0.2, 0.5, 0.99)
+    hist, over = s.density_stats(m, 100, 0.01, th, dth)
+    hr, orf = density_stats_ref(r1, r2, 100, 0.01, th, dth)
+    assert np.array_equal(hist, hr) and np.array_equal(over, orf)
+    assert np.allclose(hist.sum(1), 1.0)
+    s.close()
