@@ -130,7 +130,7 @@ void release_streams(bdk_t *h) {
 
 void free_dev(bdk_t *h) {
     void *ptrs[] = {h->d.cell, h->d.rel, h->d.pid, h->d.fwca, h->d.fkap, h->d.noise_up, h->d.noise_out,
-                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.dts, h->d.comp, h->d.lcomp, h->d.rho, h->d.ht,
+                    h->d.rank, h->d.count, h->d.arr, h->d.start, h->d.list, h->d.lcell, h->d.lrel, h->d.lpid, h->d.lrelf, h->d.bcount, h->d.own, h->d.start_old, h->d.exo, h->d.dts, h->d.dtlog, h->d.comp, h->d.lcomp, h->d.rho, h->d.ht,
                     h->d.gt, h->d.fmesh, h->d.tw, h->d.sq, h->d.lap, h->d.qv, h->d.sqf, h->d.lapf,
                     h->d.qvf, h->d.sc};
     for (void *q : ptrs)
@@ -586,6 +586,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     }
     ALLOC(dalloc(d.exo, 64 * 16));
     ALLOC(dalloc(d.dts, 64 * 16));
+    ALLOC(dalloc(d.dtlog, bdkx::kDtLog));
     // list-ordered arrays carry one padding entry (read ahead by k_wca_f32)
     ALLOC(dalloc(d.list, N + 1));
     ALLOC(dalloc(d.lcell, N + 1));
@@ -621,6 +622,7 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     if (e == hipSuccess) e = hipMemset(d.count, 0, ncell * 4);
     if (e == hipSuccess) e = hipMemset(d.exo, 0, 64 * 16 * 8);
     if (e == hipSuccess) e = hipMemset(d.dts, 0xff, 64 * 16 * 8);
+    if (e == hipSuccess) e = hipMemset(d.dtlog, 0, bdkx::kDtLog * 8);
     if (e == hipSuccess && d.comp) e = hipMemset(d.comp, 0, (N + 1) * 8);
     if (e == hipSuccess && d.lcomp) e = hipMemset(d.lcomp, 0, (N + 1) * 8);
     if (e == hipSuccess) e = hipMemset(d.arr, 0, ncell * 4);
@@ -853,12 +855,45 @@ int bdk_step_info(bdk_t *h, double tmax, bdk_step_info_t *out) {
     return check_nonfinite(h);
 }
 
+// adaptive dt whose steps can be replayed from the step graphs (the dt
+// stays on the device; the host reads the log of the dt's afterwards)
+static bool adaptive_graphable(const bdk_t *h) {
+    return h->g.adaptive && !h->g.idgas && h->p.noise_mode != BDK_NOISE_UPLOAD;
+}
+
+// k <= kDtLog adaptive steps replayed from the step graphs, then their dt's
+// read from the device log and added to *t in step order, in long double
+// as zeitschritte_bis_obs adds them (dm:444-455).  Forces must be ready.
+static int run_logged(bdk_t *h, int64_t k, long double *t) {
+    int rc = ensure_graph(h);
+    if (rc) return rc;
+    int64_t i = 0;
+    for (; i + kGraphSteps <= k; i += kGraphSteps) HIPCHK(h, hipGraphLaunch(h->gexec_multi, h->stream));
+    for (; i < k; ++i) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    std::vector<double> log(bdkx::kDtLog);
+    DevScalars sc;
+    HIPCHK(h, hipMemcpyAsync(log.data(), h->d.dtlog, bdkx::kDtLog * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->host_step += k;
+    const unsigned long long s0 = sc.step - (unsigned long long)k;
+    for (int64_t j = 0; j < k; ++j) *t += (long double)log[(s0 + (unsigned long long)j) & (bdkx::kDtLog - 1)];
+    return BDK_OK;
+}
+
 int bdk_run_steps(bdk_t *h, int64_t k, double *t_out) {
     int rc = ensure_forces(h);
     if (rc) return rc;
     if (k < 0) return fail(h, BDK_ERR_ARG, "k must be >= 0");
     long double t = 0.0L;
-    if (!h->g.adaptive && h->p.noise_mode != BDK_NOISE_UPLOAD) {
+    if (adaptive_graphable(h)) {
+        for (int64_t i = 0; i < k;) {
+            const int64_t b = std::min<int64_t>(k - i, bdkx::kDtLog);
+            rc = run_logged(h, b, &t);
+            if (rc) return rc;
+            i += b;
+        }
+    } else if (!h->g.adaptive && h->p.noise_mode != BDK_NOISE_UPLOAD) {
         rc = ensure_graph(h);
         if (rc) return rc;
         int64_t i = 0;
@@ -901,6 +936,21 @@ int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out) {
         if (rc) return rc;
     } else {
         long double t = 0.0L;
+        if (adaptive_graphable(h)) {
+            // dt <= dt_max, so while the clock is more than (k + 2) dt_max
+            // short of obs_dt the next k steps are all taken: replay them
+            // from the graphs and add their logged dt's in order
+            rc = ensure_forces(h);
+            if (rc) return rc;
+            for (;;) {
+                const long double room = ((long double)obs_dt - t) / (long double)h->p.dt_max;
+                if (!(room > 18.0L)) break;
+                const int64_t k = room > (long double)(bdkx::kDtLog + 2) ? bdkx::kDtLog : (int64_t)room - 2;
+                rc = run_logged(h, k, &t);
+                if (rc) return rc;
+                n += k;
+            }
+        }
         while (t < obs_dt) {
             double dt = 0.0;
             rc = bdk_step(h, (double)((long double)obs_dt - t), &dt);

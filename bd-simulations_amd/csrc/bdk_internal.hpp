@@ -47,6 +47,9 @@ struct Geo {
 };
 
 // Device-resident scalars of one simulation (graph replays read them).
+// entries of the adaptive-dt log: the most steps replayed without reading it
+constexpr int kDtLog = 4096;
+
 struct DevScalars {
     unsigned long long step;      // step counter (noise counter)
     unsigned long long dt_bits;   // adaptive dt: running min, double bits
@@ -96,6 +99,7 @@ struct DevBufs {
     unsigned long long *exo = nullptr; // [64 x 16] max distance of a new position outside its previous cell (double bits)
     unsigned long long *dts = nullptr; // [64 x 16] adaptive dt: minima of the candidates by block (double bits;
                                        // all ones = none), folded into sc->dt_bits by k_dt_fold
+    double *dtlog = nullptr;  // [kDtLog] adaptive dt of step s at s mod kDtLog (written by k_dt_fold)
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
     // the state in list order (k_fill): cell, rel and particle id of list entry q
     int *lcell = nullptr;

@@ -1295,8 +1295,8 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
 }
 
 // dt of the step: dt_max and the 64 slot minima of k_gather folded into
-// sc->dt_bits; the slots are re-armed (all ones) for the next reduction.
-// One wave.
+// sc->dt_bits and logged by step; the slots are re-armed (all ones) for the
+// next reduction.  One wave.
 __global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) {
     const int t = threadIdx.x;
     unsigned long long m = d.dts[t * 16];
@@ -1308,7 +1308,9 @@ __global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) {
     }
     if (t == 0) {
         const unsigned long long dm = bitsd(g.dt_max);
-        d.sc->dt_bits = m < dm ? m : dm;
+        const unsigned long long dt = m < dm ? m : dm;
+        d.sc->dt_bits = dt;
+        d.dtlog[d.sc->step & (kDtLog - 1)] = dbits(dt);
     }
 }
 
