@@ -299,6 +299,12 @@ def main():
                           "frac": canon / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
                           "step_ms_events": step_ms_events},
         "kernels_ms": kernels,
+        # every kernel against the HBM peak by its algorithmic bytes; k_update is
+        # the fused gather + noise + Euler update (40 N + 8 Z^2 bytes)
+        "kernels_roofline": {k: {"algorithmic_bytes": algorithmic_bytes(k, N, w["Z"]),
+                                 "achieved_GBs": algorithmic_bytes(k, N, w["Z"]) / (ms * 1e-3) / 1e9,
+                                 "frac": algorithmic_bytes(k, N, w["Z"]) / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+                             for k, ms in kernels.items()},
     }
     sim.close()
     out["adaptive_dt"] = adaptive_figure(bdk, w, git, rel, args.steps, args.warmup, N)
