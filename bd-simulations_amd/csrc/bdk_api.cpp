@@ -927,15 +927,20 @@ int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out) {
 }
 
 int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out) {
+    return bdk_run_interval_clock(h, obs_dt, steps_out, nullptr, nullptr);
+}
+
+int bdk_run_interval_clock(bdk_t *h, double obs_dt, int64_t *steps_out, double *t_hi, double *t_lo) {
     int rc = ensure(h);
     if (rc) return rc;
     int64_t n = 0;
+    long double t = 0.0L;
     if (!h->g.adaptive) {
         bdk_steps_for_interval(obs_dt, h->p.dt_max, &n);
         rc = bdk_run_steps(h, n, nullptr);
         if (rc) return rc;
+        for (int64_t j = 0; j < n; ++j) t += (long double)h->p.dt_max;
     } else {
-        long double t = 0.0L;
         if (adaptive_graphable(h)) {
             // dt <= dt_max, so while the clock is more than (k + 2) dt_max
             // short of obs_dt the next k steps are all taken: replay them
@@ -962,6 +967,11 @@ int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out) {
         if (rc) return rc;
     }
     if (steps_out) *steps_out = n;
+    // the clock as an unevaluated sum of two doubles: the 64-bit significand
+    // of the long double fits in the two 53-bit ones, so hi + lo is exact
+    const double hi = (double)t;
+    if (t_hi) *t_hi = hi;
+    if (t_lo) *t_lo = (double)(t - (long double)hi);
     return BDK_OK;
 }
 

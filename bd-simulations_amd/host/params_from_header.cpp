@@ -3,7 +3,7 @@
 // the reference compiles every translation unit against its parameter.h.
 // Environment overrides: BDK_SEED (MT seed; the reference uses time(NULL)),
 // BDK_NOISE=mt|disc (host MT stream uploaded per step, or device Philox
-// disc noise), BDK_PRECISION=fp64|fp32.
+// disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -49,5 +49,6 @@ RunConfig config_from_parameter_h() {
     c.p.seed = c.seed;
     const char *pr = std::getenv("BDK_PRECISION");
     c.p.precision = (pr && std::strcmp(pr, "fp64") == 0) ? BDK_FP64 : BDK_FP32;
+    if (const char *mt = std::getenv("BDK_MAX_TRAVEL")) c.p.max_travel = std::strtod(mt, nullptr);
     return c;
 }

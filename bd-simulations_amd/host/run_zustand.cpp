@@ -169,14 +169,15 @@ void RunZustand::init(int nr_) {
 
 void RunZustand::zeitschritte_bis_obs(time_t last_write) {
     (void)last_write;
-    if (std::isinf(cfg_.p.max_travel) && cfg_.p.noise_mode != BDK_NOISE_UPLOAD) {
-        // fixed dt: the long-double loop of dm:444-455 has a closed step count
+    if (cfg_.p.noise_mode != BDK_NOISE_UPLOAD) {
+        // the library runs the loop of dm:444-455 on the device (fixed dt: a
+        // closed step count; adaptive dt: logged dt's summed in long double)
+        // and returns the clock exactly as hi + lo
         int64_t k = 0;
-        check(dyn.h, bdk_run_interval(dyn.h, cfg_.obs_dt, &k), "run_interval");
+        double hi = 0.0, lo = 0.0;
+        check(dyn.h, bdk_run_interval_clock(dyn.h, cfg_.obs_dt, &k, &hi, &lo), "run_interval");
         schritte_seit_obs += (int)k;
-        long double tt = 0.0L;
-        for (int64_t i = 0; i < k; ++i) tt += (long double)cfg_.p.dt_max;
-        t = tt;
+        t = (long double)hi + (long double)lo;
         return;
     }
     while (t < cfg_.obs_dt) {

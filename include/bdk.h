@@ -130,6 +130,10 @@ int bdk_run_steps(bdk_t *h, int64_t k, double *t_out);
 /* zeitschritte_bis_obs (dm:444-455): steps until the long-double clock,
    starting at 0 (obs_point, au:90), reaches obs_dt. */
 int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out);
+/* The same interval, also returning the long-double clock it ended with
+   (RunZustand::t, s2:375) exactly as t_hi + t_lo (the 64-bit significand of
+   the x87 long double fits in two doubles); either pointer may be NULL. */
+int bdk_run_interval_clock(bdk_t *h, double obs_dt, int64_t *steps_out, double *t_hi, double *t_lo);
 
 /* Number of steps zeitschritte_bis_obs takes for a constant dt. */
 int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out);

@@ -3,7 +3,8 @@ are certain to be taken (the clock is more than k + 2 dt_max short of obs_dt)
 and adds their logged dt's in long double; bdk_run_steps replays all of
 them.  Both must take the same steps as the reference's loop of
 zeitschritt calls (dm:444-455: while t < obs_dt, t += dt in long double):
-same step count, same clock, positions bit for bit.  A small travel cap
+same step count, same clock (bdk_run_interval_clock returns it exactly),
+positions bit for bit.  A small travel cap
 makes dt vary from step to step."""
 import numpy as np
 import pytest
@@ -27,7 +28,10 @@ def test_interval_from_graphs_equals_step_loop(prec, travel, obs_dt):
         s.set_positions(git, rel)
     dts = []
     for nr in range(2):
-        n_a = a.zeitschritte_bis_obs(obs_dt)
+        if nr == 0:
+            n_a, t_a = a.zeitschritte_bis_obs(obs_dt), None
+        else:
+            n_a, t_a = a.run_interval_clock(obs_dt)  # the clock, exactly
         t = np.longdouble(0.0)
         n_b = 0
         while t < np.longdouble(obs_dt):
@@ -36,6 +40,7 @@ def test_interval_from_graphs_equals_step_loop(prec, travel, obs_dt):
             t += np.longdouble(dt)
             n_b += 1
         assert n_a == n_b, (nr, n_a, n_b)
+        assert t_a is None or t_a == t
         ga, ra = a.positions()
         gb, rb = b.positions()
         assert np.array_equal(ga, gb) and np.array_equal(ra, rb)

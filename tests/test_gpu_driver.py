@@ -1,0 +1,57 @@
+"""The C++ drop-in driver (bd-simulations_amd/bdk_sim: main.cpp:130-300's run
+loop over RunZustand, built against host/parameter.h: N1 = N2 = 1000, L = 64,
+CIC 128^2, lambda 8, dt_max 1e-4, obs_dt 0.1, 11 checkpoints) writes the same
+pos1_1.txt / pos2_1.txt bytes as the same run driven through the Python
+binding: reference start positions from the seeded mt19937_64 (init_zufallspos,
+dm:1647-1678), then per checkpoint zeitschritte_bis_obs and schreibe_pos
+(au:631-648).  Fixed dt, and adaptive dt with a travel cap (BDK_MAX_TRAVEL),
+whose interval runs from the step graphs with the long-double clock."""
+import os
+import subprocess
+
+import pytest
+
+import bdk
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(os.path.dirname(HERE), "bd-simulations_amd", "bdk_sim")
+SEED = 12345
+
+
+def read(path):
+    with open(path, "rb") as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("travel", [None, 0.02])
+def test_driver_files_equal_binding_run(tmp_path, travel):
+    print("case: C++ driver, max_travel=%s" % travel, flush=True)
+    assert os.path.exists(EXE), "bdk_sim is not built (python bd-simulations_amd/build.py)"
+    env = dict(os.environ, BDK_SEED=str(SEED))
+    if travel is not None:
+        env["BDK_MAX_TRAVEL"] = repr(travel)
+    run_dir = tmp_path / "driver"
+    run_dir.mkdir()
+    out = subprocess.run([EXE], cwd=str(run_dir), env=env, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    kw = dict(n1=1000, n2=1000, L=64.0, Z=128, scheme=bdk.CIC, lam=8.0, kap=1.0, T=1.0, dt_max=1e-4, f2_f1=0.5,
+              precision=bdk.FP32, noise_mode=bdk.NOISE_DISC, seed=SEED)
+    if travel is not None:
+        kw["max_travel"] = travel
+    p = bdk.make_params(**kw)
+    git, rel = bdk.MT(SEED + 0).init_positions(p)  # run 0 draws from seed + run
+    s = bdk.RunDynamik(p)
+    s.set_positions(git, rel)
+    q1, q2 = str(tmp_path / "q1.txt"), str(tmp_path / "q2.txt")
+    s.write_pos(q1, q2, 0, 0.1)
+    steps = []
+    for nr in range(1, 11):
+        steps.append(s.zeitschritte_bis_obs(0.1))
+        s.write_pos(q1, q2, nr, 0.1)
+    s.close()
+    if travel is None:
+        assert steps == [1001] * 10
+    assert read(str(run_dir / "pos1_1.txt")) == read(q1)
+    assert read(str(run_dir / "pos2_1.txt")) == read(q2)
