@@ -9,7 +9,10 @@ host oracle (an mt19937_64 seed per replica).  Per replica: the mean squared
 displacement of each species, the mean squared distance of each species from
 its circular-mean centre (berechne_mittelpunkte, observablen_auswertung.cpp:
 140-223), and the first bins of the species-1 radial profile around the
-common centre (berechne_dichteprofile, :227-330).  Tolerance: every ensemble
+common centre (berechne_dichteprofile, :227-330), and shells 1-3 of the
+species-1 rho(k) (record_rhoFFTW, auswertung.cpp:529-626: the device
+observable on the device ensemble, its numpy restatement on the host
+ensemble).  Tolerance: every ensemble
 mean agrees within 4.5 combined standard errors.  A noise variance of 1
 instead of 5/4 moves the displacement means by about 15 standard errors.
 """
@@ -19,11 +22,12 @@ import pytest
 import bdk
 import oracle as O
 from common import params
-from helpers import min_image, radial_profile_ref
+from helpers import min_image, radial_profile_ref, rho_k_ref
 
 pytestmark = pytest.mark.gpu
 
 M, STEPS, NB, DR = 48, 400, 4, 1.0
+DQR, NK = 1.0, 4  # rho(k) shells 1-3 of species 1 (record_rhoFFTW, auswertung.cpp:529-626)
 SPEC = dict(n1=150, n2=150, L=30.0, Z=64, scheme=bdk.CIC, lam=10.0, kap=3.0, T=1.0, dt_max=2e-4,
             precision=bdk.FP64)
 
@@ -34,7 +38,7 @@ def centre(x, L):
                      for a in range(2)])
 
 
-def stats(x0, x, n1, L):
+def stats(x0, x, n1, L, rhok):
     d = min_image(x - x0, L)
     out = [np.mean((d[:n1] ** 2).sum(1)), np.mean((d[n1:] ** 2).sum(1))]
     for sp in (x[:n1], x[n1:]):
@@ -42,6 +46,7 @@ def stats(x0, x, n1, L):
         out.append(np.mean((r ** 2).sum(1)))
     _, prof = radial_profile_ref(x[:n1], x[n1:], L, NB, DR)
     out += list(prof[0][:3])
+    out += list(rhok)
     return np.array(out)
 
 
@@ -64,7 +69,8 @@ def test_philox_ensemble_matches_reference_mt_ensemble():
     dev = []
     for s in sims:
         g, rr = s.positions()
-        dev.append(stats(x0, g * B + rr, n1, L))
+        re, _ = s.rho_k(1, DQR, NK)  # the device observable (record_rhoFFTW)
+        dev.append(stats(x0, g * B + rr, n1, L, re[1:NK]))
         s.close()
     b.close()
     host = []
@@ -76,10 +82,13 @@ def test_philox_ensemble_matches_reference_mt_ensemble():
         for _ in range(STEPS):
             o.step(1.0)
         g, rr = o.all_positions()
-        host.append(stats(x0, g * o.B + rr, n1, L))
+        xh = g * o.B + rr
+        rk = rho_k_ref(xh[:n1], L, SPEC["Z"], SPEC["scheme"], DQR, NK)
+        host.append(stats(x0, xh, n1, L, rk[1:NK]))
         o.close()
     dev, host = np.array(dev), np.array(host)
-    names = ["msd1", "msd2", "rg2_1", "rg2_2", "rho1_b0", "rho1_b1", "rho1_b2"]
+    names = ["msd1", "msd2", "rg2_1", "rg2_2", "rho1_b0", "rho1_b1", "rho1_b2", "rhok1_s1", "rhok1_s2",
+             "rhok1_s3"]
     for k, nm in enumerate(names):
         md, mh = dev[:, k].mean(), host[:, k].mean()
         se = np.sqrt(dev[:, k].var(ddof=1) / M + host[:, k].var(ddof=1) / M)
