@@ -328,3 +328,47 @@ def test_error_paths_on_device():
     assert e.value.code == bdk.ERR_NONFINITE
     g.set_positions(git, rel)
     assert g.zeitschritt(-1.0) == 1e10 and g.step_counter() == 0
+
+
+@pytest.mark.parametrize("prec", [bdk.FP64, bdk.FP32])
+def test_single_species_run(prec, tmp_path):
+    """N2 = 0 (the reference then opens no pos2 file, au:57-69): forces
+    against the oracle, an FP64 trajectory, and the writer."""
+    say("case: single species N2=0 prec=%d" % prec)
+    g, o, git, rel = pair(n1=800, n2=0, L=40.0, Z=64, scheme=bdk.TSC, seed=81, precision=prec)
+    tol = 1e-12 if prec == bdk.FP64 else 1e-5
+    assert rel_rms(g.berechne_WCA(), o.wca()) < tol
+    assert rel_rms(g.berechne_kapkraefte(), o.kap()) < tol
+    if prec == bdk.FP64:
+        for _ in range(50):
+            o.step(1.0)
+        g.run_steps(50)
+        gg, gr = g.positions()
+        og, orl = o.all_positions()
+        assert np.array_equal(gg, og) and np.abs(gr - orl).max() < 1e-10
+    p1, p2 = str(tmp_path / "p1.txt"), str(tmp_path / "p2.txt")
+    g.write_pos(p1, p2, 0, 0.1)
+    assert os.path.exists(p1) and not os.path.exists(p2)
+    assert len(open(p1).read().split("\n")) == 2 + 800 + 2
+
+
+def test_two_particles():
+    """One particle of each species 1.05 apart: the WCA pair (Newton's third
+    law) and the capillary pair force against the oracle, and 20 steps."""
+    say("case: two particles")
+    bp, _ = params(n1=1, n2=1, L=20.0, Z=32)
+    der = bdk.derived(bp)
+    x = np.array([[5.0, 5.0], [5.9, 5.54]])
+    git, rel = to_git_rel(x, der["B"], der["nc"])
+    g, o, _, _ = pair(git=git, rel=rel, n1=1, n2=1, L=20.0, Z=32, scheme=bdk.CIC)
+    fw, ow = g.berechne_WCA(), o.wca()
+    assert np.abs(fw - ow).max() <= 1e-12 * np.abs(ow).max()
+    assert np.abs(fw.sum(0)).max() <= 1e-12 * np.abs(fw).max()
+    fk, ok = g.berechne_kapkraefte(), o.kap()
+    assert np.abs(fk - ok).max() <= 1e-12 * np.abs(ok).max()
+    for _ in range(20):
+        o.step(1.0)
+    g.run_steps(20)
+    gg, gr = g.positions()
+    og, orl = o.all_positions()
+    assert np.array_equal(gg, og) and np.abs(gr - orl).max() < 1e-10
