@@ -1315,8 +1315,8 @@ __global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) {
 }
 
 // Euler step (dm:116-157) + refresh (dm:216-325) + cell counting of list
-// position s (written to slot s).  Returns how far the new position lies
-// outside the particle's previous cell (<= 0 inside).
+// position s (written to slot s).  Returns how far the stored new position
+// (cell and offset) lies outside the particle's previous cell (<= 0 inside).
 // FUSED: capillary force gathered here (fixed dt); else read from fkap.
 template <typename R, int SCHEME, bool FUSED>
 __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int s) {
@@ -1387,14 +1387,22 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
         atomicOr(&d.sc->nonfinite, 1);
         rn = r;
     }
-    const double ex_prev = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
+    // the stored new position (cell, offset) relative to the previous cell's
+    // corner: the bound the deposit beside the rebuild stages by
+    double ox = rn.x, oy = rn.y;
     if (finite && (rn.x < 0.0 || rn.x > B || rn.y < 0.0 || rn.y > B)) {
-        int ic = (int)floor((gx * B + rn.x) / B);
-        int jc = (int)floor((gy * B + rn.y) / B);
-        ic = wrapc(ic + g.nc, g.nc);
-        jc = wrapc(jc + g.nc, g.nc);
+        const int ic0 = (int)floor((gx * B + rn.x) / B);
+        const int jc0 = (int)floor((gy * B + rn.y) / B);
+        const int ic = wrapc(ic0 + g.nc, g.nc);
+        const int jc = wrapc(jc0 + g.nc, g.nc);
         rn.x = fmod(rn.x + B, B);
         rn.y = fmod(rn.y + B, B);
+        // a move of more than one cell downwards keeps a negative offset
+        // (fmod takes the sign of rel + B, dm:216-325): the stored position
+        // then lies one cell below the moved one, so the bound follows the
+        // stored cell and offset
+        ox = (ic0 - gx) * B + rn.x;
+        oy = (jc0 - gy) * B + rn.y;
         gx = ic;
         gy = jc;
         nc_new = gx * g.nc + gy;
@@ -1404,10 +1412,13 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
             const float wx = (float)rn.x, wy = (float)rn.y;
             kc.x += (float)((double)wx - rn.x);
             kc.y += (float)((double)wy - rn.y);
+            ox += (double)wx - rn.x;
+            oy += (double)wy - rn.y;
             rn.x = (double)wx;
             rn.y = (double)wy;
         }
     }
+    const double ex_prev = fmax(fmax(-ox, ox - B), fmax(-oy, oy - B));
     d.cell[s] = nc_new;
     d.rel[s] = rn;
     d.pid[s] = pid;
