@@ -83,6 +83,7 @@ def lib():
             "bdk_run_steps": (C.c_int, [vp, i64, vp]),
             "bdk_run_interval": (C.c_int, [vp, dbl, vp]),
             "bdk_run_interval_clock": (C.c_int, [vp, dbl, vp, vp, vp]),
+            "bdk_set_max_travel": (C.c_int, [vp, dbl]),
             "bdk_steps_for_interval": (C.c_int, [dbl, dbl, vp]),
             "bdk_refresh": (C.c_int, [vp]),
             "bdk_sync": (C.c_int, [vp]),
@@ -138,7 +139,7 @@ def exported_symbols():
     return ["bdk_params_default", "bdk_params_derived", "bdk_create", "bdk_destroy", "bdk_last_error",
             "bdk_set_positions", "bdk_get_positions", "bdk_compute_wca", "bdk_compute_capillary",
             "bdk_set_noise", "bdk_get_noise", "bdk_optimal_dt", "bdk_step", "bdk_run_steps",
-            "bdk_run_interval", "bdk_run_interval_clock", "bdk_steps_for_interval", "bdk_refresh", "bdk_sync", "bdk_step_counter",
+            "bdk_run_interval", "bdk_run_interval_clock", "bdk_set_max_travel", "bdk_steps_for_interval", "bdk_refresh", "bdk_sync", "bdk_step_counter",
             "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_profile_kernel",
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
@@ -355,6 +356,10 @@ class RunDynamik:
         n = C.c_int64(0)
         self._chk(self.L.bdk_run_interval(self.h, obs_dt, C.byref(n)))
         return n.value
+
+    def set_max_travel(self, max_travel):
+        """Travel cap of the following steps (+inf: fixed dt)."""
+        self._chk(self.L.bdk_set_max_travel(self.h, max_travel))
 
     def run_interval_clock(self, obs_dt):
         """(steps, clock) of zeitschritte_bis_obs; the clock as numpy.longdouble

@@ -201,6 +201,10 @@ int check_nonfinite(bdk_t *h) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (sc.nonfinite & 2)
         return fail(h, BDK_ERR_HIP, "internal error: inconsistent cell lists (cell counts do not sum to N)");
+    if (sc.nonfinite & 4)
+        return fail(h, BDK_ERR_HIP,
+                    "internal error: a particle's charge stencil lay outside its deposit tile's window "
+                    "(displacement bound too small)");
     if (sc.nonfinite)
         return fail(h, BDK_ERR_NONFINITE,
                     "a particle position became non-finite or ran away (|rel| >= 1e9); it was held in place");
@@ -912,6 +916,18 @@ int bdk_run_steps(bdk_t *h, int64_t k, double *t_out) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (t_out) *t_out = (double)t;
     return check_nonfinite(h);
+}
+
+int bdk_set_max_travel(bdk_t *h, double max_travel) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    if (!(max_travel > 0.0)) return fail(h, BDK_ERR_ARG, "max_travel must be > 0 (+inf: fixed dt)");
+    if (h->p.max_travel == max_travel) return BDK_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->p.max_travel = max_travel;
+    h->g.adaptive = std::isfinite(max_travel) ? 1 : 0;
+    h->g.mrd = max_travel;
+    drop_graph(h);  // the captured steps carry the old Geo
+    return BDK_OK;
 }
 
 int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out) {

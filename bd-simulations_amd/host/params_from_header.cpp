@@ -3,7 +3,8 @@
 // the reference compiles every translation unit against its parameter.h.
 // Environment overrides: BDK_SEED (MT seed; the reference uses time(NULL)),
 // BDK_NOISE=mt|disc (host MT stream uploaded per step, or device Philox
-// disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz).
+// disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz),
+// BDK_MODE=run|debug|krafttest (sets debugmode / krafttestmode).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -41,6 +42,11 @@ RunConfig config_from_parameter_h() {
     c.max_write_interval = max_write_interval;
     c.auswerten_animation = auswerten_animation;
     c.krafttestmode = krafttestmode;
+    c.debugmode = debugmode;
+    if (const char *m = std::getenv("BDK_MODE")) {
+        c.debugmode = std::strcmp(m, "debug") == 0;
+        c.krafttestmode = std::strcmp(m, "krafttest") == 0;
+    }
     const char *s = std::getenv("BDK_SEED");
     c.seed = s ? std::strtoull(s, nullptr, 10) : (uint64_t)std::time(nullptr);
     const char *nz = std::getenv("BDK_NOISE");

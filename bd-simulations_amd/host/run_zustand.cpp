@@ -97,9 +97,49 @@ double RunZustand::RunDynamik::zeitschritt(double tmax) {
         return 1.0e10;
     }
     if (cfg->p.noise_mode == BDK_NOISE_UPLOAD && !cfg->p.no_wca) berechne_zufallskraefte();
+    // the parameter's travel cap (zeitschritt_debug may have set another)
+    check(h, bdk_set_max_travel(h, cfg->p.max_travel), "set_max_travel");
     double dt = 0.0;
     check(h, bdk_step(h, tmax, &dt), "step");
     return dt;
+}
+
+void RunZustand::RunDynamik::zeitschritt_debug(double mrd, TimestepInfo &info, bool print) {
+    if (cfg->p.noise_mode == BDK_NOISE_UPLOAD && !cfg->p.no_wca) berechne_zufallskraefte();
+    check(h, bdk_set_max_travel(h, mrd), "set_max_travel");
+    bdk_step_info_t s;
+    check(h, bdk_step_info(h, 1.0, &s), "step_info");
+    info.num_WCA_berechnet = (int)s.num_wca;
+    info.maxWCAx = s.max_wca[0];
+    info.maxWCAy = s.max_wca[1];
+    info.maxWCAtype = s.max_wca_type;
+    info.maxWCAindex = s.max_wca_index;
+    info.maxWCA_numWW1 = s.max_wca_nlist[0];
+    info.maxWCA_numWW2 = s.max_wca_nlist[1];
+    info.maxKapx = s.max_kap[0];
+    info.maxKapy = s.max_kap[1];
+    info.maxKaptype = s.max_kap_type;
+    info.maxKapindex = s.max_kap_index;
+    info.maxRNDx = s.max_rnd[0];
+    info.maxRNDy = s.max_rnd[1];
+    info.maxRNDtype = s.max_rnd_type;
+    info.maxRNDindex = s.max_rnd_index;
+    info.dt = s.dt;
+    info.num_Zellwechsel1 = s.cell_changes[0];
+    info.num_Zellwechsel2 = s.cell_changes[1];
+    if (print) info.ausgabe();
+}
+
+void RunZustand::RunDynamik::TimestepInfo::ausgabe() const {
+    std::cout << "WCA pairs evaluated: " << num_WCA_berechnet << "; largest WCA force (" << maxWCAx << ","
+              << maxWCAy << ") on particle " << maxWCAindex << " of species " << maxWCAtype << ", neighbour list "
+              << maxWCA_numWW1 << "+" << maxWCA_numWW2 << "\n"
+              << "cell changes: " << num_Zellwechsel1 << "+" << num_Zellwechsel2 << "\n"
+              << "largest capillary force (" << maxKapx << "," << maxKapy << ") on particle " << maxKapindex
+              << " of species " << maxKaptype << "\n"
+              << "largest random force (" << maxRNDx << "," << maxRNDy << ") on particle " << maxRNDindex
+              << " of species " << maxRNDtype << "\n"
+              << "dt = " << dt << std::endl;
 }
 
 RunZustand::~RunZustand() { obs.close(); }

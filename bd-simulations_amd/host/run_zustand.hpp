@@ -28,6 +28,7 @@ struct RunConfig {
     int max_write_interval;
     bool auswerten_animation;
     bool krafttestmode;
+    bool debugmode;
     uint64_t seed;      // MT seed (the reference seeds with time(NULL), zufall.cpp:18)
     int noise_source;   // 0: device Philox disc noise; 1: host MT stream uploaded per step
 };
@@ -43,8 +44,27 @@ class RunZustand {
         RunDynamik(const RunDynamik &) = delete;
         RunDynamik &operator=(const RunDynamik &) = delete;
 
+        // The counters of one step (signaturen2.h:71-145), filled by
+        // zeitschritt_debug.
+        struct TimestepInfo {
+            int num_WCA_berechnet = -1;
+            double maxWCAx = 0.0, maxWCAy = 0.0;
+            int maxWCAtype = -1, maxWCAindex = -1;
+            int maxWCA_numWW1 = -1, maxWCA_numWW2 = -1;
+            double maxKapx = 0.0, maxKapy = 0.0;
+            int maxKaptype = -1, maxKapindex = -1;
+            double maxRNDx = 0.0, maxRNDy = 0.0;
+            int maxRNDtype = -1, maxRNDindex = -1;
+            double dt = -1.0;
+            int num_Zellwechsel1 = -1, num_Zellwechsel2 = -1;
+            void ausgabe() const;
+        };
+
         // dm:74-163
         double zeitschritt(double tmax);
+        // dm:168-211: one step with travel cap mrd, counters into info
+        // (print: info.ausgabe())
+        void zeitschritt_debug(double mrd, TimestepInfo &info, bool print);
         // am:361-364: lists from the current positions, spectral tables
         void init();
         void berechne_kapkraefte();

@@ -135,6 +135,12 @@ int bdk_run_interval(bdk_t *h, double obs_dt, int64_t *steps_out);
    the x87 long double fits in two doubles); either pointer may be NULL. */
 int bdk_run_interval_clock(bdk_t *h, double obs_dt, int64_t *steps_out, double *t_hi, double *t_lo);
 
+/* The travel cap of the following steps (max_reisedistanz; +inf gives
+   dt = dt_max every step).  zeitschritt_debug(mrd, ...) (dm:168-211) takes
+   the cap per call, as the mrd sweep of main.cpp:156-216 does.  Not while the
+   handle belongs to a batch. */
+int bdk_set_max_travel(bdk_t *h, double max_travel);
+
 /* Number of steps zeitschritte_bis_obs takes for a constant dt. */
 int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out);
 

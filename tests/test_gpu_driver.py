@@ -55,3 +55,61 @@ def test_driver_files_equal_binding_run(tmp_path, travel):
         assert steps == [1001] * 10
     assert read(str(run_dir / "pos1_1.txt")) == read(q1)
     assert read(str(run_dir / "pos2_1.txt")) == read(q2)
+
+
+def test_driver_force_scan_equals_binding(tmp_path):
+    """krafttestmode (main.cpp:220-248): kapkraft.txt of the driver equals the
+    same scan through the binding (particle 1 moved to y = x_0 + t, wrapped
+    into the box; F1kap[1][1] and F1kap[1][1] + F1_WCA[1][1] written)."""
+    import math
+    print("case: C++ driver, krafttestmode", flush=True)
+    assert os.path.exists(EXE), "bdk_sim is not built (python bd-simulations_amd/build.py)"
+    env = dict(os.environ, BDK_SEED=str(SEED), BDK_MODE="krafttest")
+    out = subprocess.run([EXE], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    p = bdk.make_params(n1=1000, n2=1000, L=64.0, Z=128, scheme=bdk.CIC, lam=8.0, kap=1.0, T=1.0, dt_max=1e-4,
+                        f2_f1=0.5, precision=bdk.FP32, noise_mode=bdk.NOISE_DISC, seed=SEED)
+    B = bdk.derived(p)["B"]
+    git, rel = bdk.MT(SEED).init_positions(p)
+    s = bdk.RunDynamik(p)
+    lines = ["#Format: r Fkap Ftotal", ""]
+    t = 0.001
+    while t < 5.0:
+        origin = git[0][0] * B + rel[0][0]
+        y = math.fmod(origin + t, 64.0)
+        git[1][1] = int(y / B)
+        rel[1][1] = y - git[1][1] * B
+        s.set_positions(git, rel)
+        fk = s.berechne_kapkraefte()
+        fw = s.berechne_WCA()
+        lines.append("%g \t %g \t %g " % (t, fk[1][1], fk[1][1] + fw[1][1]))
+        t += 0.01
+    s.close()
+    got = (tmp_path / "kapkraft.txt").read_text().split("\n")
+    assert got[:-1] == lines
+    assert len(lines) == 2 + 500
+
+
+def test_driver_mrd_sweep(tmp_path):
+    """debugmode (main.cpp:156-216): for mrd = 1, 0.95, ... > 0.01 the mean
+    and largest |F_WCA|max and the mean dt over 50 steps early and late; dt
+    stays within dt_max and shrinks as the travel cap does."""
+    print("case: C++ driver, debugmode", flush=True)
+    assert os.path.exists(EXE), "bdk_sim is not built (python bd-simulations_amd/build.py)"
+    env = dict(os.environ, BDK_SEED=str(SEED), BDK_MODE="debug")
+    out = subprocess.run([EXE], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    mrds = []
+    m = 1.0
+    while m > 0.01:
+        mrds.append(m)
+        m *= 0.95
+    for name in ("mrd_data1.txt", "mrd_data2.txt"):
+        rows = [[float(v) for v in ln.split()] for ln in (tmp_path / name).read_text().split("\n")
+                if ln.strip() and not ln.startswith("#")]
+        assert len(rows) == len(mrds)
+        for (mrd, f_mean, f_high, dt), want in zip(rows, mrds):
+            assert abs(mrd - want) <= 1e-5 * want
+            assert 0.0 < dt <= 1e-4 * (1 + 1e-5)
+            assert 0.0 < f_mean <= f_high * (1 + 1e-5)
+        assert rows[-1][3] < rows[0][3]  # a small cap shortens the steps
