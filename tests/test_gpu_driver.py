@@ -93,7 +93,11 @@ def test_driver_force_scan_equals_binding(tmp_path):
 def test_driver_mrd_sweep(tmp_path):
     """debugmode (main.cpp:156-216): for mrd = 1, 0.95, ... > 0.01 the mean
     and largest |F_WCA|max and the mean dt over 50 steps early and late; dt
-    stays within dt_max and shrinks as the travel cap does."""
+    stays within dt_max and shrinks as the travel cap does.  The sweep's
+    fixed-dt steps from overlapping starts (forces near maxPairWCA) move
+    particles by many cells; it stopped with a deposit-window bound error
+    while the rebuild-side bound ignored the extra cell of a negative stored
+    offset (fmod in the re-binning, dm:216-325) and runs through since."""
     print("case: C++ driver, debugmode", flush=True)
     assert os.path.exists(EXE), "bdk_sim is not built (python bd-simulations_amd/build.py)"
     env = dict(os.environ, BDK_SEED=str(SEED), BDK_MODE="debug")

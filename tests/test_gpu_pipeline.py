@@ -4,8 +4,8 @@ halo widened by the update's displacement bound) equals bit for bit the mesh
 that the list-order deposit computes for the same state, and so do the mesh
 forces.  Cases: FP32 (stencil window) and FP64 (bounds-checked path), a box
 small enough that a tile stages whole axes, and hot runs whose particles move
-more than a cell per step (bound beyond the window's capacity, and bounds
-within it that must cover offsets left negative by the re-binning)."""
+more than a cell per step (bound beyond the window's capacity, or within
+it)."""
 import numpy as np
 import pytest
 
@@ -22,9 +22,9 @@ CASES = [
     (800, 800, 60.0, 64, bdk.NGP, bdk.FP32, 1.0, 1e-4, 20),
     (1500, 1000, 70.0, 128, bdk.TSC, bdk.FP32, 400.0, 2e-3, 10),
     (1500, 1000, 70.0, 128, bdk.CIC, bdk.FP64, 400.0, 2e-3, 10),
-    # moves of about a cell: the window stays in use while many particles
-    # cross more than a cell downwards, whose stored offset stays negative
-    # (fmod keeps the sign, dm:216-325) and lies a cell below the move
+    # moves of about a cell: the window stays in use while particles cross
+    # more than a cell downwards (their stored offset stays negative, fmod
+    # keeps the sign, dm:216-325)
     (1500, 1000, 70.0, 128, bdk.TSC, bdk.FP32, 100.0, 2e-3, 10),
     (1500, 1000, 70.0, 128, bdk.CIC, bdk.FP64, 100.0, 2e-3, 10),
 ]
