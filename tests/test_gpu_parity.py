@@ -372,3 +372,17 @@ def test_two_particles():
     gg, gr = g.positions()
     og, orl = o.all_positions()
     assert np.array_equal(gg, og) and np.abs(gr - orl).max() < 1e-10
+
+
+@pytest.mark.parametrize("prec", [bdk.FP64, bdk.FP32])
+def test_largest_mesh_forces(prec):
+    """The largest mesh the boundary accepts (Z = 4096, h = 0.049): capillary
+    and WCA forces against the oracle, TSC."""
+    say("case: Z=4096 prec=%d" % prec)
+    g, o, git, rel = pair(n1=1500, n2=1500, L=200.0, Z=4096, scheme=bdk.TSC, lam=20.0, seed=91, precision=prec)
+    tol = 1e-12 if prec == bdk.FP64 else 1e-5
+    assert rel_rms(g.berechne_kapkraefte(), o.kap()) < tol
+    assert rel_rms(g.berechne_WCA(), o.wca()) < tol
+    g.run_steps(3)
+    gg, gr = g.positions()
+    assert np.all(np.isfinite(gr))
