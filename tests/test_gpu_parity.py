@@ -386,3 +386,54 @@ def test_largest_mesh_forces(prec):
     g.run_steps(3)
     gg, gr = g.positions()
     assert np.all(np.isfinite(gr))
+
+
+def test_restrict_radial_trajectory():
+    """restrictRadial (dm:138-155): species 2 kept on the fixed angle
+    2 pi i / N2 around the box centre at its own distance, against the oracle
+    over 30 noisy FP64 steps.  Species 2 starts on a ring of radius 15 at
+    those angles (spacing 3 sigma) and species 1 away from it, so that no
+    projection lands on another particle (a landing inside the WCA core
+    would make the run chaotic and its rounding-level differences grow)."""
+    say("case: restrictRadial")
+    from helpers import rsa_positions
+    L, n2 = 40.0, 30
+    bp, _ = params(n1=300, n2=n2, L=L, restrict_radial=1)
+    der = bdk.derived(bp)
+    ang = 2 * np.pi * np.arange(n2) / n2
+    ring = np.stack([0.5 * L + 15.0 * np.cos(ang), 0.5 * L + 15.0 * np.sin(ang)], 1)
+    x1 = rsa_positions(600, L, 0.9, 95)
+    d = np.sqrt(((x1[:, None, :] - ring[None, :, :]) ** 2).sum(2)).min(1)
+    x1 = x1[d > 2.0][:300]
+    x = np.concatenate([x1, ring])
+    git, rel = to_git_rel(x, der["B"], der["nc"])
+    g, o, _, _ = pair(git=git, rel=rel, n1=len(x1), n2=n2, L=L, restrict_radial=1, seed=95)
+    for _ in range(30):
+        o.step(1.0)
+    g.run_steps(30)
+    gg, gr = g.positions()
+    og, orl = o.all_positions()
+    assert np.array_equal(gg, og)
+    assert np.abs(gr - orl).max() < 1e-10
+    xs = gg[len(x1):] * g.B + gr[len(x1):] - 0.5 * L
+    assert np.abs(np.arctan2(xs[:, 1], xs[:, 0]) % (2 * np.pi) - ang).max() < 1e-9
+
+
+@pytest.mark.parametrize("switch", [dict(no_wca=1), dict(idgas_only=1), dict(max_pair_wca=50.0)])
+def test_parameter_switches_trajectory(switch):
+    """parameter.h switches against the oracle over 30 noisy FP64 steps:
+    noWCA (which switches the noise off, dm:90-101), idgas_only (no pair or
+    mesh forces), and a maxPairWCA cap that binds (P6: |F|/r capped, pairs
+    as close as 0.8 sigma)."""
+    say("case: switch %s" % switch)
+    dmin = 0.8 if "max_pair_wca" in switch else 0.9
+    g, o, git, rel = pair(n1=400, n2=300, L=40.0, Z=64, scheme=bdk.CIC, seed=95, dmin=dmin, **switch)
+    if "max_pair_wca" in switch:
+        assert rel_rms(g.berechne_WCA(), o.wca()) < 1e-12
+    for _ in range(30):
+        o.step(1.0)
+    g.run_steps(30)
+    gg, gr = g.positions()
+    og, orl = o.all_positions()
+    assert np.array_equal(gg, og)
+    assert np.abs(gr - orl).max() < 1e-10
