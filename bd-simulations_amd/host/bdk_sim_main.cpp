@@ -112,6 +112,7 @@ int main() {
             theRun.obs_point(obs_nr);
             last_write = std::time(nullptr);
         }
+        theRun.schreibe_obs();  // main.cpp:296
     }
     std::cout << "done after " << std::time(nullptr) - started << " s" << std::endl;
     return 0;

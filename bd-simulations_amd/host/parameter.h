@@ -40,6 +40,11 @@ const double obs_dt = 0.1;
 const int obs_anzahl = 11;
 const int startpos_methode = 1;
 const bool auswerten_animation = true;
+const bool auswerten_korrfunk = false;        // g11, g22 at every checkpoint
+const bool auswerten_korrfunk_mixed = false;  // and g12
+const double korr_dr = 0.2;
+const double korr_rmax = L * 0.5;
+const int korr_bins = (int)(korr_rmax / korr_dr) + 1;
 
 const double zweihoch1_6 = pow(2.0, 1.0 / 6.0);
 const int nachList_Zellen = (int)floor(L / zweihoch1_6);

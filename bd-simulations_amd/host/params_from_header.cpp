@@ -4,7 +4,8 @@
 // Environment overrides: BDK_SEED (MT seed; the reference uses time(NULL)),
 // BDK_NOISE=mt|disc (host MT stream uploaded per step, or device Philox
 // disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz),
-// BDK_MODE=run|debug|krafttest (sets debugmode / krafttestmode).
+// BDK_MODE=run|debug|krafttest (sets debugmode / krafttestmode),
+// BDK_KORRFUNK=0|1|mixed (auswerten_korrfunk, auswerten_korrfunk_mixed).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -43,6 +44,14 @@ RunConfig config_from_parameter_h() {
     c.auswerten_animation = auswerten_animation;
     c.krafttestmode = krafttestmode;
     c.debugmode = debugmode;
+    c.korrfunk = auswerten_korrfunk;
+    c.korrfunk_mixed = auswerten_korrfunk_mixed;
+    c.korr_bins = korr_bins;
+    c.korr_dr = korr_dr;
+    if (const char *k = std::getenv("BDK_KORRFUNK")) {  // 1: g11, g22; mixed: also g12
+        c.korrfunk = std::strcmp(k, "0") != 0;
+        c.korrfunk_mixed = std::strcmp(k, "mixed") == 0;
+    }
     if (const char *m = std::getenv("BDK_MODE")) {
         c.debugmode = std::strcmp(m, "debug") == 0;
         c.krafttestmode = std::strcmp(m, "krafttest") == 0;

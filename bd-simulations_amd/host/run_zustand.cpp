@@ -228,6 +228,7 @@ void RunZustand::zeitschritte_bis_obs(time_t last_write) {
 
 void RunZustand::obs_point(int nr_obspoint) {
     dyn.sync_positions();
+    if (cfg_.korrfunk) obs.record_korrelationsfunktion(dyn.h, nr_obspoint);
     if (cfg_.auswerten_animation) obs.schreibe_pos(nr_obspoint);
     schritte_seit_obs = 0;
     t = 0.0L;
@@ -272,6 +273,42 @@ void RunZustand::RunObs::schreibe_pos(int nr) {
         std::fprintf(f, "\n");
         std::fclose(f);
     }
+}
+
+void RunZustand::RunObs::record_korrelationsfunktion(bdk_t *h, int nr) {
+    const int nb = cfg->korr_bins;
+    if ((int)g11.size() < cfg->obs_anzahl) {
+        g11.assign(cfg->obs_anzahl, std::vector<double>(nb, 0.0));
+        g12.assign(cfg->obs_anzahl, std::vector<double>(nb, 0.0));
+        g22.assign(cfg->obs_anzahl, std::vector<double>(nb, 0.0));
+    }
+    if (nr < 0 || nr >= cfg->obs_anzahl) return;
+    check(h, bdk_pair_correlation(h, nb, cfg->korr_dr, g11[nr].data(), g12[nr].data(), g22[nr].data()),
+          "pair_correlation");
+}
+
+void RunZustand::RunObs::schreibe_korrfunk(int run) const {
+    const std::string name = "g_run" + std::to_string(run) + ".txt";
+    FILE *f = std::fopen(name.c_str(), "w");
+    if (!f) return;
+    const bool mixed = cfg->korrfunk_mixed;
+    std::fprintf(f, mixed ? "# Format: t TAB r TAB g11 TAB g12 TAB g22\n\n" : "# Format: t TAB r TAB g11 TAB g22\n\n");
+    for (int o = 0; o < (int)g11.size(); ++o) {
+        const double t = o * cfg->obs_dt;
+        for (int b = 0; b < cfg->korr_bins; ++b) {
+            const double x = b * cfg->korr_dr;
+            if (mixed)
+                std::fprintf(f, "%g \t %g \t %g \t %g \t %g \n", t, x, g11[o][b], g12[o][b], g22[o][b]);
+            else
+                std::fprintf(f, "%g \t %g \t %g \t %g \n", t, x, g11[o][b], g22[o][b]);
+        }
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+}
+
+void RunZustand::schreibe_obs() {
+    if (cfg_.korrfunk) obs.schreibe_korrfunk(nr);
 }
 
 void RunZustand::RunObs::close() {}

@@ -29,6 +29,9 @@ struct RunConfig {
     bool auswerten_animation;
     bool krafttestmode;
     bool debugmode;
+    bool korrfunk, korrfunk_mixed;  // auswerten_korrfunk(_mixed)
+    int korr_bins;
+    double korr_dr;
     uint64_t seed;      // MT seed (the reference seeds with time(NULL), zufall.cpp:18)
     int noise_source;   // 0: device Philox disc noise; 1: host MT stream uploaded per step
 };
@@ -102,6 +105,11 @@ class RunZustand {
         double **r1_rel = nullptr, **r2_rel = nullptr;
         void obs_init(int run, const RunConfig &cfg);  // au:52-75
         void schreibe_pos(int nr);                     // au:631-648
+        // g11, g12, g22 of checkpoint nr (record_korrelationsfunktion11/12/22,
+        // au:110-188) and their file at the end of the run (au:934-975)
+        void record_korrelationsfunktion(bdk_t *h, int nr);
+        void schreibe_korrfunk(int run) const;
+        std::vector<std::vector<double>> g11, g12, g22;
         void close();
         std::string path1, path2;
         const RunConfig *cfg = nullptr;
@@ -123,6 +131,7 @@ class RunZustand {
     void init(int nr);                            // am:256-320
     void zeitschritte_bis_obs(time_t last_write);  // dm:444-455
     void obs_point(int nr_obspoint);              // au:78-93
+    void schreibe_obs();                          // au:912-928
 
   private:
     const RunConfig &cfg_;

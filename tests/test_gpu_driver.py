@@ -117,3 +117,31 @@ def test_driver_mrd_sweep(tmp_path):
             assert 0.0 < dt <= 1e-4 * (1 + 1e-5)
             assert 0.0 < f_mean <= f_high * (1 + 1e-5)
         assert rows[-1][3] < rows[0][3]  # a small cap shortens the steps
+
+
+def test_driver_pair_correlation_file(tmp_path):
+    """auswerten_korrfunk(_mixed) (BDK_KORRFUNK=mixed): obs_point records
+    g11, g12, g22 at every checkpoint (au:81-83, au:110-188) and the run ends
+    with g_run0.txt (au:953-975); equal to the same records through the
+    binding."""
+    print("case: C++ driver, g(r) file", flush=True)
+    assert os.path.exists(EXE), "bdk_sim is not built (python bd-simulations_amd/build.py)"
+    env = dict(os.environ, BDK_SEED=str(SEED), BDK_KORRFUNK="mixed")
+    out = subprocess.run([EXE], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    p = bdk.make_params(n1=1000, n2=1000, L=64.0, Z=128, scheme=bdk.CIC, lam=8.0, kap=1.0, T=1.0, dt_max=1e-4,
+                        f2_f1=0.5, precision=bdk.FP32, noise_mode=bdk.NOISE_DISC, seed=SEED)
+    git, rel = bdk.MT(SEED).init_positions(p)
+    s = bdk.RunDynamik(p)
+    s.set_positions(git, rel)
+    nb, dr = int(32.0 / 0.2) + 1, 0.2
+    lines = ["# Format: t TAB r TAB g11 TAB g12 TAB g22", ""]
+    for o in range(11):
+        if o:
+            s.zeitschritte_bis_obs(0.1)
+        g11, g12, g22 = s.pair_correlation(nb, dr)
+        lines += ["%g \t %g \t %g \t %g \t %g " % (o * 0.1, b * dr, g11[b], g12[b], g22[b]) for b in range(nb)]
+        lines.append("")
+    s.close()
+    got = (tmp_path / "g_run0.txt").read_text().split("\n")
+    assert got[:-1] == lines
