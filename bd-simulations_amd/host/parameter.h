@@ -45,6 +45,7 @@ const bool auswerten_korrfunk_mixed = false;  // and g12
 const double korr_dr = 0.2;
 const double korr_rmax = L * 0.5;
 const int korr_bins = (int)(korr_rmax / korr_dr) + 1;
+const bool auswerten_rhoviaFFTW = false;  // rho(k) shells at every checkpoint
 
 const double zweihoch1_6 = pow(2.0, 1.0 / 6.0);
 const int nachList_Zellen = (int)floor(L / zweihoch1_6);
@@ -52,3 +53,6 @@ const double nachList_Breite = L / ((double)nachList_Zellen);
 const double densGrid_Breite = L / FFT_samples;
 const int densGrid_Zellen = FFT_samples;
 const double dq = 2.0 * M_PI / L;
+const double dq_rhoFFTW = 2.0 * dq;
+const double dq_dqFT = dq / dq_rhoFFTW;
+const int rhoFFTW_bins = 0.5 * FFT_samples * dq / dq_rhoFFTW;

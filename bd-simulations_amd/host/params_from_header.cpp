@@ -5,7 +5,8 @@
 // BDK_NOISE=mt|disc (host MT stream uploaded per step, or device Philox
 // disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz),
 // BDK_MODE=run|debug|krafttest (sets debugmode / krafttestmode),
-// BDK_KORRFUNK=0|1|mixed (auswerten_korrfunk, auswerten_korrfunk_mixed).
+// BDK_KORRFUNK=0|1|mixed (auswerten_korrfunk, auswerten_korrfunk_mixed),
+// BDK_RHOFFTW=0|1 (auswerten_rhoviaFFTW).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -48,6 +49,11 @@ RunConfig config_from_parameter_h() {
     c.korrfunk_mixed = auswerten_korrfunk_mixed;
     c.korr_bins = korr_bins;
     c.korr_dr = korr_dr;
+    c.rhofftw = auswerten_rhoviaFFTW;
+    c.rhofftw_bins = rhoFFTW_bins;
+    c.dq_rhofftw = dq_rhoFFTW;
+    c.dq_dqft = dq_dqFT;
+    if (const char *r = std::getenv("BDK_RHOFFTW")) c.rhofftw = std::strcmp(r, "0") != 0;
     if (const char *k = std::getenv("BDK_KORRFUNK")) {  // 1: g11, g22; mixed: also g12
         c.korrfunk = std::strcmp(k, "0") != 0;
         c.korrfunk_mixed = std::strcmp(k, "mixed") == 0;

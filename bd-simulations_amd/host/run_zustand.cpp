@@ -229,6 +229,7 @@ void RunZustand::zeitschritte_bis_obs(time_t last_write) {
 void RunZustand::obs_point(int nr_obspoint) {
     dyn.sync_positions();
     if (cfg_.korrfunk) obs.record_korrelationsfunktion(dyn.h, nr_obspoint);
+    if (cfg_.rhofftw) obs.record_rhoFFTW(dyn.h, nr_obspoint);
     if (cfg_.auswerten_animation) obs.schreibe_pos(nr_obspoint);
     schritte_seit_obs = 0;
     t = 0.0L;
@@ -307,8 +308,41 @@ void RunZustand::RunObs::schreibe_korrfunk(int run) const {
     std::fclose(f);
 }
 
+void RunZustand::RunObs::record_rhoFFTW(bdk_t *h, int nr) {
+    const int nb = cfg->rhofftw_bins;
+    if ((int)rho1_re.size() < cfg->obs_anzahl) {
+        for (auto *v : {&rho1_re, &rho1_im, &rho2_re, &rho2_im}) v->assign(cfg->obs_anzahl, std::vector<double>(nb, 0.0));
+    }
+    if (nr < 0 || nr >= cfg->obs_anzahl || nb < 1) return;
+    check(h, bdk_rho_k(h, 1, cfg->dq_dqft, nb, rho1_re[nr].data(), rho1_im[nr].data()), "rho_k");
+    if (cfg->p.n2) check(h, bdk_rho_k(h, 2, cfg->dq_dqft, nb, rho2_re[nr].data(), rho2_im[nr].data()), "rho_k");
+}
+
+void RunZustand::RunObs::schreibe_rhoFFT(int run) const {
+    const std::string name = "rhoFFT_run" + std::to_string(run) + ".txt";
+    FILE *f = std::fopen(name.c_str(), "w");
+    if (!f) return;
+    const bool one = cfg->p.n2 == 0;
+    std::fprintf(f, one ? "# Format: t TAB q TAB rho_re TAB rho_im \n\n"
+                        : "# Format: t TAB q TAB rho1_re TAB rho1_im TAB rho2_re TAB rho2_im \n\n");
+    for (int o = 0; o < (int)rho1_re.size(); ++o) {
+        const double t = o * cfg->obs_dt;
+        for (int i = 0; i < cfg->rhofftw_bins; ++i) {
+            const double q = cfg->dq_rhofftw * i;
+            if (one)
+                std::fprintf(f, "%g \t %g \t %g \t %g \n", t, q, rho1_re[o][i], rho1_im[o][i]);
+            else
+                std::fprintf(f, "%g \t %g \t %g \t %g \t %g \t %g \n", t, q, rho1_re[o][i], rho1_im[o][i],
+                             rho2_re[o][i], rho2_im[o][i]);
+        }
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+}
+
 void RunZustand::schreibe_obs() {
     if (cfg_.korrfunk) obs.schreibe_korrfunk(nr);
+    if (cfg_.rhofftw) obs.schreibe_rhoFFT(nr);
 }
 
 void RunZustand::RunObs::close() {}

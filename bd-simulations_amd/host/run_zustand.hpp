@@ -32,6 +32,9 @@ struct RunConfig {
     bool korrfunk, korrfunk_mixed;  // auswerten_korrfunk(_mixed)
     int korr_bins;
     double korr_dr;
+    bool rhofftw;             // auswerten_rhoviaFFTW
+    int rhofftw_bins;         // rhoFFTW_bins
+    double dq_rhofftw, dq_dqft;
     uint64_t seed;      // MT seed (the reference seeds with time(NULL), zufall.cpp:18)
     int noise_source;   // 0: device Philox disc noise; 1: host MT stream uploaded per step
 };
@@ -110,6 +113,11 @@ class RunZustand {
         void record_korrelationsfunktion(bdk_t *h, int nr);
         void schreibe_korrfunk(int run) const;
         std::vector<std::vector<double>> g11, g12, g22;
+        // rho(k) shells of each species at checkpoint nr (record_rhoFFTW,
+        // au:527-626) and their file at the end of the run (au:994-1014)
+        void record_rhoFFTW(bdk_t *h, int nr);
+        void schreibe_rhoFFT(int run) const;
+        std::vector<std::vector<double>> rho1_re, rho1_im, rho2_re, rho2_im;
         void close();
         std::string path1, path2;
         const RunConfig *cfg = nullptr;

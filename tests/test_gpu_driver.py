@@ -145,3 +145,37 @@ def test_driver_pair_correlation_file(tmp_path):
     s.close()
     got = (tmp_path / "g_run0.txt").read_text().split("\n")
     assert got[:-1] == lines
+
+
+def test_driver_rho_k_file(tmp_path):
+    """auswerten_rhoviaFFTW (BDK_RHOFFTW=1): obs_point records the rho(k)
+    shells of both species (record_rhoFFTW, au:527-626) with the reference's
+    dq_rhoFFTW = 2 dq and rhoFFTW_bins = Z/4, and the run ends with
+    rhoFFT_run0.txt (au:994-1014); equal to the same records through the
+    binding."""
+    import math
+    print("case: C++ driver, rho(k) file", flush=True)
+    assert os.path.exists(EXE), "bdk_sim is not built (python bd-simulations_amd/build.py)"
+    env = dict(os.environ, BDK_SEED=str(SEED), BDK_RHOFFTW="1")
+    out = subprocess.run([EXE], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    p = bdk.make_params(n1=1000, n2=1000, L=64.0, Z=128, scheme=bdk.CIC, lam=8.0, kap=1.0, T=1.0, dt_max=1e-4,
+                        f2_f1=0.5, precision=bdk.FP32, noise_mode=bdk.NOISE_DISC, seed=SEED)
+    git, rel = bdk.MT(SEED).init_positions(p)
+    s = bdk.RunDynamik(p)
+    s.set_positions(git, rel)
+    dq = 2.0 * math.pi / 64.0
+    dq_r = 2.0 * dq
+    nb = int(0.5 * 128 * dq / dq_r)
+    lines = ["# Format: t TAB q TAB rho1_re TAB rho1_im TAB rho2_re TAB rho2_im ", ""]
+    for o in range(11):
+        if o:
+            s.zeitschritte_bis_obs(0.1)
+        r1, i1 = s.rho_k(1, dq / dq_r, nb)
+        r2, i2 = s.rho_k(2, dq / dq_r, nb)
+        lines += ["%g \t %g \t %g \t %g \t %g \t %g " % (o * 0.1, dq_r * i, r1[i], i1[i], r2[i], i2[i])
+                  for i in range(nb)]
+        lines.append("")
+    s.close()
+    got = (tmp_path / "rhoFFT_run0.txt").read_text().split("\n")
+    assert got[:-1] == lines
