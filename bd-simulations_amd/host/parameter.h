@@ -46,6 +46,7 @@ const double korr_dr = 0.2;
 const double korr_rmax = L * 0.5;
 const int korr_bins = (int)(korr_rmax / korr_dr) + 1;
 const bool auswerten_rhoviaFFTW = false;  // rho(k) shells at every checkpoint
+const bool auswerten_abstand = false;     // mean distance of species 2 from the box centre
 
 const double zweihoch1_6 = pow(2.0, 1.0 / 6.0);
 const int nachList_Zellen = (int)floor(L / zweihoch1_6);

@@ -6,7 +6,7 @@
 // disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz),
 // BDK_MODE=run|debug|krafttest (sets debugmode / krafttestmode),
 // BDK_KORRFUNK=0|1|mixed (auswerten_korrfunk, auswerten_korrfunk_mixed),
-// BDK_RHOFFTW=0|1 (auswerten_rhoviaFFTW).
+// BDK_RHOFFTW=0|1 (auswerten_rhoviaFFTW), BDK_ABSTAND=0|1 (auswerten_abstand).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -54,6 +54,8 @@ RunConfig config_from_parameter_h() {
     c.dq_rhofftw = dq_rhoFFTW;
     c.dq_dqft = dq_dqFT;
     if (const char *r = std::getenv("BDK_RHOFFTW")) c.rhofftw = std::strcmp(r, "0") != 0;
+    c.abstand = auswerten_abstand;
+    if (const char *a = std::getenv("BDK_ABSTAND")) c.abstand = std::strcmp(a, "0") != 0;
     if (const char *k = std::getenv("BDK_KORRFUNK")) {  // 1: g11, g22; mixed: also g12
         c.korrfunk = std::strcmp(k, "0") != 0;
         c.korrfunk_mixed = std::strcmp(k, "mixed") == 0;

@@ -231,6 +231,7 @@ void RunZustand::obs_point(int nr_obspoint) {
     if (cfg_.korrfunk) obs.record_korrelationsfunktion(dyn.h, nr_obspoint);
     if (cfg_.rhofftw) obs.record_rhoFFTW(dyn.h, nr_obspoint);
     if (cfg_.auswerten_animation) obs.schreibe_pos(nr_obspoint);
+    if (cfg_.abstand) obs.schreibe_abstaende(nr_obspoint);
     schritte_seit_obs = 0;
     t = 0.0L;
     obs_nr++;
@@ -241,6 +242,11 @@ void RunZustand::RunObs::obs_init(int run, const RunConfig &c) {
     double d[6];
     bdk_params_derived(&c.p, d);
     B = d[2];
+    if (c.abstand) {  // au:71-74, a new file every run
+        if (ab2) std::fclose(ab2);
+        ab2 = std::fopen("abstaende2.txt", "w");
+        if (ab2) std::fprintf(ab2, "# Format: t TAB mean distance of the species-2 particles from the box centre \n\n");
+    }
     if (!c.auswerten_animation) return;
     path1 = "pos1_" + std::to_string(run + 1) + ".txt";  // au:58-59
     path2 = c.p.n2 ? "pos2_" + std::to_string(run + 1) + ".txt" : "";
@@ -345,4 +351,21 @@ void RunZustand::schreibe_obs() {
     if (cfg_.rhofftw) obs.schreibe_rhoFFT(nr);
 }
 
-void RunZustand::RunObs::close() {}
+void RunZustand::RunObs::schreibe_abstaende(int nr) {
+    if (!ab2) return;
+    const double mitte = 0.5 * cfg->p.L;
+    double mean = 0.0;
+    for (int i = 0; i < cfg->p.n2; ++i) {
+        const double dx = r2_rel[i][0] + B * r2_git[i][0] - mitte;
+        const double dy = r2_rel[i][1] + B * r2_git[i][1] - mitte;
+        mean += std::sqrt(dx * dx + dy * dy);
+    }
+    mean /= cfg->p.n2;
+    std::fprintf(ab2, "%g \t %g \n", nr * cfg->obs_dt, mean);
+    std::fflush(ab2);
+}
+
+void RunZustand::RunObs::close() {
+    if (ab2) std::fclose(ab2);
+    ab2 = nullptr;
+}

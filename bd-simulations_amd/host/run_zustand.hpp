@@ -33,6 +33,7 @@ struct RunConfig {
     int korr_bins;
     double korr_dr;
     bool rhofftw;             // auswerten_rhoviaFFTW
+    bool abstand;             // auswerten_abstand
     int rhofftw_bins;         // rhoFFTW_bins
     double dq_rhofftw, dq_dqft;
     uint64_t seed;      // MT seed (the reference seeds with time(NULL), zufall.cpp:18)
@@ -117,6 +118,10 @@ class RunZustand {
         // au:527-626) and their file at the end of the run (au:994-1014)
         void record_rhoFFTW(bdk_t *h, int nr);
         void schreibe_rhoFFT(int run) const;
+        // mean distance of the species-2 particles from the box centre
+        // (schreibe_abstaende, au:651-670) into abstaende2.txt
+        void schreibe_abstaende(int nr);
+        FILE *ab2 = nullptr;
         std::vector<std::vector<double>> rho1_re, rho1_im, rho2_re, rho2_im;
         void close();
         std::string path1, path2;

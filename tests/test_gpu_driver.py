@@ -179,3 +179,33 @@ def test_driver_rho_k_file(tmp_path):
     s.close()
     got = (tmp_path / "rhoFFT_run0.txt").read_text().split("\n")
     assert got[:-1] == lines
+
+
+def test_driver_abstand_file(tmp_path):
+    """auswerten_abstand (BDK_ABSTAND=1): the mean distance of the species-2
+    particles from the box centre at every checkpoint (schreibe_abstaende,
+    au:651-670), equal to the same means of the binding's positions."""
+    import math
+    print("case: C++ driver, abstaende2.txt", flush=True)
+    env = dict(os.environ, BDK_SEED=str(SEED), BDK_ABSTAND="1")
+    out = subprocess.run([EXE], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    p = bdk.make_params(n1=1000, n2=1000, L=64.0, Z=128, scheme=bdk.CIC, lam=8.0, kap=1.0, T=1.0, dt_max=1e-4,
+                        f2_f1=0.5, precision=bdk.FP32, noise_mode=bdk.NOISE_DISC, seed=SEED)
+    git, rel = bdk.MT(SEED).init_positions(p)
+    s = bdk.RunDynamik(p)
+    s.set_positions(git, rel)
+    lines = []
+    for o in range(11):
+        if o:
+            s.zeitschritte_bis_obs(0.1)
+        g, r = s.positions()
+        mean = 0.0
+        for i in range(1000, 2000):  # the reference's loop order and arithmetic
+            dx = r[i][0] + s.B * g[i][0] - 32.0
+            dy = r[i][1] + s.B * g[i][1] - 32.0
+            mean += math.sqrt(dx * dx + dy * dy)
+        lines.append("%g \t %g " % (o * 0.1, mean / 1000))
+    s.close()
+    got = [ln for ln in (tmp_path / "abstaende2.txt").read_text().split("\n") if ln and not ln.startswith("#")]
+    assert got == lines
