@@ -103,6 +103,7 @@ def lib():
             "bdk_write_pos": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
             "bdk_write_pos_async": (C.c_int, [vp, C.c_char_p, C.c_char_p, i32, dbl]),
             "bdk_density_grid": (C.c_int, [vp, i32, vp, vp]),
+            "bdk_ftrho_sums": (C.c_int, [vp, i32, i32, dbl, vp, vp, vp]),
             "bdk_density_stats": (C.c_int, [vp, i32, i32, dbl, vp, vp, vp, vp]),
             "bdk_write_wait": (C.c_int, [vp]),
             "bdk_step_info": (C.c_int, [vp, dbl, vp]),
@@ -145,7 +146,7 @@ def exported_symbols():
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos", "bdk_write_pos_async", "bdk_write_wait",
             "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
-            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_density_grid", "bdk_density_stats", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
+            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_density_grid", "bdk_density_stats", "bdk_ftrho_sums", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
@@ -435,6 +436,13 @@ class RunDynamik:
         prof = np.zeros((6, nbins))
         self._chk(self.L.bdk_radial_profile(self.h, nbins, dr, _p(cen), _p(prof)))
         return cen, prof
+
+    def ftrho_sums(self, species, qbins, dq_ft):
+        """(cos sums, sin sums) over the reference's (i, j) pairs (au:307-360)."""
+        re, im = np.zeros(qbins * qbins), np.zeros(qbins * qbins)
+        n = C.c_int32(0)
+        self._chk(self.L.bdk_ftrho_sums(self.h, species, qbins, dq_ft, _p(re), _p(im), C.byref(n)))
+        return re[: n.value], im[: n.value]
 
     def density_grid(self, m):
         """rho1, rho2 [m, m] of the snapshot grid (main_auswertung.cpp:214-320)."""

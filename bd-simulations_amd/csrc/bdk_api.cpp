@@ -1817,6 +1817,39 @@ int bdk_density_stats(bdk_t *h, int32_t m, int32_t nbins, double dbin, const dou
                             "bdk_density_stats");
 }
 
+int bdk_ftrho_sums(bdk_t *h, int32_t species, int32_t qbins, double dq_ft, double *re, double *im, int32_t *npairs) {
+    int rc = ensure(h);
+    if (rc) return rc;
+    if ((species != 1 && species != 2) || qbins < 1 || qbins > 256 || !re || !im || !npairs)
+        return fail(h, BDK_ERR_ARG, "bdk_ftrho_sums: need species 1 or 2, 1 <= qbins <= 256, outputs");
+    // the reference's loop order: the inner loop stops at the first j beyond the quarter circle
+    std::vector<int2> ij;
+    for (int i = 0; i < qbins; ++i)
+        for (int j = 0; j < qbins; ++j) {
+            if (i * i + j * j > qbins * qbins) break;
+            ij.push_back(make_int2(i, j));
+        }
+    const int np = (int)ij.size();
+    const size_t nch = (size_t)bdkx::ftrho_chunks(h);
+    int2 *dij = nullptr;
+    double2 *part = nullptr;
+    double *out = nullptr;
+    hipError_t e = hipMalloc((void **)&dij, (size_t)np * sizeof(int2));
+    if (e == hipSuccess) e = hipMalloc((void **)&part, (size_t)np * nch * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void **)&out, 2 * (size_t)np * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(dij, ij.data(), (size_t)np * sizeof(int2), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = bdkx::launch_ftrho(h, dij, np, species, dq_ft, part, out, out + np);
+    if (e == hipSuccess) e = hipMemcpyAsync(re, out, (size_t)np * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(im, out + np, (size_t)np * 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (dij) (void)hipFree(dij);
+    if (part) (void)hipFree(part);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return hipfail(h, e, "bdk_ftrho_sums");
+    *npairs = np;
+    return BDK_OK;
+}
+
 int bdk_rho_k(bdk_t *h, int32_t species, double dq_ratio, int32_t nbins, double *re, double *im) {
     int rc = ensure(h);
     if (rc) return rc;

@@ -221,6 +221,11 @@ struct SnapThresh {
 };
 hipError_t launch_snap(bdk_t *h, int m, unsigned long long *acc, double *rho, int stats, int nbins, double dbin,
                        SnapThresh th, unsigned int *cnt);
+// direct rho(k) sums (au:307-360) of one species for npairs index pairs ij;
+// partial [npairs][ftrho_chunks(h)]
+hipError_t launch_ftrho(bdk_t *h, const int2 *ij, int npairs, int species, double dq, double2 *partial, double *re,
+                        double *im);
+int ftrho_chunks(const bdk_t *h);
 hipError_t set_kernel_attributes();
 hipError_t build_tables(bdk_t *h);                // spectral tables in device memory
 

@@ -2497,6 +2497,74 @@ hipError_t launch_snap(bdk_t *h, int m, unsigned long long *acc, double *rho, in
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ direct rho(k)
+// record_ftrho_unkorrigiert (au:307-360): for each wave-vector index pair
+// (i, j) the sums over one species of cos and sin(dq (x i + y j)).  For
+// species 1 the reference takes y = git_y B + rel_x (au:333); species 2 uses
+// rel_y.  Block (chunk, pair): a fixed-order tree sum over the chunk's
+// particles into partial[pair][chunk]; k_ftrho_final adds the chunks in order.
+constexpr int FT_THREADS = 256, FT_PER_THREAD = 16;
+
+__global__ __launch_bounds__(FT_THREADS) void k_ftrho(Geo g, DevBufs d, const int2 *ij, int species, double dq,
+                                                      double2 *partial, int nchunk) {
+    __shared__ double sc[FT_THREADS], ss[FT_THREADS];
+    const int p = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
+    const int2 q = ij[p];
+    double c = 0.0, s = 0.0;
+    const int q0 = chunk * FT_THREADS * FT_PER_THREAD;
+    for (int k = 0; k < FT_PER_THREAD; ++k) {
+        const int t = q0 + k * FT_THREADS + tid;
+        if (t >= g.N) break;
+        const bool s2 = d.lpid[t] >= g.n1;
+        if (s2 != (species == 2)) continue;
+        const int cell = d.lcell[t];
+        const int gx = div_nc(cell, g), gy = cell - gx * g.nc;
+        const double2 r = d.lrel[t];
+        const double x = gx * g.B + r.x;
+        const double y = gy * g.B + (species == 1 ? r.x : r.y);
+        double sn, cs;
+        sincos(dq * (x * q.x + y * q.y), &sn, &cs);
+        c += cs;
+        s += sn;
+    }
+    sc[tid] = c;
+    ss[tid] = s;
+    __syncthreads();
+    for (int w = FT_THREADS / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            sc[tid] += sc[tid + w];
+            ss[tid] += ss[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) partial[(size_t)p * nchunk + chunk] = make_double2(sc[0], ss[0]);
+}
+
+__global__ void k_ftrho_final(const double2 *partial, int nchunk, int npairs, double *re, double *im) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    double c = 0.0, s = 0.0;
+    for (int k = 0; k < nchunk; ++k) {
+        const double2 v = partial[(size_t)p * nchunk + k];
+        c += v.x;
+        s += v.y;
+    }
+    re[p] = c;
+    im[p] = s;
+}
+
+hipError_t launch_ftrho(bdk_t *h, const int2 *ij, int npairs, int species, double dq, double2 *partial, double *re,
+                        double *im) {
+    const int nchunk = (int)nblk(h->g.N, FT_THREADS * FT_PER_THREAD);
+    hipLaunchKernelGGL(k_ftrho, dim3(nchunk, npairs), dim3(FT_THREADS), 0, h->stream, h->g, h->d, ij, species, dq,
+                       partial, nchunk);
+    hipLaunchKernelGGL(k_ftrho_final, dim3(nblk(npairs, 256)), dim3(256), 0, h->stream, partial, nchunk, npairs, re,
+                       im);
+    return hipGetLastError();
+}
+
+int ftrho_chunks(const bdk_t *h) { return (int)nblk(h->g.N, FT_THREADS * FT_PER_THREAD); }
+
 // ------------------------------------------------------------------ step counters
 // TimestepInfo (signaturen2.h:71-145, filled by zeitschritt_debug,
 // dm:168-211): the largest force of each kind by |F|^2 with its particle,

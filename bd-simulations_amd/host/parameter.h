@@ -47,6 +47,8 @@ const double korr_rmax = L * 0.5;
 const int korr_bins = (int)(korr_rmax / korr_dr) + 1;
 const bool auswerten_rhoviaFFTW = false;  // rho(k) shells at every checkpoint
 const bool auswerten_abstand = false;     // mean distance of species 2 from the box centre
+const bool auswerten_rhovonk = false;     // rho(k) by direct sums on a k grid
+const int ftrho_qbins = 30;
 
 const double zweihoch1_6 = pow(2.0, 1.0 / 6.0);
 const int nachList_Zellen = (int)floor(L / zweihoch1_6);
@@ -57,3 +59,4 @@ const double dq = 2.0 * M_PI / L;
 const double dq_rhoFFTW = 2.0 * dq;
 const double dq_dqFT = dq / dq_rhoFFTW;
 const int rhoFFTW_bins = 0.5 * FFT_samples * dq / dq_rhoFFTW;
+const double ftrho_dq = 4.0 * M_PI / L;

@@ -6,7 +6,8 @@
 // disc noise), BDK_PRECISION=fp64|fp32, BDK_MAX_TRAVEL (max_reisedistanz),
 // BDK_MODE=run|debug|krafttest (sets debugmode / krafttestmode),
 // BDK_KORRFUNK=0|1|mixed (auswerten_korrfunk, auswerten_korrfunk_mixed),
-// BDK_RHOFFTW=0|1 (auswerten_rhoviaFFTW), BDK_ABSTAND=0|1 (auswerten_abstand).
+// BDK_RHOFFTW=0|1 (auswerten_rhoviaFFTW), BDK_ABSTAND=0|1 (auswerten_abstand),
+// BDK_RHOVONK=0|1 (auswerten_rhovonk).
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -55,6 +56,11 @@ RunConfig config_from_parameter_h() {
     c.dq_dqft = dq_dqFT;
     if (const char *r = std::getenv("BDK_RHOFFTW")) c.rhofftw = std::strcmp(r, "0") != 0;
     c.abstand = auswerten_abstand;
+    c.rhovonk = auswerten_rhovonk;
+    c.ftrho_qbins = ftrho_qbins;
+    c.ftrho_dq = ftrho_dq;
+    c.dq = dq;
+    if (const char *v = std::getenv("BDK_RHOVONK")) c.rhovonk = std::strcmp(v, "0") != 0;
     if (const char *a = std::getenv("BDK_ABSTAND")) c.abstand = std::strcmp(a, "0") != 0;
     if (const char *k = std::getenv("BDK_KORRFUNK")) {  // 1: g11, g22; mixed: also g12
         c.korrfunk = std::strcmp(k, "0") != 0;

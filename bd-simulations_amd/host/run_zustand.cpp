@@ -229,6 +229,7 @@ void RunZustand::zeitschritte_bis_obs(time_t last_write) {
 void RunZustand::obs_point(int nr_obspoint) {
     dyn.sync_positions();
     if (cfg_.korrfunk) obs.record_korrelationsfunktion(dyn.h, nr_obspoint);
+    if (cfg_.rhovonk) obs.record_ftrho(dyn.h, nr_obspoint);
     if (cfg_.rhofftw) obs.record_rhoFFTW(dyn.h, nr_obspoint);
     if (cfg_.auswerten_animation) obs.schreibe_pos(nr_obspoint);
     if (cfg_.abstand) obs.schreibe_abstaende(nr_obspoint);
@@ -346,8 +347,68 @@ void RunZustand::RunObs::schreibe_rhoFFT(int run) const {
     std::fclose(f);
 }
 
+void RunZustand::RunObs::record_ftrho(bdk_t *h, int nr) {
+    const int qb = cfg->ftrho_qbins;
+    if (ftrho_order.empty()) {  // init_ftrho: shells in order of first appearance
+        for (int i = 0; i < qb; ++i)
+            for (int j = 0; j < qb; ++j) {
+                const int ind2 = i * i + j * j;
+                if (ind2 > qb * qb) break;
+                int b = -1;
+                for (int k = 0; k < (int)ftrho_qabs2.size(); ++k)
+                    if (ftrho_qabs2[k] == ind2) {
+                        b = k;
+                        break;
+                    }
+                if (b < 0) {
+                    ftrho_qabs2.push_back(ind2);
+                    b = (int)ftrho_qabs2.size() - 1;
+                }
+                ftrho_order.push_back(b);
+            }
+        const std::vector<double> z(ftrho_qabs2.size(), 0.0);
+        for (auto *v : {&ft1_re, &ft1_im, &ft2_re, &ft2_im}) v->assign(cfg->obs_anzahl, z);
+    }
+    if (nr < 0 || nr >= cfg->obs_anzahl) return;
+    std::vector<double> re((size_t)qb * qb), im((size_t)qb * qb);
+    for (int sp = 1; sp <= (cfg->p.n2 ? 2 : 1); ++sp) {
+        int32_t np = 0;
+        check(h, bdk_ftrho_sums(h, sp, qb, cfg->ftrho_dq, re.data(), im.data(), &np), "ftrho_sums");
+        std::vector<double> &R = sp == 1 ? ft1_re[nr] : ft2_re[nr], &I = sp == 1 ? ft1_im[nr] : ft2_im[nr];
+        std::fill(R.begin(), R.end(), 0.0);
+        std::fill(I.begin(), I.end(), 0.0);
+        for (int k = 0; k < np && k < (int)ftrho_order.size(); ++k) {  // au:341-344
+            R[ftrho_order[k]] += re[k];
+            I[ftrho_order[k]] += im[k];
+        }
+    }
+}
+
+void RunZustand::RunObs::schreibe_ftrho(int run) const {
+    const std::string name = "ftrho_run" + std::to_string(run) + ".txt";
+    FILE *f = std::fopen(name.c_str(), "w");
+    if (!f) return;
+    const bool one = cfg->p.n2 == 0;
+    std::fprintf(f, one ? "# Format: t TAB q TAB rho_re TAB rho_im \n\n"
+                        : "# Format: t TAB q TAB rho1_re TAB rho1_im TAB rho2_re TAB rho2_im \n\n");
+    for (int o = 0; o < (int)ft1_re.size(); ++o) {
+        const double t = o * cfg->obs_dt;
+        for (int i = 0; i < (int)ft1_re[o].size(); ++i) {
+            const double q = cfg->dq * cfg->ftrho_dq * std::sqrt((double)ftrho_qabs2[i]);  // au:983
+            if (one)
+                std::fprintf(f, "%g \t %g \t %g \t %g \n", t, q, ft1_re[o][i], ft1_im[o][i]);
+            else
+                std::fprintf(f, "%g \t %g \t %g \t %g \t %g \t %g \n", t, q, ft1_re[o][i], ft1_im[o][i],
+                             ft2_re[o][i], ft2_im[o][i]);
+        }
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+}
+
 void RunZustand::schreibe_obs() {
     if (cfg_.korrfunk) obs.schreibe_korrfunk(nr);
+    if (cfg_.rhovonk) obs.schreibe_ftrho(nr);
     if (cfg_.rhofftw) obs.schreibe_rhoFFT(nr);
 }
 

@@ -34,6 +34,9 @@ struct RunConfig {
     double korr_dr;
     bool rhofftw;             // auswerten_rhoviaFFTW
     bool abstand;             // auswerten_abstand
+    bool rhovonk;             // auswerten_rhovonk
+    int ftrho_qbins;
+    double ftrho_dq, dq;
     int rhofftw_bins;         // rhoFFTW_bins
     double dq_rhofftw, dq_dqft;
     uint64_t seed;      // MT seed (the reference seeds with time(NULL), zufall.cpp:18)
@@ -121,6 +124,13 @@ class RunZustand {
         // mean distance of the species-2 particles from the box centre
         // (schreibe_abstaende, au:651-670) into abstaende2.txt
         void schreibe_abstaende(int nr);
+        // rho(k) by direct sums on the (i, j) grid, shells by i^2 + j^2
+        // (init_ftrho au:262-300, record_ftrho_unkorrigiert au:307-360) and
+        // their file at the end of the run (schreibe_ftrho, au:977-992)
+        void record_ftrho(bdk_t *h, int nr);
+        void schreibe_ftrho(int run) const;
+        std::vector<int> ftrho_order, ftrho_qabs2;
+        std::vector<std::vector<double>> ft1_re, ft1_im, ft2_re, ft2_im;
         FILE *ab2 = nullptr;
         std::vector<std::vector<double>> rho1_re, rho1_im, rho2_re, rho2_im;
         void close();

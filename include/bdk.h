@@ -312,6 +312,15 @@ int bdk_density_grid(bdk_t *h, int32_t m, double *rho1, double *rho2);
 int bdk_density_stats(bdk_t *h, int32_t m, int32_t nbins, double dbin, const double *rho_thresh,
                       const double *drho_thresh, double *hist, double *over);
 
+/* record_ftrho_unkorrigiert (au:307-360): for the wave-vector index pairs
+   (i, j), i, j < qbins, of the reference's loop (in loop order; the inner
+   loop stops at the first j with i^2 + j^2 > qbins^2), the sums over one
+   species of cos and sin(dq_ft (x i + y j)), x = git_x B + rel_x.  For
+   species 1 the reference takes y = git_y B + rel_x (au:333), kept here;
+   species 2 uses rel_y.  re/im receive *npairs (<= qbins^2) entries; the
+   shells by i^2 + j^2 are the caller's (init_ftrho, au:262-300). */
+int bdk_ftrho_sums(bdk_t *h, int32_t species, int32_t qbins, double dq_ft, double *re, double *im, int32_t *npairs);
+
 /* ---------------------------------------------------------------- batches
    Many independent simulations (the reference's `runs` and parameter sweeps
    over N, L, lambda and scheme) advanced together with one kernel launch per

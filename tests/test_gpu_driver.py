@@ -209,3 +209,50 @@ def test_driver_abstand_file(tmp_path):
     s.close()
     got = [ln for ln in (tmp_path / "abstaende2.txt").read_text().split("\n") if ln and not ln.startswith("#")]
     assert got == lines
+
+
+def test_driver_ftrho_file(tmp_path):
+    """auswerten_rhovonk (BDK_RHOVONK=1): direct rho(k) sums at every
+    checkpoint, shells by i^2 + j^2 in order of first appearance (init_ftrho,
+    au:262-300), written to ftrho_run0.txt (au:977-992, q = dq ftrho_dq
+    sqrt(i^2 + j^2) as the reference writes it); equal to the same records
+    through the binding."""
+    import math
+    print("case: C++ driver, ftrho file", flush=True)
+    env = dict(os.environ, BDK_SEED=str(SEED), BDK_RHOVONK="1")
+    out = subprocess.run([EXE], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    p = bdk.make_params(n1=1000, n2=1000, L=64.0, Z=128, scheme=bdk.CIC, lam=8.0, kap=1.0, T=1.0, dt_max=1e-4,
+                        f2_f1=0.5, precision=bdk.FP32, noise_mode=bdk.NOISE_DISC, seed=SEED)
+    git, rel = bdk.MT(SEED).init_positions(p)
+    s = bdk.RunDynamik(p)
+    s.set_positions(git, rel)
+    qb, dq, fdq = 30, 2.0 * math.pi / 64.0, 4.0 * math.pi / 64.0
+    order, qabs2 = [], []
+    for i in range(qb):
+        for j in range(qb):
+            ind2 = i * i + j * j
+            if ind2 > qb * qb:
+                break
+            if ind2 not in qabs2:
+                qabs2.append(ind2)
+            order.append(qabs2.index(ind2))
+    lines = ["# Format: t TAB q TAB rho1_re TAB rho1_im TAB rho2_re TAB rho2_im ", ""]
+    for o in range(11):
+        if o:
+            s.zeitschritte_bis_obs(0.1)
+        acc = []
+        for sp in (1, 2):
+            re, im = s.ftrho_sums(sp, qb, fdq)
+            R, I = [0.0] * len(qabs2), [0.0] * len(qabs2)
+            for k, b in enumerate(order):
+                R[b] += re[k]
+                I[b] += im[k]
+            acc.append((R, I))
+        for i in range(len(qabs2)):
+            lines.append("%g \t %g \t %g \t %g \t %g \t %g " % (o * 0.1, dq * fdq * math.sqrt(qabs2[i]), acc[0][0][i],
+                                                            acc[0][1][i], acc[1][0][i], acc[1][1][i]))
+        lines.append("")
+    s.close()
+    got = (tmp_path / "ftrho_run0.txt").read_text().split("\n")
+    assert got[:-1] == lines

@@ -140,3 +140,31 @@ def test_snapshot_density_grid_and_stats(m):
     assert np.array_equal(hist, hr) and np.array_equal(over, orf)
     assert np.allclose(hist.sum(1), 1.0)
     s.close()
+
+
+def test_ftrho_sums_match_direct_formula():
+    """Direct rho(k) sums (record_ftrho_unkorrigiert, au:307-360) over the
+    reference's (i, j) pairs against numpy on the device positions; for
+    species 1 y = git_y B + rel_x, as au:333 reads; pair (0, 0) counts the
+    species exactly."""
+    print("case: ftrho sums", flush=True)
+    L, qb = 50.0, 12
+    bp, _ = params(n1=900, n2=600, L=L, Z=64, scheme=bdk.CIC, precision=bdk.FP64, seed=73)
+    git, rel = positions(bp, 73)
+    s = bdk.RunDynamik(bp)
+    s.set_positions(git, rel)
+    s.run_steps(5)
+    g, r = s.positions()
+    B, dq = s.B, 4.0 * np.pi / L
+    pairs = [(i, j) for i in range(qb) for j in range(qb) if i * i + j * j <= qb * qb]
+    # the loop breaks at the first j beyond, which keeps exactly these pairs
+    for sp, sl in ((1, slice(0, 900)), (2, slice(900, 1500))):
+        re, im = s.ftrho_sums(sp, qb, dq)
+        assert len(re) == len(pairs)
+        x = g[sl, 0] * B + r[sl, 0]
+        y = g[sl, 1] * B + (r[sl, 0] if sp == 1 else r[sl, 1])
+        for k, (i, j) in enumerate(pairs):
+            a = dq * (x * i + y * j)
+            assert abs(re[k] - np.cos(a).sum()) < 1e-9 and abs(im[k] - np.sin(a).sum()) < 1e-9
+        assert re[0] == (900 if sp == 1 else 600) and im[0] == 0.0
+    s.close()
