@@ -118,6 +118,7 @@ def lib():
             "bdk_batch_last_error": (C.c_char_p, [vp]),
             "bdk_batch_run_steps": (C.c_int, [vp, i64]),
             "bdk_batch_sync": (C.c_int, [vp]),
+            "bdk_batch_status": (C.c_int, [vp, i32, vp]),
             "bdk_batch_tune": (C.c_int, [vp, i32, vp]),
             "bdk_batch_profile": (C.c_int, [vp, i32, i32, vp]),
             "bdk_serve_create": (C.c_int, [i32, u64, i32, i64, C.POINTER(vp)]),
@@ -147,7 +148,7 @@ def exported_symbols():
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos", "bdk_write_pos_async", "bdk_write_wait",
             "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
             "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_density_grid", "bdk_density_stats", "bdk_ftrho_sums", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
-            "bdk_batch_sync", "bdk_batch_tune", "bdk_batch_profile",
+            "bdk_batch_sync", "bdk_batch_status", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
 
@@ -526,6 +527,12 @@ class Batch:
     def sync(self):
         self._chk(self.L.bdk_batch_sync(self.h))
 
+    def status(self):
+        """Per-member status after the enqueued steps (OK, ERR_NONFINITE, ...)."""
+        out = np.zeros(len(self.sims), np.int32)
+        self._chk(self.L.bdk_batch_status(self.h, len(self.sims), _p(out)))
+        return out
+
     def tune(self, reps=10):
         out = (C.c_int32 * 3)()
         self._chk(self.L.bdk_batch_tune(self.h, reps, C.cast(out, C.c_void_p)))
@@ -578,7 +585,9 @@ class Server:
         return a.value, q.value, d.value
 
     def result(self, jid):
-        n = self.sizes[jid]
+        """(git, rel, t_admit, t_done) of a finished job; raises BdkError with
+        the job's error code if it failed."""
+        n = self.sizes.get(jid, 0)  # unknown ids: the library reports them
         git = np.zeros((n, 2), np.int32)
         rel = np.zeros((n, 2))
         ta, td = C.c_double(0.0), C.c_double(0.0)

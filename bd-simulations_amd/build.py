@@ -39,15 +39,24 @@ def build_lib():
     os.replace(tmp, LIB)
 
 
-def build_driver(param_header=None):
-    """bdk_sim: main.cpp's run loop (main.cpp:130-300) over the C-ABI, with the
-    constants of a reference-format parameter.h (default: host/parameter.h)."""
+def driver_cmd(main_src, exe, param_header=None, extra=()):
+    """g++ line of a driver over the host shim, compiled against the constants
+    of a reference-format parameter.h (default: host/parameter.h)."""
     inc = os.path.dirname(os.path.abspath(param_header)) if param_header else HOST
-    exe = os.path.join(HERE, "bdk_sim")
-    run(["g++", "-O2", "-std=c++17", "-I" + inc, "-I" + HOST, "-I" + os.path.join(ROOT, "include"),
-         os.path.join(HOST, "run_zustand.cpp"), os.path.join(HOST, "params_from_header.cpp"),
-         os.path.join(HOST, "bdk_sim_main.cpp"),
-         "-L" + HERE, "-lbdk_hip", "-Wl,-rpath," + HERE, "-o", exe])
+    return (["g++", "-O2", "-std=c++17", "-Wall", "-I" + inc, "-I" + HOST, "-I" + os.path.join(ROOT, "include")]
+            + list(extra)
+            + [os.path.join(HOST, "run_zustand.cpp"), os.path.join(HOST, "params_from_header.cpp"), main_src,
+               "-L" + HERE, "-lbdk_hip", "-Wl,-rpath," + HERE, "-o", exe])
+
+
+def build_driver(param_header=None):
+    """bdk_sim: main.cpp's run loop (main.cpp:130-300) over the C-ABI; and
+    bdk_refcalls: the driver of tests/drop_in that uses only the reference's
+    calls and parameter.h globals."""
+    run(driver_cmd(os.path.join(HOST, "bdk_sim_main.cpp"), os.path.join(HERE, "bdk_sim"), param_header))
+    ref_main = os.path.join(ROOT, "tests", "drop_in", "reference_calls_main.cpp")
+    if os.path.exists(ref_main):
+        run(driver_cmd(ref_main, os.path.join(HERE, "bdk_refcalls"), param_header))
 
 
 if __name__ == "__main__":

@@ -195,20 +195,38 @@ int build_tables_host(bdk_t *h) {
     return BDK_OK;
 }
 
+// status code and message of the device flags (DevScalars::nonfinite)
+int flag_status(int flags, std::string *msg) {
+    if (flags & 2) {
+        if (msg) *msg = "internal error: inconsistent cell lists (cell counts do not sum to N)";
+        return BDK_ERR_HIP;
+    }
+    if (flags & 4) {
+        if (msg)
+            *msg = "internal error: a particle's charge stencil lay outside its deposit tile's window "
+                   "(displacement bound too small)";
+        return BDK_ERR_HIP;
+    }
+    if (flags) {
+        if (msg) *msg = "a particle position became non-finite or ran away (|rel| >= 1e9); it was held in place";
+        return BDK_ERR_NONFINITE;
+    }
+    return BDK_OK;
+}
+
 int check_nonfinite(bdk_t *h) {
     DevScalars sc;
     HIPCHK(h, hipMemcpyAsync(&sc, h->d.sc, sizeof(sc), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (sc.nonfinite & 2)
-        return fail(h, BDK_ERR_HIP, "internal error: inconsistent cell lists (cell counts do not sum to N)");
-    if (sc.nonfinite & 4)
-        return fail(h, BDK_ERR_HIP,
-                    "internal error: a particle's charge stencil lay outside its deposit tile's window "
-                    "(displacement bound too small)");
-    if (sc.nonfinite)
-        return fail(h, BDK_ERR_NONFINITE,
-                    "a particle position became non-finite or ran away (|rel| >= 1e9); it was held in place");
-    return BDK_OK;
+    std::string m;
+    const int rc = flag_status(sc.nonfinite, &m);
+    return rc ? fail(h, rc, m) : BDK_OK;
+}
+
+// a new particle state starts with no flags raised (they describe a run
+// from the state before)
+hipError_t clear_flags(bdk_t *h) {
+    return hipMemsetAsync(&h->d.sc->nonfinite, 0, sizeof(int), h->stream);
 }
 
 // upload both species in storage order (grouped by cell, ascending id) and
@@ -254,6 +272,7 @@ int upload(bdk_t *h) {
     HIPCHK(h, hipMemcpyAsync(h->d.rel, srel.data(), (size_t)N * 16, hipMemcpyHostToDevice, h->stream));
     if (h->d.comp) HIPCHK(h, hipMemsetAsync(h->d.comp, 0, (size_t)N * 8, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d.count, 0, (size_t)h->g.ncell * 4, h->stream));
+    HIPCHK(h, clear_flags(h));
     HIPCHK(h, bdkx::launch_bin(h));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->dev_valid = true;
@@ -370,11 +389,25 @@ struct bdk_batch {
         gexec = gexec_multi = nullptr;
         graph = graph_multi = nullptr;
     }
-    void free_tables() {
-        if (bp.reps) (void)hipFree(bp.reps);
+    static void free_plan(bdkx::BatchPlan &pl) {
+        if (pl.reps) (void)hipFree(pl.reps);
         for (int k = 0; k < bdkx::BS_COUNT; ++k)
-            if (bp.map[k]) (void)hipFree(bp.map[k]);
-        bp = bdkx::BatchPlan();
+            if (pl.map[k]) (void)hipFree(pl.map[k]);
+        pl = bdkx::BatchPlan();
+    }
+    void free_tables() { free_plan(bp); }
+    // device copies of a plan's member table and block maps
+    static hipError_t upload_plan(bdkx::BatchPlan &pl, const std::vector<bdkx::RepDev> &reps,
+                                  const std::vector<int2> maps[bdkx::BS_COUNT]) {
+        hipError_t e = hipMalloc((void **)&pl.reps, reps.size() * sizeof(bdkx::RepDev));
+        if (e == hipSuccess)
+            e = hipMemcpy(pl.reps, reps.data(), reps.size() * sizeof(bdkx::RepDev), hipMemcpyHostToDevice);
+        for (int k = 0; k < bdkx::BS_COUNT && e == hipSuccess; ++k) {
+            e = hipMalloc((void **)&pl.map[k], std::max<size_t>(1, maps[k].size()) * sizeof(int2));
+            if (e == hipSuccess && !maps[k].empty())
+                e = hipMemcpy(pl.map[k], maps[k].data(), maps[k].size() * sizeof(int2), hipMemcpyHostToDevice);
+        }
+        return e;
     }
     // device tables of the members for elements-per-thread choice ept
     hipError_t rebuild(const int ept[3]) {
@@ -388,14 +421,7 @@ struct bdk_batch {
         std::vector<bdkx::RepDev> reps;
         std::vector<int2> maps[bdkx::BS_COUNT];
         hipError_t e = bdkx::batch_build(mem, e3, bp, reps, maps);
-        if (e == hipSuccess) e = hipMalloc((void **)&bp.reps, reps.size() * sizeof(bdkx::RepDev));
-        if (e == hipSuccess)
-            e = hipMemcpy(bp.reps, reps.data(), reps.size() * sizeof(bdkx::RepDev), hipMemcpyHostToDevice);
-        for (int k = 0; k < bdkx::BS_COUNT && e == hipSuccess; ++k) {
-            e = hipMalloc((void **)&bp.map[k], std::max<size_t>(1, maps[k].size()) * sizeof(int2));
-            if (e == hipSuccess && !maps[k].empty())
-                e = hipMemcpy(bp.map[k], maps[k].data(), maps[k].size() * sizeof(int2), hipMemcpyHostToDevice);
-        }
+        if (e == hipSuccess) e = upload_plan(bp, reps, maps);
         return e;
     }
     hipError_t capture(int nsteps, hipGraph_t *g, hipGraphExec_t *x) {
@@ -420,15 +446,33 @@ struct bdk_batch {
             }
         return hipSuccess;
     }
-    // members uploaded since their last step get their forces first (all
-    // members: the others' forces come out the same)
+    // members uploaded since their last step get their forces first.  When
+    // only some need them (a server admitting jobs beside running ones), the
+    // batched force stages run over those members alone, with the batch's
+    // FFT tiling: a member's forces depend only on its own state and the
+    // tiling, so they equal what the whole batch would compute.
     hipError_t ensure_forces() {
-        bool all = true;
-        for (bdk_t *h : mem) all = all && h->forces_ready;
-        if (all) return hipSuccess;
-        hipError_t e = bdkx::batch_enqueue_forces(bp, stream, aux, ev_fork, ev_join);
+        std::vector<bdk_t *> need;
+        for (bdk_t *h : mem)
+            if (!h->forces_ready) need.push_back(h);
+        if (need.empty()) return hipSuccess;
+        hipError_t e = hipSuccess;
+        if (need.size() == mem.size()) {
+            e = bdkx::batch_enqueue_forces(bp, stream, aux, ev_fork, ev_join);
+        } else {
+            bdkx::BatchPlan sub;
+            std::vector<bdkx::RepDev> reps;
+            std::vector<int2> maps[bdkx::BS_COUNT];
+            e = bdkx::batch_build(need, bp.ept, sub, reps, maps);
+            if (e == hipSuccess) e = upload_plan(sub, reps, maps);
+            if (e == hipSuccess) e = bdkx::batch_enqueue_forces(sub, stream, aux, ev_fork, ev_join);
+            // the tables are freed once the force stages that read them are done
+            const hipError_t es = hipStreamSynchronize(stream);
+            if (e == hipSuccess) e = es;
+            free_plan(sub);
+        }
         if (e == hipSuccess)
-            for (bdk_t *h : mem) h->forces_ready = true;
+            for (bdk_t *h : need) h->forces_ready = true;
         return e;
     }
     hipError_t ensure_graphs() {
@@ -1689,6 +1733,7 @@ int bdk_load_state(bdk_t *h, const char *path) {
         HIPCHK(h, hipMemcpyAsync(h->d.comp, comp.data(), (size_t)N * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d.sc, &step, 8, hipMemcpyHostToDevice, h->stream));  // DevScalars::step
     HIPCHK(h, hipMemsetAsync(h->d.count, 0, (size_t)h->g.ncell * 4, h->stream));
+    HIPCHK(h, clear_flags(h));
     HIPCHK(h, bdkx::launch_bin(h));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->pos_set[0] = h->pos_set[1] = true;
@@ -1970,10 +2015,23 @@ int bdk_batch_sync(bdk_batch_t *b) {
         DevScalars sc;
         e = hipMemcpy(&sc, b->mem[i]->d.sc, sizeof(sc), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_sync: ") + hipGetErrorString(e));
-        if (sc.nonfinite)
-            return b->fail(BDK_ERR_NONFINITE, "member " + std::to_string(i) +
-                                                  ": a particle position became non-finite or ran away");
+        std::string m;
+        const int rc = flag_status(sc.nonfinite, &m);
+        if (rc) return b->fail(rc, "member " + std::to_string(i) + ": " + m);
     }
+    return BDK_OK;
+}
+
+int bdk_batch_status(bdk_batch_t *b, int32_t n, int32_t *status) {
+    if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
+    if (!status || n != (int32_t)b->mem.size()) return b->fail(BDK_ERR_ARG, "status needs one entry per member");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    for (int32_t i = 0; i < n && e == hipSuccess; ++i) {
+        DevScalars sc;
+        e = hipMemcpy(&sc, b->mem[i]->d.sc, sizeof(sc), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) status[i] = flag_status(sc.nonfinite, nullptr);
+    }
+    if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_status: ") + hipGetErrorString(e));
     return BDK_OK;
 }
 

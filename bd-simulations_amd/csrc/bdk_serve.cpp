@@ -5,10 +5,11 @@
 // (main.cpp:130-300); the server is a client of the public C-ABI only.
 //
 // One tick: admit queued jobs in arrival order while the budget and the job
-// limit allow, rebuild the batch if the set of jobs changed, advance every
-// active job by min(tick, smallest remaining step count) steps with one
-// launch per stage, then retire the jobs that are done (their positions are
-// copied to the host and their device state is freed).  A job's trajectory
+// limit allow, rebuild the batches if the set of jobs changed (one batch per
+// precision class, side by side on the device), advance every active job by
+// min(tick, smallest remaining step count) steps with one launch per stage
+// and class, then retire the jobs that are done (their positions are copied
+// to the host and their device state is freed).  A job's trajectory
 // is bit-identical to running it alone, whatever it shares the device with.
 
 #include <algorithm>
@@ -37,6 +38,8 @@ struct Job {
     uint64_t bytes = 0;
     bdk_t *h = nullptr;
     double t_submit = 0, t_admit = 0, t_done = 0;
+    int status = BDK_OK;  // a failed job: its error code (admission, run-away, read-back)
+    std::string err;
 };
 
 }  // namespace
@@ -50,7 +53,13 @@ struct bdk_server {
     std::deque<Job *> queue;
     std::vector<Job *> active;
     std::map<int64_t, Job *> finished;
-    bdk_batch_t *batch = nullptr;
+    // one batch per precision class of the active jobs (a batch needs one
+    // precision); idx = the jobs' positions in `active`
+    struct Group {
+        bdk_batch_t *batch = nullptr;
+        std::vector<int> idx;
+    };
+    std::vector<Group> groups;
     bool dirty = false;
     int64_t particle_steps = 0;
     double busy_s = 0;
@@ -59,6 +68,11 @@ struct bdk_server {
     int fail(int code, const std::string &m) {
         err = m;
         return code;
+    }
+    void drop_batches() {
+        for (Group &g : groups)
+            if (g.batch) bdk_batch_destroy(g.batch);
+        groups.clear();
     }
 };
 
@@ -78,7 +92,7 @@ int bdk_serve_create(int32_t device, uint64_t hbm_budget, int32_t max_jobs, int6
 
 void bdk_serve_destroy(bdk_server_t *s) {
     if (!s) return;
-    if (s->batch) bdk_batch_destroy(s->batch);
+    s->drop_batches();
     for (Job *j : s->active) {
         bdk_destroy(j->h);
         delete j;
@@ -114,77 +128,132 @@ int bdk_serve_submit(bdk_server_t *s, const bd_params *p, const int32_t *git, co
     return BDK_OK;
 }
 
-// One tick; *n_active / *n_queued / *n_done receive the counts after it.
+// A job leaves the device (finished or failed): its device state is freed and
+// it moves to the finished map.  The caller has already taken it out of the
+// active set (or the queue); the batch that referred to it is gone.
+static void retire(bdk_server_t *s, Job *j, int status, const std::string &err) {
+    if (j->h) {
+        bdk_destroy(j->h);
+        j->h = nullptr;
+        s->used -= j->bytes;
+    }
+    j->status = status;
+    j->err = err;
+    j->t_done = now_s();
+    s->finished[j->id] = j;
+}
+
+// One tick; *n_active / *n_queued / *n_done receive the counts after it
+// (n_done counts failed jobs too).  Every job is at all times in exactly one
+// of the queue, the active set and the finished map.  A job that cannot be
+// admitted (bdk_create or the upload fails) or whose particles run away is
+// finished with its error code, which bdk_serve_result reports; the other
+// jobs go on.  The tick itself fails only when the device fails.
 int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_t *n_done) {
     if (!s) return BDK_ERR_ARG;
     // admission in arrival order
     while (!s->queue.empty() && (int)s->active.size() < s->max_jobs) {
         Job *j = s->queue.front();
         if (s->used + j->bytes > s->budget) break;
+        s->queue.pop_front();
+        j->t_admit = now_s();
         bdk_t *h = nullptr;
         int rc = bdk_create(&j->p, &h);
-        if (rc) return s->fail(rc, std::string("admit: ") + bdk_last_error(nullptr));
-        const int n1 = j->p.n1, n2 = j->p.n2;
-        rc = bdk_set_positions(h, 1, n1, j->git.data(), j->rel.data());
-        if (!rc && n2) rc = bdk_set_positions(h, 2, n2, j->git.data() + 2 * (size_t)n1, j->rel.data() + 2 * (size_t)n1);
+        std::string m;
         if (rc) {
-            std::string m = bdk_last_error(h);
-            bdk_destroy(h);
-            return s->fail(rc, "admit: " + m);
+            m = bdk_last_error(nullptr);
+        } else {
+            const int n1 = j->p.n1, n2 = j->p.n2;
+            rc = bdk_set_positions(h, 1, n1, j->git.data(), j->rel.data());
+            if (!rc && n2)
+                rc = bdk_set_positions(h, 2, n2, j->git.data() + 2 * (size_t)n1, j->rel.data() + 2 * (size_t)n1);
+            if (rc) {
+                m = bdk_last_error(h);
+                bdk_destroy(h);
+                h = nullptr;
+            }
+        }
+        if (rc) {
+            retire(s, j, rc, "admit: " + m);
+            continue;
         }
         j->h = h;
-        j->t_admit = now_s();
         s->used += j->bytes;
-        s->queue.pop_front();
         s->active.push_back(j);
         s->dirty = true;
     }
     if (!s->active.empty()) {
         if (s->dirty) {
-            if (s->batch) bdk_batch_destroy(s->batch);
-            s->batch = nullptr;
-            std::vector<bdk_t *> hs;
-            for (Job *j : s->active) hs.push_back(j->h);
-            int rc = bdk_batch_create(hs.data(), (int32_t)hs.size(), &s->batch);
-            if (rc) return s->fail(rc, std::string("batch: ") + bdk_batch_last_error(nullptr));
+            s->drop_batches();
+            std::map<int, std::vector<int>> cls;
+            for (int i = 0; i < (int)s->active.size(); ++i) cls[s->active[i]->p.precision].push_back(i);
+            for (auto &kv : cls) {
+                bdk_server::Group g;
+                g.idx = kv.second;
+                std::vector<bdk_t *> hs;
+                for (int i : g.idx) hs.push_back(s->active[i]->h);
+                int rc = bdk_batch_create(hs.data(), (int32_t)hs.size(), &g.batch);
+                if (rc) {
+                    s->drop_batches();
+                    return s->fail(rc, std::string("batch: ") + bdk_batch_last_error(nullptr));
+                }
+                s->groups.push_back(g);
+            }
             s->dirty = false;
         }
         int64_t k = s->tick;
         for (Job *j : s->active) k = std::min(k, j->steps - j->done);
         const double t0 = now_s();
-        int rc = bdk_batch_run_steps(s->batch, k);
-        if (!rc) rc = bdk_batch_sync(s->batch);
-        if (rc) return s->fail(rc, std::string("tick: ") + bdk_batch_last_error(s->batch));
-        s->busy_s += now_s() - t0;
-        std::vector<Job *> keep;
+        // every class's steps are enqueued before the first wait: the batches
+        // run side by side on their own streams
+        for (bdk_server::Group &g : s->groups) {
+            const int rc = bdk_batch_run_steps(g.batch, k);
+            if (rc) return s->fail(rc, std::string("tick: ") + bdk_batch_last_error(g.batch));
+        }
+        // the steps are enqueued: they count whatever the members' state
         for (Job *j : s->active) {
             j->done += k;
             s->particle_steps += k * ((int64_t)j->p.n1 + j->p.n2);
-            if (j->done < j->steps) {
+        }
+        const int n = (int)s->active.size();
+        std::vector<int32_t> st(n, BDK_OK);
+        for (bdk_server::Group &g : s->groups) {
+            std::vector<int32_t> gs(g.idx.size(), BDK_OK);
+            const int rc = bdk_batch_status(g.batch, (int32_t)gs.size(), gs.data());
+            if (rc) return s->fail(rc, std::string("tick: ") + bdk_batch_last_error(g.batch));
+            for (size_t q = 0; q < g.idx.size(); ++q) st[g.idx[q]] = gs[q];
+        }
+        s->busy_s += now_s() - t0;
+        std::vector<Job *> keep, out;
+        std::vector<int> out_rc;
+        std::vector<std::string> out_err;
+        for (int i = 0; i < n; ++i) {
+            Job *j = s->active[i];
+            if (st[i] == BDK_OK && j->done < j->steps) {
                 keep.push_back(j);
                 continue;
             }
-            const int n1 = j->p.n1, n2 = j->p.n2;
-            rc = bdk_get_positions(j->h, 1, n1, j->git.data(), j->rel.data());
-            if (!rc && n2)
-                rc = bdk_get_positions(j->h, 2, n2, j->git.data() + 2 * (size_t)n1, j->rel.data() + 2 * (size_t)n1);
-            if (rc) return s->fail(rc, std::string("retire: ") + bdk_last_error(j->h));
-            j->t_done = now_s();
-            s->finished[j->id] = j;
-            s->used -= j->bytes;
+            int jrc = st[i];
+            std::string m = jrc ? "run: a particle position became non-finite or ran away" : "";
+            if (!jrc) {
+                const int n1 = j->p.n1, n2 = j->p.n2;
+                jrc = bdk_get_positions(j->h, 1, n1, j->git.data(), j->rel.data());
+                if (!jrc && n2)
+                    jrc = bdk_get_positions(j->h, 2, n2, j->git.data() + 2 * (size_t)n1,
+                                            j->rel.data() + 2 * (size_t)n1);
+                if (jrc) m = std::string("retire: ") + bdk_last_error(j->h);
+            }
+            out.push_back(j);
+            out_rc.push_back(jrc);
+            out_err.push_back(m);
+        }
+        if (!out.empty()) {
+            // the batches refer to the leaving handles: drop them before freeing those
+            s->drop_batches();
             s->dirty = true;
+            s->active.swap(keep);
+            for (size_t i = 0; i < out.size(); ++i) retire(s, out[i], out_rc[i], out_err[i]);
         }
-        if (s->dirty) {
-            // the batch refers to the retiring handles: drop it before freeing them
-            if (s->batch) bdk_batch_destroy(s->batch);
-            s->batch = nullptr;
-            for (Job *j : s->active)
-                if (j->done >= j->steps) {
-                    bdk_destroy(j->h);
-                    j->h = nullptr;
-                }
-        }
-        s->active.swap(keep);
     }
     if (n_active) *n_active = (int32_t)s->active.size();
     if (n_queued) *n_queued = (int32_t)s->queue.size();
@@ -193,16 +262,18 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
 }
 
 // Result of a finished job: positions (species 1 first) and its times in
-// seconds from submission: admitted after t_admit, finished after t_done.
+// seconds from submission: admitted after t_admit, finished after t_done.  A
+// failed job returns its error code (positions not written, times written).
 int bdk_serve_result(bdk_server_t *s, int64_t job_id, int32_t *git, double *rel, double *t_admit, double *t_done) {
     if (!s) return BDK_ERR_ARG;
     auto it = s->finished.find(job_id);
     if (it == s->finished.end()) return s->fail(BDK_ERR_STATE, "job not finished (or unknown id)");
     const Job *j = it->second;
-    if (git) std::memcpy(git, j->git.data(), j->git.size() * sizeof(int32_t));
-    if (rel) std::memcpy(rel, j->rel.data(), j->rel.size() * sizeof(double));
     if (t_admit) *t_admit = j->t_admit - j->t_submit;
     if (t_done) *t_done = j->t_done - j->t_submit;
+    if (j->status) return s->fail(j->status, "job " + std::to_string(job_id) + " failed: " + j->err);
+    if (git) std::memcpy(git, j->git.data(), j->git.size() * sizeof(int32_t));
+    if (rel) std::memcpy(rel, j->rel.data(), j->rel.size() * sizeof(double));
     return BDK_OK;
 }
 

@@ -43,8 +43,9 @@ def batched_mix(nrep=256, seed=3):
     return out
 
 
-def served_jobs(njobs=64, seed=4, max_steps=100000):
-    """[(params, git, rel, steps)] of the served job stream."""
+def served_jobs(njobs=64, seed=4, max_steps=100000, step_scale=1.0):
+    """[(params, git, rel, steps)] of the served job stream; step_scale
+    multiplies every job's step count (at least 1 step)."""
     rng = np.random.default_rng(seed)
     out = []
     for j in range(njobs):
@@ -54,6 +55,7 @@ def served_jobs(njobs=64, seed=4, max_steps=100000):
         rho = rng.uniform(0.3, 0.7)
         L = math.sqrt(N / rho)
         steps = int(round(math.exp(rng.uniform(math.log(1000), math.log(max_steps)))))
+        steps = max(1, int(round(steps * step_scale)))
         p = bdk.make_params(n1=n1, n2=N - n1, L=L, Z=mesh_for(L), scheme=int(rng.choice([bdk.NGP, bdk.CIC, bdk.TSC])),
                             lam=float(rng.uniform(5, 100)), kap=1.0, f2_f1=0.5, T=1.0, dt_max=1e-4,
                             precision=bdk.FP32, seed=int(seed * 100003 + j))

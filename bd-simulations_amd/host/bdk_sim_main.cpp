@@ -1,4 +1,6 @@
-// Driver with the run loop of main.cpp:130-300 over the GPU timestep:
+// Driver with the run loop of main.cpp:130-300 over the GPU timestep (the
+// parameter.h switches can be overridden by environment variables, see
+// params_from_header.cpp):
 // for each run, init, obs_point(0), then obs_anzahl-1 checkpoint intervals
 // of zeitschritte_bis_obs + obs_point, writing pos1_<run+1>.txt and
 // pos2_<run+1>.txt; or, as in main.cpp, the mrd sweep (debugmode) or the
@@ -74,7 +76,7 @@ void force_scan(RunZustand &theRun, const RunConfig &cfg) {
         const double y = std::fmod(origin + t, cfg.p.L);
         theRun.r1_git[1][1] = (int)(y / B);
         theRun.r1_rel[1][1] = y - theRun.r1_git[1][1] * B;
-        theRun.dyn.push_positions();
+        // the shim uploads the edited arrays before the next force call
         theRun.dyn.berechne_kapkraefte();
         theRun.dyn.berechne_WCA11();
         std::fprintf(out, "%g \t %g \t %g \n", t, theRun.dyn.getFKap(), theRun.dyn.getF());
@@ -85,10 +87,11 @@ void force_scan(RunZustand &theRun, const RunConfig &cfg) {
 }  // namespace
 
 int main() {
-    RunConfig cfg = config_from_parameter_h();
+    const RunConfig cfg = config_from_parameter_h();  // the loop bounds and modes of the runs below
     const time_t started = std::time(nullptr);
     for (int run = 0; run < cfg.runs; ++run) {
-        RunZustand theRun(cfg);
+        RunZustand theRun;  // main.cpp:132: configured from parameter.h
+        theRun.ausgabe_jeansgroessen();
         theRun.init(run);
         theRun.obs_point(0);
         if (cfg.debugmode) {  // main.cpp:156-216

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <iostream>
 #include <stdexcept>
 
@@ -40,6 +41,40 @@ void RunZustand::RunDynamik::push_positions() {
     check(h, bdk_set_positions(h, 1, cfg->p.n1, g.data(), r.data()), "set_positions(1)");
     gather_pos(cfg->p.n2, r2_git, r2_rel, g, r);
     check(h, bdk_set_positions(h, 2, cfg->p.n2, g.data(), r.data()), "set_positions(2)");
+    remember_device_positions();
+}
+
+void RunZustand::RunDynamik::remember_device_positions() {
+    std::vector<int32_t> g2;
+    std::vector<double> r2;
+    gather_pos(cfg->p.n1, r1_git, r1_rel, dev_git_, dev_rel_);
+    gather_pos(cfg->p.n2, r2_git, r2_rel, g2, r2);
+    dev_git_.insert(dev_git_.end(), g2.begin(), g2.end());
+    dev_rel_.insert(dev_rel_.end(), r2.begin(), r2.end());
+}
+
+void RunZustand::RunDynamik::push_if_changed() {
+    if (!h) return;
+    const int n1 = cfg->p.n1, n2 = cfg->p.n2;
+    if (dev_git_.size() != 2 * (size_t)(n1 + n2)) {
+        push_positions();
+        return;
+    }
+    for (int sp = 0; sp < 2; ++sp) {
+        const int n = sp ? n2 : n1;
+        const size_t off = sp ? 2 * (size_t)n1 : 0;
+        int **git = sp ? r2_git : r1_git;
+        double **rel = sp ? r2_rel : r1_rel;
+        for (int i = 0; i < n; ++i) {
+            const size_t k = off + 2 * (size_t)i;
+            // bitwise comparison: a NaN written by the driver counts as a change
+            if (git[i][0] != dev_git_[k] || git[i][1] != dev_git_[k + 1] ||
+                std::memcmp(rel[i], &dev_rel_[k], 2 * sizeof(double)) != 0) {
+                push_positions();
+                return;
+            }
+        }
+    }
 }
 
 void RunZustand::RunDynamik::sync_positions() {
@@ -57,6 +92,7 @@ void RunZustand::RunDynamik::sync_positions() {
             rel[i][1] = r[2 * i + 1];
         }
     }
+    remember_device_positions();
 }
 
 void RunZustand::RunDynamik::init() {
@@ -65,12 +101,14 @@ void RunZustand::RunDynamik::init() {
 }
 
 void RunZustand::RunDynamik::berechne_WCA11() {
+    push_if_changed();
     F1_WCA.assign(2 * (size_t)cfg->p.n1, 0.0);
     F2_WCA.assign(2 * (size_t)cfg->p.n2, 0.0);
     check(h, bdk_compute_wca(h, F1_WCA.data(), F2_WCA.data()), "compute_wca");
 }
 
 void RunZustand::RunDynamik::berechne_kapkraefte() {
+    push_if_changed();
     F1kap.assign(2 * (size_t)cfg->p.n1, 0.0);
     F2kap.assign(2 * (size_t)cfg->p.n2, 0.0);
     check(h, bdk_compute_capillary(h, F1kap.data(), F2kap.data()), "compute_capillary");
@@ -84,31 +122,41 @@ void RunZustand::RunDynamik::berechne_zufallskraefte() {
 }
 
 double RunZustand::RunDynamik::optimaler_zeitschritt() {
+    push_if_changed();
     double dt = 0.0;
     check(h, bdk_optimal_dt(h, &dt), "optimal_dt");
     return dt;
 }
 
-void RunZustand::RunDynamik::refresh_erwNachbar() { check(h, bdk_refresh(h), "refresh"); }
+void RunZustand::RunDynamik::refresh_erwNachbar() {
+    push_if_changed();
+    check(h, bdk_refresh(h), "refresh");
+}
 
 double RunZustand::RunDynamik::zeitschritt(double tmax) {
     if (tmax <= 0.0) {  // dm:80-83
         std::cout << "Fehler: Negative Zeitschrittweite " << tmax << " gefordert, breche ab!" << std::endl;
         return 1.0e10;
     }
+    push_if_changed();
     if (cfg->p.noise_mode == BDK_NOISE_UPLOAD && !cfg->p.no_wca) berechne_zufallskraefte();
     // the parameter's travel cap (zeitschritt_debug may have set another)
     check(h, bdk_set_max_travel(h, cfg->p.max_travel), "set_max_travel");
     double dt = 0.0;
     check(h, bdk_step(h, tmax, &dt), "step");
+    // the reference's arrays always hold the current positions (dm:116-157
+    // updates them in place); a single step keeps the aliases current
+    sync_positions();
     return dt;
 }
 
 void RunZustand::RunDynamik::zeitschritt_debug(double mrd, TimestepInfo &info, bool print) {
+    push_if_changed();
     if (cfg->p.noise_mode == BDK_NOISE_UPLOAD && !cfg->p.no_wca) berechne_zufallskraefte();
     check(h, bdk_set_max_travel(h, mrd), "set_max_travel");
     bdk_step_info_t s;
     check(h, bdk_step_info(h, 1.0, &s), "step_info");
+    sync_positions();
     info.num_WCA_berechnet = (int)s.num_wca;
     info.maxWCAx = s.max_wca[0];
     info.maxWCAy = s.max_wca[1];
@@ -142,7 +190,22 @@ void RunZustand::RunDynamik::TimestepInfo::ausgabe() const {
               << "dt = " << dt << std::endl;
 }
 
+RunZustand::RunZustand() : cfg_(config_from_parameter_h()) {}
+
 RunZustand::~RunZustand() { obs.close(); }
+
+void RunZustand::ausgabe_jeansgroessen() const {
+    // mean density, Jeans time 1/(rho v) and the Jeans wave number with the
+    // hard-disc correction of am:324-348 (close packing rho_c = 2/sqrt 3)
+    const double rho = (cfg_.p.n1 + cfg_.p.n2) / (cfg_.p.L * cfg_.p.L);
+    const double rho_c = 2.0 / std::sqrt(3.0);
+    const double v = cfg_.p.kap;
+    const double tJ = 1.0 / (rho * v);
+    const double corr = 1.0 - 3.0 * rho * rho / (rho_c * rho_c) - 8.0 * rho * rho * rho / (rho_c * rho_c * rho_c);
+    const double kJ = std::sqrt(v / cfg_.p.T * rho / corr);
+    std::cout << "Jeans-Zeit: \t\t" << tJ << std::endl;
+    std::cout << "Jeans-Wellenzahl:\t" << kJ << std::endl;
+}
 
 void RunZustand::init_zufallspos() {
     bd_params p = cfg_.p;
@@ -209,6 +272,7 @@ void RunZustand::init(int nr_) {
 
 void RunZustand::zeitschritte_bis_obs(time_t last_write) {
     (void)last_write;
+    dyn.push_if_changed();
     if (cfg_.p.noise_mode != BDK_NOISE_UPLOAD) {
         // the library runs the loop of dm:444-455 on the device (fixed dt: a
         // closed step count; adaptive dt: logged dt's summed in long double)

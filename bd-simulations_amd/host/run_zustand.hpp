@@ -8,6 +8,10 @@
 // Positions keep the reference's layout, int** r*_git and double** r*_rel
 // of shape [N][2] (s2:380-383).  They are the truth at init() and are
 // refreshed from the device at every obs_point() and by sync_positions().
+// A driver may also write them directly, as the reference's force test does
+// (main.cpp:220-248, which moves a particle and recomputes the forces): the
+// next force or step call sees that the arrays differ from what the device
+// holds and uploads them first.
 #pragma once
 
 #include <ctime>
@@ -94,6 +98,9 @@ class RunZustand {
         void sync_positions();
         // push the aliased host arrays to the device (after an external change)
         void push_positions();
+        // push_positions() if the aliased arrays changed since the last
+        // push or sync (a driver wrote them)
+        void push_if_changed();
 
         int **r1_git = nullptr, **r2_git = nullptr;
         double **r1_rel = nullptr, **r2_rel = nullptr;
@@ -102,6 +109,12 @@ class RunZustand {
         const RunConfig *cfg = nullptr;
         bdk_mt_t *mt = nullptr;  // MT stream for the upload noise source
         std::vector<double> F1kap, F2kap, F1_WCA, F2_WCA;
+
+      private:
+        // what the device holds, species 1 then 2, as of the last push or sync
+        std::vector<int32_t> dev_git_;
+        std::vector<double> dev_rel_;
+        void remember_device_positions();
     };
 
     RunDynamik dyn;
@@ -140,8 +153,16 @@ class RunZustand {
     };
     RunObs obs;
 
+    // The reference's `RunZustand theRun;` (main.cpp:132): the run
+    // configuration comes from the parameter.h that params_from_header.cpp
+    // was compiled against.
+    RunZustand();
     explicit RunZustand(const RunConfig &cfg) : cfg_(cfg) {}
     ~RunZustand();
+    RunZustand(const RunZustand &) = delete;
+    RunZustand &operator=(const RunZustand &) = delete;
+
+    const RunConfig &config() const { return cfg_; }
 
     int nr = 0;
     long double t = 0.0L;
@@ -155,9 +176,12 @@ class RunZustand {
     void zeitschritte_bis_obs(time_t last_write);  // dm:444-455
     void obs_point(int nr_obspoint);              // au:78-93
     void schreibe_obs();                          // au:912-928
+    // Jeans time and wave number of the run's mean density (am:324-348),
+    // printed to stdout
+    void ausgabe_jeansgroessen() const;
 
   private:
-    const RunConfig &cfg_;
+    RunConfig cfg_;
     std::vector<int> git_store1_, git_store2_;
     std::vector<double> rel_store1_, rel_store2_;
     std::vector<int *> git_ptr1_, git_ptr2_;

@@ -345,6 +345,11 @@ const char *bdk_batch_last_error(const bdk_batch_t *b);
 int bdk_batch_run_steps(bdk_batch_t *b, int64_t k);
 /* Wait for the batch; BDK_ERR_NONFINITE names the first member that ran away. */
 int bdk_batch_sync(bdk_batch_t *b);
+/* Wait for the batch and report each member's state: status[i] = BDK_OK, or
+   BDK_ERR_NONFINITE for a member whose particle ran away (held in place; the
+   flag stays until the member's positions are set again), or BDK_ERR_HIP for
+   an internal inconsistency.  n = the number of members. */
+int bdk_batch_status(bdk_batch_t *b, int32_t n, int32_t *status);
 /* Project autotuner: times the FFT stages for elements per thread 8/16/32
    (reps launches each) and keeps the fastest; ept_out[3] receives the choice. */
 int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *ept_out);
@@ -361,8 +366,13 @@ int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_pe
    most `max_jobs` run, advances every running job by
    min(tick_steps, smallest remaining step count) steps in one batch, and
    retires the jobs that are done (positions copied back, device state
-   freed).  Served jobs use fixed dt, capillary forces and device noise.
-   Each job's result equals running it alone with bdk_run_steps. */
+   freed).  Served jobs use fixed dt, capillary forces and device noise;
+   jobs of different precision advance in one batch per precision, side by
+   side on the device.
+   Each job's result equals running it alone with bdk_run_steps.  A job
+   that cannot be admitted (bdk_create or the upload fails) or whose
+   particles run away is finished as failed and the others go on;
+   bdk_serve_step itself fails only when the device fails. */
 typedef struct bdk_server bdk_server_t;
 
 int bdk_serve_create(int32_t device, uint64_t hbm_budget, int32_t max_jobs, int64_t tick_steps,
@@ -373,7 +383,9 @@ int bdk_serve_submit(bdk_server_t *s, const bd_params *p, const int32_t *git, co
                      int64_t *job_id);
 int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_t *n_done);
 /* Positions of a finished job ([n1+n2][2], species 1 first) and its admission
-   and completion times in seconds after submission. */
+   and completion times in seconds after submission.  For a failed job the
+   call returns the job's error code (bdk_serve_last_error says why) and
+   writes only the times. */
 int bdk_serve_result(bdk_server_t *s, int64_t job_id, int32_t *git, double *rel, double *t_admit, double *t_done);
 int bdk_serve_stats(const bdk_server_t *s, int64_t *particle_steps, double *busy_s, uint64_t *hbm_used);
 

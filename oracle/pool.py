@@ -31,3 +31,18 @@ def run_replica(args):
             break
     o.close()
     return n * steps, dt
+
+
+def forces(args):
+    """args = (params dict for oracle.make_params, git, rel).  Returns the
+    oracle's (WCA, capillary) forces [N][2] of all particles, species 1
+    first, on these positions (dm:617-1059, dm:1180-1233)."""
+    import oracle as O
+
+    kw, git, rel = args
+    o = O.Oracle(O.make_params(**kw))
+    o.set_all(np.asarray(git), np.asarray(rel))
+    w = o.wca()
+    k = o.kap()
+    o.close()
+    return w, k

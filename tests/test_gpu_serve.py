@@ -68,3 +68,67 @@ def test_server_rejects_oversized_and_adaptive_jobs():
     with pytest.raises(bdk.BdkError):
         srv.submit(bpa, git, rel, 10)
     srv.close()
+
+
+def overlapping(bp, git, rel, i=0, j=1, gap=1e-3):
+    """Particle j moved to gap from particle i along x (same y): with an
+    unbounded WCA cap the pair force overflows, and the update holds such
+    particles in place and raises the run-away flag."""
+    d = bdk.derived(bp)
+    B, nc = d["B"], d["nc"]
+    git, rel = git.copy(), rel.copy()
+    x = (git[i] * B + rel[i]) + np.array([gap, 0.0])
+    x %= bp.L
+    git[j] = np.floor(x / B).astype(np.int32) % nc
+    rel[j] = x - git[j] * B
+    return git, rel
+
+
+def test_runaway_flag_cleared_by_new_positions():
+    print("case: run-away flag and recovery", flush=True)
+    bp, _ = params(n1=300, n2=200, L=30.0, Z=32, precision=bdk.FP32, max_pair_wca=1e300, seed=8)
+    git, rel = positions(bp, 8, 0.9)
+    s = bdk.RunDynamik(bp)
+    s.set_positions(*overlapping(bp, git, rel))
+    with pytest.raises(bdk.BdkError) as e:
+        s.run_steps(3)
+    assert e.value.code == bdk.ERR_NONFINITE
+    with pytest.raises(bdk.BdkError) as e:
+        s.positions()
+    assert e.value.code == bdk.ERR_NONFINITE
+    s.set_positions(git, rel)  # a new state starts without the flag
+    s.run_steps(3)
+    g1, r1 = s.positions()
+    t = bdk.RunDynamik(bp)
+    t.set_positions(git, rel)
+    t.run_steps(3)
+    g2, r2 = t.positions()
+    assert np.array_equal(g1, g2) and np.array_equal(r1, r2)
+
+
+def test_runaway_job_fails_alone_others_finish():
+    print("case: served run-away job", flush=True)
+    jobs = [job(JOBS[0]), job(JOBS[4]), job(JOBS[2])]
+    bp, git, rel, steps = jobs[2]
+    bad = bdk.make_params(**{f: getattr(bp, f) for f, _ in bdk.Params._fields_})
+    bad.max_pair_wca = 1e300
+    jobs[2] = (bad,) + overlapping(bad, git, rel) + (steps,)
+    srv = bdk.Server(hbm_budget=1 << 34, max_jobs=3, tick_steps=25)
+    ids = [srv.submit(*j) for j in jobs]
+    for _ in range(100):
+        active, queued, done = srv.step()
+        if done == 3:
+            break
+    assert done == 3
+    with pytest.raises(bdk.BdkError) as e:
+        srv.result(ids[2])
+    assert e.value.code == bdk.ERR_NONFINITE
+    for jid, (p, g0, r0, st) in zip(ids[:2], jobs[:2]):
+        g_srv, r_srv, _, _ = srv.result(jid)
+        s = bdk.RunDynamik(p)
+        s.set_positions(g0, r0)
+        s.run_steps(st)
+        g_alone, r_alone = s.positions()
+        assert np.array_equal(g_srv, g_alone) and np.array_equal(r_srv, r_alone), "job %d" % jid
+    assert srv.stats()[2] == 0
+    srv.close()

@@ -374,3 +374,50 @@ def test_pos_writer_format(tmp_path):
             lines.append("%g \t %g \t %g " % (nr * 0.2, xs[i, 0], xs[i, 1]))
         lines.append("")
     assert txt == "\n".join(lines) + "\n"
+
+
+def test_restrict_radial_invariants():
+    """restrictRadial (dm:138-155): after a step, species-2 particle i lies
+    on the ray at angle 2 pi i / N2 about the box centre, at the distance
+    from the centre that its move without the switch reaches; species 1 moves
+    as without the switch.  Species 2 starts on its rays (random distances)
+    so that a step moves it by much less than a list cell: the reference's
+    refresh re-bins with a single fmod (dm:216-325), which misplaces moves of
+    more than a cell, as the projection from an arbitrary start would make.
+    Checked from three states (the start, after 5 and after 10 steps)
+    against a second oracle that steps the same state (same forces, same
+    Philox draws) without the switch."""
+    # L = 31: the centre lies inside a list cell (13.5 B), so the rays at
+    # multiples of pi/2 do not run along cell edges
+    n1, n2, L = 200, 40, 31.0
+    kw = dict(n1=n1, n2=n2, L=L, Z=32, scheme=2, T=1.0, dt_max=1e-4, noise_mode=1, seed=77)
+    m = 0.5 * L
+    want = 2 * np.pi * np.arange(n2) / n2
+    rad = np.random.default_rng(4).uniform(8.0, 13.0, n2)  # neighbouring rays >= 1.25 apart
+    x2 = np.stack([m + rad * np.cos(want), m + rad * np.sin(want)], 1)
+    x1 = rsa_positions(600, L, 0.9, 95)
+    d = np.sqrt(((x1[:, None, :] - x2[None, :, :]) ** 2).sum(2)).min(1)
+    x1 = x1[d > 1.5][:n1]
+    base = O.Oracle(O.make_params(restrict_radial=1, **kw))
+    git, rel = to_git_rel(np.concatenate([x1, x2]), base.B, base.nc)
+    base.set_all(git, rel)
+    for warm in (0, 5, 10):
+        for _ in range(0 if warm == 0 else 5):
+            base.step(1.0)
+        git, rel = base.all_positions()
+        on, off = O.Oracle(O.make_params(restrict_radial=1, **kw)), O.Oracle(O.make_params(**kw))
+        for o in (on, off):  # both count steps from 0: the same Philox draws
+            o.set_all(git, rel)
+        on.step(1.0)
+        off.step(1.0)
+        xa, xb = on.absolute(), off.absolute()
+        assert np.array_equal(xa[:n1], xb[:n1])
+        ra = np.hypot(xa[n1:, 0] - m, xa[n1:, 1] - m)
+        rb = np.hypot(xb[n1:, 0] - m, xb[n1:, 1] - m)
+        assert np.allclose(ra, rb, rtol=0, atol=1e-9)
+        ang = np.arctan2(xa[n1:, 1] - m, xa[n1:, 0] - m)
+        assert np.abs(np.angle(np.exp(1j * (ang - want)))).max() < 1e-9
+        # the switch does move species 2 (off its free move, onto the ray)
+        assert np.abs(xa[n1:] - xb[n1:]).max() > 1e-6
+        on.close()
+        off.close()
