@@ -466,6 +466,7 @@ class RunDynamik:
         self._chk(self.L.bdk_pair_correlation(self.h, nbins, dr, _p(out[0]), _p(out[1]), _p(out[2])))
         return out
 
+
     def rho_k(self, species, dq_ratio=0.5, nbins=None):
         """record_rhoFFTW (auswertung.cpp:529-626): shell averages of the
         centred FFT of one species' density; returns (re, im)."""

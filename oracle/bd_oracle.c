@@ -721,6 +721,7 @@ void bdo_get_mesh(const bdo_state *s, int which, double *out) {
 }
 
 int bdo_get_list(const bdo_state *s, int species, int cx, int cy, int32_t *out, int cap) {
+    if (species < 1 || species > 2 || cx < 0 || cx >= s->nc || cy < 0 || cy >= s->nc) return -1;
     const ilist *l = &s->nb[species - 1][cx * s->nc + cy];
     for (int i = 0; i < l->n && i < cap; ++i) out[i] = l->a[i];
     return l->n;
@@ -739,6 +740,7 @@ int bdo_write_pos_header(const char *path) {
 /* schreibe_pos (au:631-648) for one species */
 int bdo_append_pos(const bdo_state *s, int species, const char *path, int nr, double obs_dt) {
     int sp = species - 1;
+    if (sp < 0 || sp > 1) return -1;
     FILE *f = fopen(path, "a");
     if (!f) return -1;
     for (int i = 0; i < nsp(s, sp); ++i) {

@@ -144,6 +144,7 @@ def per_component(a, b):
     return out
 
 
+@pytest.mark.timeout(900)  # the first case builds the oracle's forces (three processes, ~40 s each)
 @pytest.mark.parametrize("scheme", list(SCHEMES))
 def test_config2_forces_every_rung_vs_oracle(c2_state, scheme):
     say("case: config-2 forces, %s, 4M particles, Z=4096" % scheme)
@@ -169,6 +170,7 @@ def test_config2_forces_every_rung_vs_oracle(c2_state, scheme):
         assert np.array_equal(forces[rung][1], forces["fp32"][1])
 
 
+@pytest.mark.timeout(900)
 @pytest.mark.parametrize("T", [0.0, 1.0])
 def test_config2_rung_drift_vs_fp64(c2_state, T):
     say("case: config-2 drift of the FP32 rungs, TSC, kT=%g, 100 steps" % T)

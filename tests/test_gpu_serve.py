@@ -99,10 +99,15 @@ def test_runaway_flag_cleared_by_new_positions():
     s.set_positions(git, rel)  # a new state starts without the flag
     s.run_steps(3)
     g1, r1 = s.positions()
+    # the same new state on a handle at the same step counter (the Philox
+    # draws are keyed by it): the same trajectory
     t = bdk.RunDynamik(bp)
     t.set_positions(git, rel)
     t.run_steps(3)
+    t.set_positions(git, rel)
+    t.run_steps(3)
     g2, r2 = t.positions()
+    assert s.step_counter() == t.step_counter() == 6
     assert np.array_equal(g1, g2) and np.array_equal(r1, r2)
 
 
