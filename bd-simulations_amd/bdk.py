@@ -113,6 +113,7 @@ def lib():
             "bdk_radial_profile": (C.c_int, [vp, i32, dbl, vp, vp]),
             "bdk_rho_k": (C.c_int, [vp, i32, dbl, i32, vp, vp]),
             "bdk_pair_correlation": (C.c_int, [vp, i32, dbl, vp, vp, vp]),
+            "bdk_pair_correlation_noalloc": (C.c_int, [vp, i32, dbl, vp, vp, vp]),
             "bdk_batch_create": (C.c_int, [vp, i32, C.POINTER(vp)]),
             "bdk_batch_destroy": (None, [vp]),
             "bdk_batch_last_error": (C.c_char_p, [vp]),
@@ -147,7 +148,7 @@ def exported_symbols():
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos", "bdk_write_pos_async", "bdk_write_wait",
             "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
-            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_density_grid", "bdk_density_stats", "bdk_ftrho_sums", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
+            "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_pair_correlation_noalloc", "bdk_density_grid", "bdk_density_stats", "bdk_ftrho_sums", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_status", "bdk_batch_tune", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
@@ -466,6 +467,11 @@ class RunDynamik:
         self._chk(self.L.bdk_pair_correlation(self.h, nbins, dr, _p(out[0]), _p(out[1]), _p(out[2])))
         return out
 
+    def pair_correlation_noalloc(self, nbins, dr):
+        """The same through bdk_pair_correlation_noalloc (no device allocation)."""
+        out = [np.zeros(nbins) for _ in range(3)]
+        self._chk(self.L.bdk_pair_correlation_noalloc(self.h, nbins, dr, _p(out[0]), _p(out[1]), _p(out[2])))
+        return out
 
     def rho_k(self, species, dq_ratio=0.5, nbins=None):
         """record_rhoFFTW (auswertung.cpp:529-626): shell averages of the

@@ -29,7 +29,7 @@ def run(cmd):
 def build_lib():
     objs = []
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    for src in ("bdk_kernels.hip", "bdk_api.cpp", "bdk_serve.cpp"):
+    for src in ("bdk_kernels.hip", "bdk_noalloc.hip", "bdk_api.cpp", "bdk_serve.cpp"):
         obj = os.path.join(HERE, "_build", src + ".o")
         extra = ["-x", "hip"] if src.endswith(".hip") else []
         run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj])

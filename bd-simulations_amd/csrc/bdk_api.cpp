@@ -545,8 +545,9 @@ int bdk_create(const bd_params *p, bdk_t **out) {
     h->p = *p;
     h->dev = p->device;
     hipError_t e = hipSetDevice(h->dev);
+    if (e == hipSuccess) e = bdkx::noalloc_setup(h->dev);  // once per device: the no-allocation entry's scratch
     if (e != hipSuccess) {
-        int rc = hipfail(nullptr, e, "hipSetDevice");
+        int rc = hipfail(nullptr, e, "hipSetDevice / scratch address");
         delete h;
         return rc;
     }

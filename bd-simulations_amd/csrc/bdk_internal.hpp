@@ -227,6 +227,9 @@ hipError_t launch_ftrho(bdk_t *h, const int2 *ij, int npairs, int species, doubl
                         double *im);
 int ftrho_chunks(const bdk_t *h);
 hipError_t set_kernel_attributes();
+// device address of the allocation-free entry's scratch array on device
+// `dev` (current), looked up once per device (bdk_noalloc.hip)
+hipError_t noalloc_setup(int dev);
 hipError_t build_tables(bdk_t *h);                // spectral tables in device memory
 
 // batches (a batch is a view over member handles, each with its own buffers)

@@ -282,6 +282,16 @@ int bdk_radial_profile(bdk_t *h, int32_t nbins, double dr, double *centers, doub
    writes it), 2/((N2-1) pi rho2 dr^2) (22), rho_a = N_a/L^2.  Any of the
    three outputs may be null. */
 int bdk_pair_correlation(bdk_t *h, int32_t nbins, double dr, double *g11, double *g12, double *g22);
+/* The same histograms without allocating anything: no counterpart in the
+   reference; for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   counts go through a fixed scratch array in device memory, one per GPU
+   (96 KiB, kept until the process ends), and are written straight into the
+   caller's arrays.  Limits: 1 <= nbins <= 4096; all three outputs are
+   required; the particle state must already be on the device (any force,
+   step or refresh call after bdk_set_positions), else BDK_ERR_STATE.  Calls
+   that overlap on one GPU (several host threads) race on the array. */
+int bdk_pair_correlation_noalloc(bdk_t *h, int32_t nbins, double dr, double *g11, double *g12, double *g22);
 
 /* rho(k) of one species (record_rhoFFTW, auswertung.cpp:529-626): the number
    density of `species` on the mesh (the force deposit's scheme and weights

@@ -21,7 +21,7 @@ ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
 
 @pytest.fixture(scope="module")
 def built():
-    objs = [os.path.join(ROOT, "bd-simulations_amd", "_build", n) for n in ("bdk_kernels.hip.o",)]
+    objs = [os.path.join(ROOT, "bd-simulations_amd", "_build", n) for n in ("bdk_kernels.hip.o", "bdk_noalloc.hip.o")]
     if not all(os.path.exists(o) for o in objs):
         subprocess.check_call([sys.executable, os.path.join(ROOT, "bd-simulations_amd", "build.py")])
     out = subprocess.run(["make", "-s", "-C", SAN], capture_output=True, text=True, timeout=600)
