@@ -52,6 +52,9 @@ template <typename R> struct V4;
 template <> struct V4<float> { using T = float4; };
 template <> struct V4<double> { using T = double4; };
 
+// Complex helpers of the FFT and spectral arithmetic.  The complex product
+// uses explicit fused multiply-adds (the file compiles without automatic
+// contraction, so every kernel that inlines it rounds the same way).
 template <typename C> __device__ __forceinline__ C cm(decltype(C::x) a, decltype(C::x) b) {
     C r;
     r.x = a;
@@ -62,7 +65,7 @@ template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cm<C>
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cm<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cconj(C a) { return cm<C>(a.x, -a.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-    return cm<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return cm<C>(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 template <typename C> __device__ __forceinline__ C cmul_i(C a) { return cm<C>(-a.y, a.x); }    // i*a
 template <typename C> __device__ __forceinline__ C cmul_mi(C a) { return cm<C>(a.y, -a.x); }   // -i*a
@@ -1041,6 +1044,8 @@ __device__ __forceinline__ void symbols(int j, int mm, const Geo &g, const R *sq
     R G;
     if (j == 0 && mm == 0 && g.zero_mode_g)
         G = (R)0.0;
+    else if (sizeof(R) == 4)  // FP32: the hardware reciprocal (1 ulp) instead of a correctly rounded division
+        G = (R)__builtin_amdgcn_rcpf((float)((lap[j] + lap[mm]) + (R)g.lam2inv));
     else
         G = (R)1.0 / ((lap[j] + lap[mm]) + (R)g.lam2inv);
     // x / Z / Z == x * 2^(-2 log2 Z) exactly (Z is a power of two)
@@ -1200,18 +1205,20 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
     const int kx[3] = {(k - 1 + Z) & (Z - 1), k, (k + 1) & (Z - 1)};
     const int ly[3] = {(l - 1 + Z) & (Z - 1), l, (l + 1) & (Z - 1)};
     R ax = 0, ay = 0;
+    // weighted sums of the mesh forces with explicit fused multiply-adds
+    // (the weights and indices above follow the reference's rounding)
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const R2 *row = fm + (size_t)kx[a] * Z;
         R2 f0 = row[ly[0]], f1 = row[ly[1]], f2 = row[ly[2]];
-        R sx = wy[0] * f0.x + wy[1] * f1.x + wy[2] * f2.x;
-        R sy = wy[0] * f0.y + wy[1] * f1.y + wy[2] * f2.y;
+        const R sx = fma(wy[2], f2.x, fma(wy[1], f1.x, wy[0] * f0.x));
+        const R sy = fma(wy[2], f2.y, fma(wy[1], f1.y, wy[0] * f0.y));
         if (a == 0) {
             ax = wx[0] * sx;
             ay = wx[0] * sy;
         } else {
-            ax += wx[a] * sx;
-            ay += wx[a] * sy;
+            ax = fma(wx[a], sx, ax);
+            ay = fma(wx[a], sy, ay);
         }
     }
     if (s2) {

@@ -4,14 +4,16 @@ kT = 1, dt = 1e-4, FP32 forces with FP64 positions), synthetic clipped
 Gaussian-blob start (thinned hexagonal lattice of spacing 0.95 sigma, so every pair
 is farther apart than the 0.9 sigma exclusion).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ensemble]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ensemble] [--no-config0]
 
 Prints one JSON line.  The timed region is K graph-replayed steps with the
 particle state resident in HBM, bracketed by stream synchronisation.  The
 line also carries the ensemble figures: a batched mix of 256 uneven
 replicas (one launch per stage) against the plain one-at-a-time GPU path,
 and a served Poisson job stream (continuous batching) with p50/p99 job
-latency; see bd-simulations_amd/ensemble.py for the mixes.
+latency; see bd-simulations_amd/ensemble.py for the mixes; and the CPU
+reference configuration (2000 particles, CIC 128^2, 10,000 FP64 steps) on
+the GPU beside the FP64 CPU restatement on one host thread.
 """
 import argparse
 import json
@@ -62,6 +64,31 @@ def canonical_step_bytes(N, Z):
     written by the deposit, read and written by a real-to-complex and two
     complex-to-real transforms, and read by the gather."""
     return 64 * N + 36 * Z * Z
+
+
+# the kernels of one graph-replayed step (fixed dt, FP32 forces, TSC)
+STEP_KERNELS = ("k_update", "k_rank_scan", "k_fill", "k_sort_cells", "k_wca_f32", "k_deposit", "k_r2c_rows",
+                "k_col", "k_c2r_rows")
+# bench names of the kernels timed one at a time -> names in the profiler's records
+TRACE_NAME = {"k_wca": "k_wca_f32"}
+
+
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "usable_cpus": usable}
 
 
 def load_traffic(kernel):
@@ -135,9 +162,11 @@ def cpu_baseline(p, git, rel, budget_s=20.0):
     dt = time.perf_counter() - t0
     o.close()
     N = p.n1 + p.n2
+    hc = host_cpu()
     return {"value": N * n / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": "FP64 C restatement, 1 thread, %d steps after 1 warm-up of the same 2M-particle "
-                      "TSC 2048^2 workload (%.1f s); host %s" % (n, dt, platform.processor() or platform.machine())}
+                      "TSC 2048^2 workload (%.1f s); host %s, %d CPUs (%d usable)"
+                      % (n, dt, hc["cpu_model"], hc["nproc"], hc["usable_cpus"]), **hc}
 
 
 def cpu_ensemble_baseline(mix, budget_s=10.0):
@@ -160,11 +189,69 @@ def cpu_ensemble_baseline(mix, budget_s=10.0):
     with mp.get_context("spawn").Pool(len(jobs)) as workers:
         res = workers.map(pool.run_replica, jobs)
     value = sum(ps / t for ps, t in res)
+    hc = host_cpu()
     return {"value": value, "unit": "particle-steps/s", "cores": len(jobs), "kind": "port",
             "sample": "FP64 C restatement, %d processes side by side, each one replica of the batched mix "
                       "(the first %d, %d particles in all) stepped %.0f s after 1 warm-up step; host %s"
                       % (len(jobs), len(jobs), sum(j[0]["n1"] + j[0]["n2"] for j in jobs), budget_s,
-                         platform.processor() or platform.machine())}
+                         hc["cpu_model"]), **hc}
+
+
+CONFIG0 = dict(n1=1000, n2=1000, L=64.0, Z=128, scheme=1, lam=8.0, kap=1.0, f2_f1=0.5, T=1.0, dt=1e-4, seed=5,
+               steps=10000)
+
+
+def config0_figures(bdk, cpu=True):
+    """The reference's CPU configuration (N1 = N2 = 1000, L = 64, CIC 128^2,
+    lambda 8, q2/q1 = 0.5, kT = 1, dt = 1e-4 fixed, 10,000 steps, FP64, RSA
+    start with 0.9 exclusion): the GPU in FP64 mode, 10,000 graph-replayed
+    steps, and with cpu the FP64 oracle on one host thread over the same
+    10,000 steps (the reference's single-threaded run)."""
+    c = CONFIG0
+    p = bdk.make_params(n1=c["n1"], n2=c["n2"], L=c["L"], Z=c["Z"], scheme=c["scheme"], lam=c["lam"], kap=c["kap"],
+                        f2_f1=c["f2_f1"], T=c["T"], dt_max=c["dt"], precision=bdk.FP64, noise_mode=bdk.NOISE_DISC,
+                        seed=c["seed"])
+    git, rel = bdk.init_uniform(p, c["seed"], 0.9)
+    N = c["n1"] + c["n2"]
+    log("config0: GPU, FP64, %d steps" % c["steps"])
+    sim = bdk.RunDynamik(p)
+    sim.set_positions(git, rel)
+    sim.run_steps(100)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run_steps(c["steps"])
+    sim.sync()
+    gpu_s = time.perf_counter() - t0
+    sim.close()
+    out = {"workload": "N1 = N2 = 1000, L = 64, CIC 128^2, lambda 8, kT 1, dt 1e-4 fixed, FP64, RSA start",
+           "steps": c["steps"], "gpu_value": N * c["steps"] / gpu_s, "gpu_ms_per_step": gpu_s / c["steps"] * 1e3,
+           "unit": "particle-steps/s"}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.build()
+        op = O.make_params(n1=c["n1"], n2=c["n2"], L=c["L"], Z=c["Z"], scheme=c["scheme"], lam=c["lam"],
+                           kap=c["kap"], T=c["T"], dt_max=c["dt"], mrd=float("inf"), f2_f1=c["f2_f1"],
+                           noise_mode=1, seed=c["seed"])
+        o = O.Oracle(op)
+        o.set_all(git, rel)
+        log("config0: CPU oracle, FP64, 1 thread, %d steps" % c["steps"])
+        t0 = time.perf_counter()
+        for k in range(c["steps"]):
+            o.step(1.0)
+            if k % 2000 == 1999:
+                log("config0: CPU step %d" % (k + 1))
+        cpu_s = time.perf_counter() - t0
+        o.close()
+        hc = host_cpu()
+        out["cpu_baseline"] = {"value": N * c["steps"] / cpu_s, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port", "seconds": cpu_s,
+                               "sample": "FP64 C restatement, 1 thread, all %d steps; host %s" % (c["steps"],
+                                                                                                  hc["cpu_model"]),
+                               **hc}
+        out["gpu_over_cpu"] = out["gpu_value"] / out["cpu_baseline"]["value"]
+    return out
 
 
 def ensemble_figures(log_fn, cpu=False):
@@ -234,6 +321,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched and served figures")
+    ap.add_argument("--no-config0", action="store_true", help="skip the CPU reference configuration's figures")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--kernel-probe", type=int, default=0, metavar="REPS",
                     help="run each kernel REPS times plus a calibration copy and exit (for --pmc runs)")
@@ -276,14 +364,25 @@ def main():
     for k in ("k_wca", "k_deposit", "k_r2c_rows", "k_col", "k_c2r_rows", "k_update"):
         kernels[k] = sim.profile_kernel(k, args.profile_steps)
     step_ms_events = sim.profile_kernel("step", args.profile_steps)
-    dom = max(kernels, key=kernels.get)
-    ab = algorithmic_bytes(dom, N, w["Z"])
-    achieved = ab / (kernels[dom] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(dom),
-            "algorithmic_bytes": ab, "avg_ms": kernels[dom]}
     own_bytes = sum(algorithmic_bytes(k, N, w["Z"]) for k in kernels)
     canon = canonical_step_bytes(N, w["Z"])
+    # The headline roofline is one fixed quantity: the whole graph-replayed
+    # step (every kernel of it, on both streams) against the path's
+    # canonical compulsory bytes, 64 N + 36 Z^2, timed with HIP events over
+    # back-to-back step launches on the simulation's stream; its traffic is
+    # the measured HBM bytes of all the step's kernels (profiles/).
+    tr = [load_traffic(k) for k in STEP_KERNELS]
+    achieved = canon / (step_ms_events * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "whole step (graph replay of " + ", ".join(STEP_KERNELS) + ")",
+            "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+            "traffic": sum(tr) if all(t is not None for t in tr) else None,
+            "algorithmic_bytes": canon, "avg_ms": step_ms_events}
+    # the fused force + update path (gather, noise, Euler step, refresh) on its own
+    fu_ab = algorithmic_bytes("k_update", N, w["Z"])
+    fu = fu_ab / (kernels["k_update"] * 1e-3) / 1e9
+    fused = {"bound": "hbm", "kernel": "k_update", "achieved": fu, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+             "frac": fu / PEAK_HBM_GBS, "traffic": load_traffic("k_update"), "algorithmic_bytes": fu_ab,
+             "avg_ms": kernels["k_update"]}
     out = {
         "metric": "particle-steps/s at 2M particles, 2048\u00b2 TSC mesh; p50/p99 ensemble job latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -294,20 +393,27 @@ def main():
                    "scheme": "TSC", "L": w["L"], "lambda": w["lam"], "dt": w["dt"], "kT": w["T"],
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "fused_update_roofline": fused,
         "step_roofline": {"canonical_bytes": canon, "kernel_bytes": own_bytes,
                           "achieved_GBs": canon / (ms_per_step * 1e-3) / 1e9,
                           "frac": canon / (ms_per_step * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                          "step_ms_events": step_ms_events},
+                          "step_ms_wall": ms_per_step, "step_ms_events": step_ms_events},
         "kernels_ms": kernels,
         # every kernel against the HBM peak by its algorithmic bytes; k_update is
         # the fused gather + noise + Euler update (40 N + 8 Z^2 bytes)
         "kernels_roofline": {k: {"algorithmic_bytes": algorithmic_bytes(k, N, w["Z"]),
                                  "achieved_GBs": algorithmic_bytes(k, N, w["Z"]) / (ms * 1e-3) / 1e9,
-                                 "frac": algorithmic_bytes(k, N, w["Z"]) / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+                                 "frac": algorithmic_bytes(k, N, w["Z"]) / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                 "traffic": load_traffic(TRACE_NAME.get(k, k))}
                              for k, ms in kernels.items()},
+        # measured HBM bytes of the list-rebuild kernels (implementation
+        # overhead outside the canonical bytes)
+        "rebuild_traffic": {k: load_traffic(k) for k in ("k_rank_scan", "k_fill", "k_sort_cells")},
     }
     sim.close()
     out["adaptive_dt"] = adaptive_figure(bdk, w, git, rel, args.steps, args.warmup, N)
+    if not args.no_config0:
+        out["config0"] = config0_figures(bdk, cpu=not args.no_cpu_baseline)
     if not args.no_ensemble:
         out["batched"], out["served"] = ensemble_figures(log, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
