@@ -1944,7 +1944,6 @@ int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {
         if (h->dev != mem[0]->dev) return fail(nullptr, BDK_ERR_ARG, who + "members must share the device");
         if (h->p.precision != mem[0]->p.precision)
             return fail(nullptr, BDK_ERR_ARG, who + "members must share the precision");
-        if (h->g.adaptive) return fail(nullptr, BDK_ERR_ARG, who + "batches need a fixed dt (max_travel = +inf)");
         if (h->g.idgas) return fail(nullptr, BDK_ERR_ARG, who + "batches need capillary forces (idgas_only = 0)");
         if (h->p.noise_mode == BDK_NOISE_UPLOAD)
             return fail(nullptr, BDK_ERR_ARG, who + "batches do not take uploaded noise");

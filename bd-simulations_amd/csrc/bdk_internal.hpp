@@ -147,7 +147,8 @@ struct RepDev {
 };
 
 // Block maps of the batched stages: blockIdx.x -> (member, block of member).
-enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_COUNT };
+// BS_GATHER: particle blocks of the adaptive-dt members; BS_FOLD: one entry per adaptive member.
+enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_GATHER, BS_FOLD, BS_COUNT };
 
 struct BatchPlan {
     int nmem = 0;

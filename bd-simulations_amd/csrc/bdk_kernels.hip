@@ -1251,11 +1251,12 @@ __device__ __forceinline__ double2 particle_noise(const Geo &g, const DevBufs &d
     return n;
 }
 
-// forces only: fkap (and dt candidates when with_dt)
+// forces only: fkap (and dt candidates when with_dt) of block blk of list
+// positions; the block minimum of the candidates goes to dts slot `slot`
 template <typename R, int SCHEME>
-__global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
+__device__ __forceinline__ void gather_body(const Geo &g, const DevBufs &d, int with_dt, int blk, int slot) {
     using R2 = typename V2<R>::T;
-    const int s = xcd_block(g.xcd) * blockDim.x + threadIdx.x;  // list position
+    const int s = blk * blockDim.x + threadIdx.x;  // list position
     double cand = INFINITY;
     if (s < g.N) {
         const int c = d.lcell[s];
@@ -1293,18 +1294,23 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
             for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
                 if (wmin[w] < cand) cand = wmin[w];
             if (cand < INFINITY && cand >= 0.0) {
-                unsigned long long *slot = &d.dts[(blockIdx.x & 63) * 16];
+                unsigned long long *sp = &d.dts[(slot & 63) * 16];
                 const unsigned long long b = bitsd(cand);
-                if (b < *(volatile unsigned long long *)slot) atomicMin(slot, b);
+                if (b < *(volatile unsigned long long *)sp) atomicMin(sp, b);
             }
         }
     }
 }
 
+template <typename R, int SCHEME>
+__global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
+    gather_body<R, SCHEME>(g, d, with_dt, xcd_block(g.xcd), blockIdx.x);
+}
+
 // dt of the step: dt_max and the 64 slot minima of k_gather folded into
 // sc->dt_bits and logged by step; the slots are re-armed (all ones) for the
 // next reduction.  One wave.
-__global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) {
+__device__ __forceinline__ void dt_fold_body(const Geo &g, const DevBufs &d) {
     const int t = threadIdx.x;
     unsigned long long m = d.dts[t * 16];
     d.dts[t * 16] = ~0ull;
@@ -1320,6 +1326,8 @@ __global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) {
         d.dtlog[d.sc->step & (kDtLog - 1)] = dbits(dt);
     }
 }
+
+__global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) { dt_fold_body(g, d); }
 
 // Euler step (dm:116-157) + refresh (dm:216-325) + cell counting of list
 // position s (written to slot s).  Returns how far the stored new position
@@ -1796,15 +1804,44 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int
     c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb2, r.sR2, t.y);
 }
 
+// Members with a fixed dt gather the mesh force inside the update; members
+// with adaptive dt read the forces and the dt of kb_gather / kb_dt_fold, as
+// their single-simulation steps do.
 template <typename R>
 __global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *map) {
     const int2 t = map[blockIdx.x];
     const RepDev &r = m[t.x];
+    if (r.g.adaptive) {
+        switch (r.g.scheme) {
+        case 0: update_body<R, 0, false>(r.g, r.d, t.y); break;
+        case 1: update_body<R, 1, false>(r.g, r.d, t.y); break;
+        default: update_body<R, 2, false>(r.g, r.d, t.y); break;
+        }
+        return;
+    }
     switch (r.g.scheme) {
     case 0: update_body<R, 0, true>(r.g, r.d, t.y); break;
     case 1: update_body<R, 1, true>(r.g, r.d, t.y); break;
     default: update_body<R, 2, true>(r.g, r.d, t.y); break;
     }
+}
+
+// capillary forces and dt candidates of the adaptive members (their
+// particle blocks only), then one wave per adaptive member folds its dt
+template <typename R>
+__global__ __launch_bounds__(256) void kb_gather(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    const RepDev &r = m[t.x];
+    switch (r.g.scheme) {
+    case 0: gather_body<R, 0>(r.g, r.d, 1, t.y, blockIdx.x); break;
+    case 1: gather_body<R, 1>(r.g, r.d, 1, t.y, blockIdx.x); break;
+    default: gather_body<R, 2>(r.g, r.d, 1, t.y, blockIdx.x); break;
+    }
+}
+
+__global__ __launch_bounds__(64) void kb_dt_fold(const RepDev *m, const int2 *map) {
+    const int2 t = map[blockIdx.x];
+    dt_fold_body(m[t.x].g, m[t.x].d);
 }
 
 __global__ __launch_bounds__(256) void kb_rank_scan(const RepDev *m, const int2 *mp, int nrank, const int2 *ms) {
@@ -2823,8 +2860,10 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchP
         bp.l2 = std::max(bp.l2, (size_t)r.rb2 * r.sR2 * csz);
         r.scan_nb = (int)nblk(h->g.ncell, SCAN_BLOCK);
         r.unit = h->p.sigma11_22 == 1.0 && h->p.sigma11_12 == 1.0 && h->p.eps22_11 == 1.0 && h->p.eps12_11 == 1.0;
+        const int ad = h->g.adaptive ? 1 : 0;
         const int nb[BS_COUNT] = {(int)nblk(h->g.N, 256), (int)nblk(h->g.ncell, 256), r.scan_nb,
-                                  (int)(pl.grid.x * pl.grid.y), Z / r.rb, (Z / 2) / r.rc, Z / r.rb2};
+                                  (int)(pl.grid.x * pl.grid.y), Z / r.rb, (Z / 2) / r.rc, Z / r.rb2,
+                                  ad * (int)nblk(h->g.N, 256), ad};
         for (int k = 0; k < BS_COUNT; ++k)
             for (int b = 0; b < nb[k]; ++b) maps[k].push_back(make_int2(i, b));
     }
@@ -2873,6 +2912,12 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
     case 5:
         hipLaunchKernelGGL(kb_update<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
         break;
+    case 13:  // adaptive members: forces, dt candidates and the fold
+        if (bp.nblk[BS_GATHER] > 0) {
+            hipLaunchKernelGGL(kb_gather<R>, dim3(bp.nblk[BS_GATHER]), dim3(256), 0, st, m, bp.map[BS_GATHER]);
+            hipLaunchKernelGGL(kb_dt_fold, dim3(bp.nblk[BS_FOLD]), dim3(64), 0, st, m, bp.map[BS_FOLD]);
+        }
+        break;
     case 6:
         hipLaunchKernelGGL(kb_rank_scan, dim3(bp.nblk[BS_PART] + bp.nblk[BS_SCAN]), dim3(256), 0, st, m,
                            bp.map[BS_PART], bp.nblk[BS_PART], bp.map[BS_SCAN]);
@@ -2886,7 +2931,8 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
 }
 
 // stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update,
-// 6 rank + scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end
+// 6 rank + scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end,
+// 13 gather + dt fold of the adaptive members
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)
         batch_stage_t<float>(bp, stage, st);
@@ -2912,12 +2958,14 @@ hipError_t batch_enqueue_forces(const BatchPlan &bp, hipStream_t st, hipStream_t
     return e;
 }
 
-// One batched step, laid out like the single-simulation step: update, then
+// One batched step, laid out like the single-simulation step: (the adaptive
+// members' gather and dt fold,) update, then
 // the rebuild and the WCA on st beside the slot-order deposit and the FFT
 // stages on aux (the last of which ends the step), joined.
 hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
                               hipEvent_t join) {
-    hipError_t e = batch_stage(bp, 5, st);
+    hipError_t e = bp.nblk[BS_GATHER] > 0 ? batch_stage(bp, 13, st) : hipSuccess;
+    if (e == hipSuccess) e = batch_stage(bp, 5, st);
     hipStream_t side = aux ? aux : st;
     if (aux && e == hipSuccess) e = hipEventRecord(fork, st);
     if (aux && e == hipSuccess) e = hipStreamWaitEvent(aux, fork, 0);

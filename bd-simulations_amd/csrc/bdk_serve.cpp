@@ -111,8 +111,8 @@ int bdk_serve_submit(bdk_server_t *s, const bd_params *p, const int32_t *git, co
     int rc = bdk_footprint(p, &bytes);
     if (rc) return s->fail(rc, std::string("submit: ") + bdk_last_error(nullptr));
     if (bytes > s->budget) return s->fail(BDK_ERR_ARG, "submit: the job's footprint exceeds the HBM budget");
-    if (p->max_travel < 1e300 || p->idgas_only || p->noise_mode == BDK_NOISE_UPLOAD)
-        return s->fail(BDK_ERR_ARG, "submit: served jobs need fixed dt, capillary forces and device noise");
+    if (p->idgas_only || p->noise_mode == BDK_NOISE_UPLOAD)
+        return s->fail(BDK_ERR_ARG, "submit: served jobs need capillary forces and device noise");
     Job *j = new Job();
     j->id = s->next_id++;
     j->p = *p;

@@ -340,8 +340,11 @@ int bdk_ftrho_sums(bdk_t *h, int32_t species, int32_t qbins, double dq_ft, doubl
 
    A batch is a view over member handles made with bdk_create and given
    positions; it does not own them.  Members share the device and the
-   precision, use a fixed dt (max_travel = +inf), have capillary forces on
-   (idgas_only = 0) and do not use BDK_NOISE_UPLOAD.  Each member's state
+   precision, have capillary forces on (idgas_only = 0) and do not use
+   BDK_NOISE_UPLOAD.  Members may have a fixed dt (max_travel = +inf) or
+   the adaptive dt of optimaler_zeitschritt (dm:1420-1491), each member its
+   own dt per step; bdk_run_interval on a member afterwards keeps the
+   reference's long-double clock from its own logged dt's.  Each member's state
    stays in its own buffers: after bdk_batch_sync the ordinary per-handle
    calls (bdk_get_positions, bdk_write_pos, ...) read it, and each member's
    trajectory is bit-identical to running that handle alone.  A member must
@@ -376,9 +379,9 @@ int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_pe
    most `max_jobs` run, advances every running job by
    min(tick_steps, smallest remaining step count) steps in one batch, and
    retires the jobs that are done (positions copied back, device state
-   freed).  Served jobs use fixed dt, capillary forces and device noise;
-   jobs of different precision advance in one batch per precision, side by
-   side on the device.
+   freed).  Served jobs use capillary forces and device noise, with a fixed
+   or an adaptive dt (max_travel); jobs of different precision advance in
+   one batch per precision, side by side on the device.
    Each job's result equals running it alone with bdk_run_steps.  A job
    that cannot be admitted (bdk_create or the upload fails) or whose
    particles run away is finished as failed and the others go on;

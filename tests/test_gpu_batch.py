@@ -82,16 +82,60 @@ def test_batch_profile_and_errors():
     for st in ("k_wca", "k_deposit", "k_r2c_rows", "k_col", "k_c2r_rows", "step"):
         assert b.profile(st, 3) > 0.0
     b.close()
-    # mixed precision, adaptive dt and uploaded noise are refused
+    # mixed precision, uploaded noise and a handle given twice are refused
     bp64, git, rel = make(bdk.FP64, MIX[0])
     s64 = bdk.RunDynamik(bp64)
     s64.set_positions(git, rel)
     with pytest.raises(bdk.BdkError):
         bdk.Batch([members[0], s64])
-    bpa, _ = params(n1=400, n2=400, L=40.0, Z=64, max_travel=0.1, precision=bdk.FP32)
-    sa = bdk.RunDynamik(bpa)
-    sa.set_positions(git, rel)
+    bpu, _ = params(n1=400, n2=400, L=40.0, Z=64, noise_mode=bdk.NOISE_UPLOAD, precision=bdk.FP32)
+    su = bdk.RunDynamik(bpu)
+    su.set_positions(git, rel)
     with pytest.raises(bdk.BdkError):
-        bdk.Batch([sa])
+        bdk.Batch([su])
     with pytest.raises(bdk.BdkError):
         bdk.Batch([members[0], members[0]])
+
+
+@pytest.mark.parametrize("prec", [bdk.FP32, bdk.FP64])
+def test_batch_adaptive_members_equal_alone(prec):
+    """Members with the adaptive dt of optimaler_zeitschritt (travel caps
+    0.05 and 0.02, each member its own dt per step) beside fixed-dt members:
+    each member's trajectory and its own clock (the long-double sum of its
+    logged dt's, read back through bdk_run_interval_clock afterwards) equal
+    its run alone."""
+    say("case: batch with adaptive-dt members prec=%d" % prec)
+    caps = [0.05, None, 0.02, None, 0.05, 0.02]
+
+    def make_all():
+        out = []
+        for spec, cap in zip(MIX, caps):
+            n1, n2, L, Z, scheme, lam, seed = spec
+            # adaptive members: dt_max 2e-3, so that the travel cap binds;
+            # fixed-dt members at the usual 1e-4
+            kw = dict(n1=n1, n2=n2, L=L, Z=Z, scheme=scheme, lam=lam, precision=prec, seed=seed, f2_f1=0.5,
+                      T=2.0, dt_max=1e-4 if cap is None else 2e-3)
+            if cap is not None:
+                kw["max_travel"] = cap
+            bp, _ = params(**kw)
+            git, rel = positions(bp, seed, 0.9)
+            s = bdk.RunDynamik(bp)
+            s.set_positions(git, rel)
+            out.append(s)
+        return out
+
+    alone, members = make_all(), make_all()
+    b = bdk.Batch(members)
+    b.run_steps(29)
+    b.sync()
+    for s in alone:
+        s.run_steps(29)
+    for i, (a, m) in enumerate(zip(alone, members)):
+        ga, ra = a.positions()
+        gm, rm = m.positions()
+        assert np.array_equal(ga, gm) and np.array_equal(ra, rm), "member %d (cap %s)" % (i, caps[i])
+    b.close()
+    # the members go on alone: their next interval (clock included) equals
+    for i, (a, m) in enumerate(zip(alone, members)):
+        assert a.run_interval_clock(0.02) == m.run_interval_clock(0.02), "member %d" % i
+        assert np.array_equal(a.positions()[1], m.positions()[1])

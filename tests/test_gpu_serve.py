@@ -57,16 +57,69 @@ def test_served_jobs_equal_alone():
     srv.close()
 
 
-def test_server_rejects_oversized_and_adaptive_jobs():
+def test_server_rejects_oversized_and_uploaded_noise_jobs():
     bp, git, rel, steps = job(JOBS[0])
     srv = bdk.Server(hbm_budget=bdk.footprint(bp) // 2, max_jobs=2, tick_steps=10)
     with pytest.raises(bdk.BdkError):
         srv.submit(bp, git, rel, steps)
     srv.close()
-    bpa, _ = params(n1=400, n2=400, L=40.0, Z=64, max_travel=0.1, precision=bdk.FP32)
+    bpu, _ = params(n1=400, n2=400, L=40.0, Z=64, noise_mode=bdk.NOISE_UPLOAD, precision=bdk.FP32)
     srv = bdk.Server(hbm_budget=1 << 34, max_jobs=2, tick_steps=10)
     with pytest.raises(bdk.BdkError):
-        srv.submit(bpa, git, rel, 10)
+        srv.submit(bpu, git, rel, 10)
+    srv.close()
+
+
+def test_served_adaptive_jobs():
+    """Jobs with the reference's adaptive dt (max_reisedistanz, travel caps
+    0.24 and 0.05; dt_max 2e-3 so that the cap binds) served beside
+    fixed-dt jobs (dt 1e-4), FP32 and FP64 together (one batch per precision): every
+    job equals its run alone, and the FP64 adaptive job follows the oracle's
+    trajectory (the same Philox draws), cells identical and positions within
+    1e-10."""
+    import oracle as O
+    print("case: served adaptive-dt jobs", flush=True)
+    specs = [  # n1, n2, L, Z, scheme, lam, seed, steps, cap, precision
+        (400, 400, 40.0, 64, bdk.CIC, 5.0, 41, 90, 0.24, bdk.FP32),
+        (600, 300, 48.0, 64, bdk.TSC, 9.0, 42, 60, None, bdk.FP32),
+        (300, 500, 40.0, 64, bdk.TSC, 12.0, 43, 120, 0.05, bdk.FP32),
+        (500, 500, 45.0, 64, bdk.CIC, 8.0, 44, 80, 0.05, bdk.FP64),
+        (700, 200, 50.0, 128, bdk.NGP, 20.0, 45, 40, None, bdk.FP64),
+    ]
+    jobs, oracles = [], {}
+    for k, (n1, n2, L, Z, scheme, lam, seed, steps, cap, prec) in enumerate(specs):
+        kw = dict(n1=n1, n2=n2, L=L, Z=Z, scheme=scheme, lam=lam, precision=prec, seed=seed, T=2.0,
+                  dt_max=1e-4 if cap is None else 2e-3)
+        if cap is not None:
+            kw["max_travel"] = cap
+        bp, op = params(**kw)
+        git, rel = positions(bp, seed, 0.9)
+        jobs.append((bp, git, rel, steps))
+        if prec == bdk.FP64 and cap is not None:
+            oracles[k] = op
+    srv = bdk.Server(hbm_budget=1 << 34, max_jobs=3, tick_steps=25)
+    ids = []
+    for k, j in enumerate(jobs):
+        ids.append(srv.submit(*j))
+        if k % 2:
+            srv.step()
+    for _ in range(200):
+        if srv.step()[2] == len(jobs):
+            break
+    for k, (jid, (bp, git, rel, steps)) in enumerate(zip(ids, jobs)):
+        g_srv, r_srv, _, _ = srv.result(jid)
+        s = bdk.RunDynamik(bp)
+        s.set_positions(git, rel)
+        s.run_steps(steps)
+        g_alone, r_alone = s.positions()
+        assert np.array_equal(g_srv, g_alone) and np.array_equal(r_srv, r_alone), "job %d" % k
+        if k in oracles:
+            o = O.Oracle(oracles[k])
+            o.set_all(git, rel)
+            for _ in range(steps):
+                o.step(1.0)
+            og, orl = o.all_positions()
+            assert np.array_equal(g_srv, og) and np.abs(r_srv - orl).max() < 1e-10, "job %d vs oracle" % k
     srv.close()
 
 
