@@ -121,6 +121,7 @@ def lib():
             "bdk_batch_sync": (C.c_int, [vp]),
             "bdk_batch_status": (C.c_int, [vp, i32, vp]),
             "bdk_batch_tune": (C.c_int, [vp, i32, vp]),
+            "bdk_batch_set_tuning": (C.c_int, [vp, vp]),
             "bdk_batch_profile": (C.c_int, [vp, i32, i32, vp]),
             "bdk_serve_create": (C.c_int, [i32, u64, i32, i64, C.POINTER(vp)]),
             "bdk_serve_destroy": (None, [vp]),
@@ -149,7 +150,7 @@ def exported_symbols():
             "bdk_init_uniform", "bdk_init_blob", "bdk_write_pos", "bdk_write_pos_async", "bdk_write_wait",
             "bdk_step_info", "bdk_read_startpos", "bdk_save_state", "bdk_load_state",
             "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_pair_correlation_noalloc", "bdk_density_grid", "bdk_density_stats", "bdk_ftrho_sums", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
-            "bdk_batch_sync", "bdk_batch_status", "bdk_batch_tune", "bdk_batch_profile",
+            "bdk_batch_sync", "bdk_batch_status", "bdk_batch_tune", "bdk_batch_set_tuning", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
             "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
 
@@ -540,10 +541,20 @@ class Batch:
         self._chk(self.L.bdk_batch_status(self.h, len(self.sims), _p(out)))
         return out
 
+    TUNE_KEYS = ("ept_r2c", "ept_col", "ept_c2r", "wca_block", "deposit_tile", "persist")
+
     def tune(self, reps=10):
-        out = (C.c_int32 * 3)()
+        """The autotuner's choice (bdk_batch_tune) as a dict."""
+        out = (C.c_int32 * len(self.TUNE_KEYS))()
         self._chk(self.L.bdk_batch_tune(self.h, reps, C.cast(out, C.c_void_p)))
-        return list(out)
+        return dict(zip(self.TUNE_KEYS, list(out)))
+
+    def set_tuning(self, **choice):
+        """bdk_batch_set_tuning; unnamed knobs keep the defaults (8, 8, 8, 256, 38, 0)."""
+        d = dict(ept_r2c=8, ept_col=8, ept_c2r=8, wca_block=256, deposit_tile=38, persist=0)
+        d.update(choice)
+        arr = (C.c_int32 * len(self.TUNE_KEYS))(*[d[k] for k in self.TUNE_KEYS])
+        self._chk(self.L.bdk_batch_set_tuning(self.h, C.cast(arr, C.c_void_p)))
 
     def profile(self, stage, reps=20):
         ms = C.c_double(0.0)

@@ -373,6 +373,7 @@ struct bdk_batch {
     int dev = 0;
     bdkx::BatchPlan bp;
     int ept_min[3] = {8, 8, 8};
+    bdkx::BatchTuning tuning;  // the autotuner's current choice
     hipGraph_t graph = nullptr, graph_multi = nullptr;
     hipGraphExec_t gexec = nullptr, gexec_multi = nullptr;
     std::string err;
@@ -409,18 +410,17 @@ struct bdk_batch {
         }
         return e;
     }
-    // device tables of the members for elements-per-thread choice ept
-    hipError_t rebuild(const int ept[3]) {
+    // device tables of the members for the tuning choice t
+    hipError_t rebuild(const bdkx::BatchTuning &t) {
         drop_graphs();
         free_tables();
         int zmax = 8;
         for (bdk_t *h : mem) zmax = std::max(zmax, h->g.Z);
         for (int k = 0; k < 3; ++k) ept_min[k] = bdkx::batch_min_ept(k, zmax);
-        int e3[3];
-        for (int k = 0; k < 3; ++k) e3[k] = std::max(ept[k], ept_min[k]);
+        tuning = t;
         std::vector<bdkx::RepDev> reps;
         std::vector<int2> maps[bdkx::BS_COUNT];
-        hipError_t e = bdkx::batch_build(mem, e3, bp, reps, maps);
+        hipError_t e = bdkx::batch_build(mem, t, bp, reps, maps);
         if (e == hipSuccess) e = upload_plan(bp, reps, maps);
         return e;
     }
@@ -463,7 +463,7 @@ struct bdk_batch {
             bdkx::BatchPlan sub;
             std::vector<bdkx::RepDev> reps;
             std::vector<int2> maps[bdkx::BS_COUNT];
-            e = bdkx::batch_build(need, bp.ept, sub, reps, maps);
+            e = bdkx::batch_build(need, bp.tune, sub, reps, maps);
             if (e == hipSuccess) e = upload_plan(sub, reps, maps);
             if (e == hipSuccess) e = bdkx::batch_enqueue_forces(sub, stream, aux, ev_fork, ev_join);
             // the tables are freed once the force stages that read them are done
@@ -1964,9 +1964,9 @@ int bdk_batch_create(bdk_t *const *members, int32_t n, bdk_batch_t **out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
-    int ept[3];
-    for (int k = 0; k < 3; ++k) ept[k] = bdkx::batch_min_ept(k, zmax);
-    if (e == hipSuccess) e = b->rebuild(ept);
+    bdkx::BatchTuning t;
+    for (int k = 0; k < 3; ++k) t.ept[k] = bdkx::batch_min_ept(k, zmax);
+    if (e == hipSuccess) e = b->rebuild(t);
     if (e != hipSuccess) {
         rc = fail(nullptr, BDK_ERR_HIP, std::string("bdk_batch_create: ") + hipGetErrorString(e));
         bdk_batch_destroy(b);
@@ -2071,37 +2071,95 @@ int bdk_batch_profile(bdk_batch_t *b, int32_t stage, int32_t reps, double *ms_pe
     return BDK_OK;
 }
 
-int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *ept_out) {
+int bdk_batch_set_tuning(bdk_batch_t *b, const int32_t *choice) {
+    if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
+    if (!choice) return b->fail(BDK_ERR_ARG, "null choice");
+    bdkx::BatchTuning t;
+    for (int k = 0; k < 3; ++k) {
+        const int v = choice[BDK_TUNE_EPT_R2C + k];
+        if (v != 8 && v != 16 && v != 32) return b->fail(BDK_ERR_ARG, "FFT elements per thread must be 8, 16 or 32");
+        t.ept[k] = v;
+    }
+    t.wca_block = choice[BDK_TUNE_WCA_BLOCK];
+    if (t.wca_block != 128 && t.wca_block != 256 && t.wca_block != 512)
+        return b->fail(BDK_ERR_ARG, "WCA block size must be 128, 256 or 512");
+    t.dep_tile = choice[BDK_TUNE_DEPOSIT_TILE];
+    if (t.dep_tile < 8 || t.dep_tile > 64) return b->fail(BDK_ERR_ARG, "deposit tile must be 8..64 mesh points");
+    t.persist = choice[BDK_TUNE_PERSIST];
+    if (t.persist < 0) return b->fail(BDK_ERR_ARG, "persistent grid must be >= 0");
+    hipError_t e = hipSetDevice(b->dev);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e == hipSuccess) e = b->rebuild(t);
+    if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_set_tuning: ") + hipGetErrorString(e));
+    return BDK_OK;
+}
+
+// The project's autotuner.  Each knob is timed on the batch's own members
+// with the stages that leave the particles unchanged, one knob at a time in
+// this order, keeping the fastest value: the FFT elements per thread of each
+// pass (8, 16, 32; stages 2-4), the WCA block size (128, 256, 512; stage 0),
+// the deposit tile (mesh points per tile side 24, 32, 38, 48; stage 1), and
+// the persistent grid (one block per map entry, or 4 or 8 blocks per compute
+// unit for every stage; stages 0-4 together).  No choice changes a result.
+int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *out) {
     if (!b) return fail(nullptr, BDK_ERR_ARG, "null batch");
     if (reps < 1) return b->fail(BDK_ERR_ARG, "reps must be >= 1");
     hipError_t e = hipSetDevice(b->dev);
-    int best[3];
-    for (int k = 0; k < 3; ++k) best[k] = b->bp.ept[k];
-    for (int stage = 0; stage < 3 && e == hipSuccess; ++stage) {
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->dev);
+    bdkx::BatchTuning best = b->tuning;
+    auto timed = [&](const bdkx::BatchTuning &t, std::initializer_list<int> stages, double *ms) -> int {
+        e = b->rebuild(t);
+        if (e != hipSuccess) return BDK_ERR_HIP;
+        *ms = 0.0;
+        for (int st : stages) {
+            double one = 0.0;
+            const int rc = bdk_batch_profile(b, st, reps, &one);
+            if (rc) return rc;
+            *ms += one;
+        }
+        return BDK_OK;
+    };
+    auto pick = [&](auto set, const std::vector<int> &values, std::initializer_list<int> stages) -> int {
         double tbest = 1e30;
-        for (int cand : {8, 16, 32}) {
-            if (cand < b->ept_min[stage]) continue;
-            int ept[3] = {best[0], best[1], best[2]};
-            ept[stage] = cand;
-            e = b->rebuild(ept);
-            if (e != hipSuccess) break;
+        bdkx::BatchTuning keep = best;
+        for (int v : values) {
+            bdkx::BatchTuning t = best;
+            if (!set(t, v)) continue;
             double ms = 0.0;
-            int rc = bdk_batch_profile(b, 2 + stage, reps, &ms);
+            const int rc = timed(t, stages, &ms);
             if (rc) return rc;
             if (ms < tbest) {
                 tbest = ms;
-                best[stage] = cand;
+                keep = t;
             }
         }
-    }
-    if (e == hipSuccess) e = b->rebuild(best);
+        best = keep;
+        return BDK_OK;
+    };
+    int rc = e == hipSuccess ? BDK_OK : BDK_ERR_HIP;
+    for (int stage = 0; stage < 3 && !rc; ++stage)
+        rc = pick([&](bdkx::BatchTuning &t, int v) { return v >= b->ept_min[stage] ? (t.ept[stage] = v, true) : false; },
+                  {8, 16, 32}, {2 + stage});
+    if (!rc) rc = pick([](bdkx::BatchTuning &t, int v) { return (t.wca_block = v, true); }, {128, 256, 512}, {0});
+    if (!rc) rc = pick([](bdkx::BatchTuning &t, int v) { return (t.dep_tile = v, true); }, {24, 32, 38, 48}, {1});
+    if (!rc)
+        rc = pick([](bdkx::BatchTuning &t, int v) { return (t.persist = v, true); }, {0, 4 * cus, 8 * cus},
+                  {0, 1, 2, 3, 4});
+    if (rc == BDK_ERR_HIP || e != hipSuccess)
+        return b->fail(BDK_ERR_HIP, std::string("bdk_batch_tune: ") + hipGetErrorString(e));
+    if (rc) return rc;
+    e = b->rebuild(best);
     if (e != hipSuccess) return b->fail(BDK_ERR_HIP, std::string("bdk_batch_tune: ") + hipGetErrorString(e));
-    // the timed FFT stages overwrote the members' mesh forces with other
-    // tilings (whose twiddle roundings can differ): recompute them with the
-    // chosen one before the next step
+    // the timed stages overwrote the members' forces (with equal values):
+    // recompute them with the chosen plan before the next step
     for (bdk_t *h : b->mem) h->forces_ready = false;
-    if (ept_out)
-        for (int k = 0; k < 3; ++k) ept_out[k] = best[k];
+    if (out) {
+        for (int k = 0; k < 3; ++k) out[BDK_TUNE_EPT_R2C + k] = b->bp.ept[k];
+        out[BDK_TUNE_WCA_BLOCK] = best.wca_block;
+        out[BDK_TUNE_DEPOSIT_TILE] = best.dep_tile;
+        out[BDK_TUNE_PERSIST] = best.persist;
+    }
     return BDK_OK;
 }
 

@@ -147,8 +147,18 @@ struct RepDev {
 };
 
 // Block maps of the batched stages: blockIdx.x -> (member, block of member).
-// BS_GATHER: particle blocks of the adaptive-dt members; BS_FOLD: one entry per adaptive member.
-enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_GATHER, BS_FOLD, BS_COUNT };
+// BS_GATHER: particle blocks of the adaptive-dt members; BS_FOLD: one entry per adaptive member;
+// BS_WCA: particle blocks of the WCA stage (its own block size).
+enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_GATHER, BS_FOLD, BS_WCA,
+                  BS_COUNT };
+
+// The choices of the batch autotuner (bdk_batch_tune); none changes a result.
+struct BatchTuning {
+    int ept[3] = {8, 8, 8};  // FFT elements per thread (r2c, col, c2r)
+    int wca_block = 256;     // threads per block of the WCA stage (128, 256 or 512)
+    int dep_tile = 38;       // deposit tile: list cells per side such that a tile owns <= dep_tile^2 mesh points
+    int persist = 0;         // blocks per launch of every stage (0: one block per map entry)
+};
 
 struct BatchPlan {
     int nmem = 0;
@@ -156,7 +166,8 @@ struct BatchPlan {
     RepDev *reps = nullptr;          // device [nmem]
     int2 *map[BS_COUNT] = {};        // device maps
     int nblk[BS_COUNT] = {};
-    int ept[3] = {8, 8, 8};          // FFT elements per thread (r2c, col, c2r)
+    BatchTuning tune;
+    int ept[3] = {8, 8, 8};          // the FFT elements per thread in use (at least the members' minimum)
     size_t dep_lds = 0, l1 = 0, lc = 0, l2 = 0;
 };
 
@@ -235,8 +246,8 @@ hipError_t build_tables(bdk_t *h);                // spectral tables in device m
 
 // batches (a batch is a view over member handles, each with its own buffers)
 int batch_min_ept(int stage, int zmax);
-hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
-                       std::vector<int2> maps[BS_COUNT]);
+hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune, BatchPlan &bp,
+                       std::vector<RepDev> &reps, std::vector<int2> maps[BS_COUNT]);
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st);
 // one batched step of members whose forces are ready; with aux and the two
 // events the mesh stages run beside the list rebuild

@@ -895,6 +895,24 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
     const int nbp = nb + (nb >> 3), Nsp = Ns + (Ns >> 3);
     C v[NB][RAD];
     C w[PRE ? NB : 1][RAD - 1];
+    // twiddles of butterfly k: tw^1, tw^2, tw^4 from the table, the other
+    // powers as products (one to two roundings deeper; 3 loads instead of 7
+    // for radix 8).  Every tiling forms them the same way -- before the
+    // barrier when the registers allow (PRE), else after it -- so the
+    // transform's rounding does not depend on the elements per thread.
+    auto twiddles = [&](int k, C *ww) {
+        ww[0] = tw[k << twsh];
+        if (RAD >= 4) {
+            ww[1] = tw[(2 * k) << twsh];
+            ww[2] = cmul(ww[0], ww[1]);
+        }
+        if (RAD == 8) {
+            ww[3] = tw[(4 * k) << twsh];
+            ww[4] = cmul(ww[3], ww[0]);
+            ww[5] = cmul(ww[3], ww[1]);
+            ww[6] = cmul(ww[3], ww[2]);
+        }
+    };
 #pragma unroll
     for (int e = 0; e < NB; ++e) {
         const int b = tid + e * nt;
@@ -909,23 +927,7 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
 #pragma unroll
                 for (int t = 0; t < RAD; ++t) v[e][t] = p[t * nbp];
             }
-            if (PRE) {
-                // tw^1, tw^2, tw^4 from the table, the other powers as products
-                // (one to two roundings deeper; 3 loads instead of 7 for radix 8)
-                const int k = j & (Ns - 1);
-                C *ww = w[PRE ? e : 0];
-                ww[0] = tw[k << twsh];
-                if (RAD >= 4) {
-                    ww[1] = tw[(2 * k) << twsh];
-                    ww[2] = cmul(ww[0], ww[1]);
-                }
-                if (RAD == 8) {
-                    ww[3] = tw[(4 * k) << twsh];
-                    ww[4] = cmul(ww[3], ww[0]);
-                    ww[5] = cmul(ww[3], ww[1]);
-                    ww[6] = cmul(ww[3], ww[2]);
-                }
-            }
+            if (PRE) twiddles(j & (Ns - 1), w[PRE ? e : 0]);
         }
     }
     __syncthreads();
@@ -935,9 +937,11 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
         if (b < tot) {
             const int row = b >> lognb, j = b & (nb - 1);
             const int k = j & (Ns - 1);
+            C wl[RAD - 1];
+            if (!PRE) twiddles(k, wl);
 #pragma unroll
             for (int t = 1; t < RAD; ++t) {
-                C ww = PRE ? w[PRE ? e : 0][t - 1] : tw[(t * k) << twsh];
+                C ww = PRE ? w[PRE ? e : 0][t - 1] : wl[t - 1];
                 if (inv) ww.y = -ww.y;
                 v[e][t] = cmul(v[e][t], ww);
             }
@@ -1745,127 +1749,174 @@ __global__ void k_sort_cells(Geo g, DevBufs d, int moved_ranks) { sort_cells_bod
 // single-simulation kernel on that member's geometry and buffers, so every
 // member's results are bit-identical alone and in a batch.
 
-template <typename R>
-__global__ __launch_bounds__(256) void kb_wca(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    if (std::is_same<R, float>::value && r.g.fast_wca) {
-        if (r.unit)
-            wca_f32_body<true>(r.g, r.d, t.y);
-        else
-            wca_f32_body<false>(r.g, r.d, t.y);
-    } else {
-        wca_ref_body<R>(r.g, r.d, t.y);
+// Every batched kernel walks its stage's block map: one map entry per block,
+// or in the persistent form (a grid smaller than the map) each block takes
+// the entries blockIdx.x, blockIdx.x + gridDim.x, ...  Kernels whose bodies
+// use LDS end every entry with a barrier, so the next entry may refill it.
+// A member's results do not depend on the grid or on the WCA block size
+// (per-particle sums, integer deposit sums, order-free min/max reductions).
+// P (persistent) selects the form at compile time: without it a kernel
+// handles exactly its own entry (no loop, so the code and its register use
+// are those of a plain one-entry kernel).
+template <bool P, bool SYNC, typename F> __device__ __forceinline__ void for_entries(int nmap, F &&f) {
+    if (!P) {
+        f((int)blockIdx.x);
+        return;
+    }
+    for (int e = blockIdx.x; e < nmap; e += gridDim.x) {
+        f(e);
+        if (SYNC) __syncthreads();
     }
 }
 
-template <typename R>
-__global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const int2 *map, int slot) {
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    DepArgs a = r.dep;
-    if (slot) {  // beside the rebuild: the step's end re-arms the scalars
-        a.slot = 1;
-        a.reset = 0;
-    }
-    switch (r.g.scheme) {
-    case 0: deposit_body<R, 0>(r.g, r.d, a, t.y, r.dep_tx); break;
-    case 1: deposit_body<R, 1>(r.g, r.d, a, t.y, r.dep_tx); break;
-    default: deposit_body<R, 2>(r.g, r.d, a, t.y, r.dep_tx); break;
-    }
+template <typename R, bool P>
+__global__ __launch_bounds__(512) void kb_wca(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, false>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        if (std::is_same<R, float>::value && r.g.fast_wca) {
+            if (r.unit)
+                wca_f32_body<true>(r.g, r.d, t.y);
+            else
+                wca_f32_body<false>(r.g, r.d, t.y);
+        } else {
+            wca_ref_body<R>(r.g, r.d, t.y);
+        }
+    });
 }
 
-template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int2 *map) {
+template <typename R, bool P>
+__global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const int2 *map, int nmap, int slot) {
+    for_entries<P, true>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        DepArgs a = r.dep;
+        if (slot) {  // beside the rebuild: the step's end re-arms the scalars
+            a.slot = 1;
+            a.reset = 0;
+        }
+        switch (r.g.scheme) {
+        case 0: deposit_body<R, 0>(r.g, r.d, a, t.y, r.dep_tx); break;
+        case 1: deposit_body<R, 1>(r.g, r.d, a, t.y, r.dep_tx); break;
+        default: deposit_body<R, 2>(r.g, r.d, a, t.y, r.dep_tx); break;
+        }
+    });
+}
+
+template <typename R, int EPT, bool P>
+__global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int2 *map, int nmap) {
     using C = typename V2<R>::T;
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    r2c_rows_body<R, EPT>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb, r.sR1, t.y);
+    for_entries<P, true>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        r2c_rows_body<R, EPT>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb, r.sR1, t.y);
+    });
 }
 
-template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int2 *map) {
+template <typename R, int EPT, bool P>
+__global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int2 *map, int nmap) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    const R *sq = std::is_same<R, float>::value ? (const R *)r.d.sqf : (const R *)r.d.sq;
-    const R *lap = std::is_same<R, float>::value ? (const R *)r.d.lapf : (const R *)r.d.lap;
-    col_body<R, EPT>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
+    for_entries<P, true>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        const R *sq = std::is_same<R, float>::value ? (const R *)r.d.sqf : (const R *)r.d.sq;
+        const R *lap = std::is_same<R, float>::value ? (const R *)r.d.lapf : (const R *)r.d.lap;
+        col_body<R, EPT>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
+    });
 }
 
-template <typename R, int EPT>
-__global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map, int end_step) {
+template <typename R, int EPT, bool P>
+__global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map, int nmap, int end_step) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    if (end_step && t.y == 0) step_end_body(r.g, r.d);
-    c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb2, r.sR2, t.y);
+    for_entries<P, true>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        if (end_step && t.y == 0) step_end_body(r.g, r.d);
+        c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb2, r.sR2,
+                              t.y);
+    });
 }
 
 // Members with a fixed dt gather the mesh force inside the update; members
 // with adaptive dt read the forces and the dt of kb_gather / kb_dt_fold, as
 // their single-simulation steps do.
-template <typename R>
-__global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    if (r.g.adaptive) {
-        switch (r.g.scheme) {
-        case 0: update_body<R, 0, false>(r.g, r.d, t.y); break;
-        case 1: update_body<R, 1, false>(r.g, r.d, t.y); break;
-        default: update_body<R, 2, false>(r.g, r.d, t.y); break;
+template <typename R, bool P>
+__global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, false>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        if (r.g.adaptive) {
+            switch (r.g.scheme) {
+            case 0: update_body<R, 0, false>(r.g, r.d, t.y); break;
+            case 1: update_body<R, 1, false>(r.g, r.d, t.y); break;
+            default: update_body<R, 2, false>(r.g, r.d, t.y); break;
+            }
+        } else {
+            switch (r.g.scheme) {
+            case 0: update_body<R, 0, true>(r.g, r.d, t.y); break;
+            case 1: update_body<R, 1, true>(r.g, r.d, t.y); break;
+            default: update_body<R, 2, true>(r.g, r.d, t.y); break;
+            }
         }
-        return;
-    }
-    switch (r.g.scheme) {
-    case 0: update_body<R, 0, true>(r.g, r.d, t.y); break;
-    case 1: update_body<R, 1, true>(r.g, r.d, t.y); break;
-    default: update_body<R, 2, true>(r.g, r.d, t.y); break;
-    }
+    });
 }
 
 // capillary forces and dt candidates of the adaptive members (their
 // particle blocks only), then one wave per adaptive member folds its dt
-template <typename R>
-__global__ __launch_bounds__(256) void kb_gather(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    const RepDev &r = m[t.x];
-    switch (r.g.scheme) {
-    case 0: gather_body<R, 0>(r.g, r.d, 1, t.y, blockIdx.x); break;
-    case 1: gather_body<R, 1>(r.g, r.d, 1, t.y, blockIdx.x); break;
-    default: gather_body<R, 2>(r.g, r.d, 1, t.y, blockIdx.x); break;
-    }
-}
-
-__global__ __launch_bounds__(64) void kb_dt_fold(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    dt_fold_body(m[t.x].g, m[t.x].d);
-}
-
-__global__ __launch_bounds__(256) void kb_rank_scan(const RepDev *m, const int2 *mp, int nrank, const int2 *ms) {
-    if ((int)blockIdx.x < nrank) {
-        const int2 t = mp[blockIdx.x];
-        rank_body(m[t.x].g, m[t.x].d, t.y);
-    } else {
-        const int2 t = ms[blockIdx.x - nrank];
+template <typename R, bool P>
+__global__ __launch_bounds__(256) void kb_gather(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, true>(nmap, [&](int e_) {
+        const int2 t = map[e_];
         const RepDev &r = m[t.x];
-        scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
-    }
+        switch (r.g.scheme) {
+        case 0: gather_body<R, 0>(r.g, r.d, 1, t.y, e_); break;
+        case 1: gather_body<R, 1>(r.g, r.d, 1, t.y, e_); break;
+        default: gather_body<R, 2>(r.g, r.d, 1, t.y, e_); break;
+        }
+    });
 }
 
-
-__global__ void kb_fill(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    fill_body(m[t.x].g, m[t.x].d, 1, t.y);
+template <bool P>
+__global__ __launch_bounds__(64) void kb_dt_fold(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, false>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        dt_fold_body(m[t.x].g, m[t.x].d);
+    });
 }
 
-__global__ void kb_sort_cells(const RepDev *m, const int2 *map) {
-    const int2 t = map[blockIdx.x];
-    sort_cells_body(m[t.x].g, m[t.x].d, 1, t.y);
+// entries [0, nrank): ranks (map mp); entries [nrank, nrank + nscan): scans (map ms)
+template <bool P>
+__global__ __launch_bounds__(256) void kb_rank_scan(const RepDev *m, const int2 *mp, int nrank, const int2 *ms,
+                                                    int nscan) {
+    for_entries<P, true>(nrank + nscan, [&](int e_) {
+        if (e_ < nrank) {
+            const int2 t = mp[e_];
+            rank_body(m[t.x].g, m[t.x].d, t.y);
+        } else {
+            const int2 t = ms[e_ - nrank];
+            const RepDev &r = m[t.x];
+            scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
+        }
+    });
 }
 
+template <bool P>
+__global__ __launch_bounds__(256) void kb_fill(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, false>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        fill_body(m[t.x].g, m[t.x].d, 1, t.y);
+    });
+}
+
+template <bool P>
+__global__ __launch_bounds__(256) void kb_sort_cells(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, false>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        sort_cells_body(m[t.x].g, m[t.x].d, 1, t.y);
+    });
+}
 
 // ------------------------------------------------------------------ launchers
 static inline unsigned nblk(int n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1943,16 +1994,19 @@ hipError_t set_kernel_attributes() {
     BDK_ATTR_E(16)
     BDK_ATTR_E(32)
 #undef BDK_ATTR_E
-#define BDK_ATTR_B(E)            \
-    BDK_ATTR((kb_r2c<float, E>))  \
-    BDK_ATTR((kb_r2c<double, E>)) \
-    BDK_ATTR((kb_col<float, E>))  \
-    BDK_ATTR((kb_col<double, E>)) \
-    BDK_ATTR((kb_c2r<float, E>))  \
-    BDK_ATTR((kb_c2r<double, E>))
-    BDK_ATTR_B(8)
-    BDK_ATTR_B(16)
-    BDK_ATTR_B(32)
+#define BDK_ATTR_B(E, P)            \
+    BDK_ATTR((kb_r2c<float, E, P>))  \
+    BDK_ATTR((kb_r2c<double, E, P>)) \
+    BDK_ATTR((kb_col<float, E, P>))  \
+    BDK_ATTR((kb_col<double, E, P>)) \
+    BDK_ATTR((kb_c2r<float, E, P>))  \
+    BDK_ATTR((kb_c2r<double, E, P>))
+    BDK_ATTR_B(8, false)
+    BDK_ATTR_B(16, false)
+    BDK_ATTR_B(32, false)
+    BDK_ATTR_B(8, true)
+    BDK_ATTR_B(16, true)
+    BDK_ATTR_B(32, true)
 #undef BDK_ATTR_B
 #undef BDK_ATTR
     // the deposit also has a little static LDS: ask for 128 KB of dynamic LDS
@@ -1964,8 +2018,10 @@ hipError_t set_kernel_attributes() {
     BDK_ATTR_D((k_deposit<double, 0>))
     BDK_ATTR_D((k_deposit<double, 1>))
     BDK_ATTR_D((k_deposit<double, 2>))
-    BDK_ATTR_D((kb_deposit<float>))
-    BDK_ATTR_D((kb_deposit<double>))
+    BDK_ATTR_D((kb_deposit<float, false>))
+    BDK_ATTR_D((kb_deposit<double, false>))
+    BDK_ATTR_D((kb_deposit<float, true>))
+    BDK_ATTR_D((kb_deposit<double, true>))
 #undef BDK_ATTR_D
     return e;
 }
@@ -1991,7 +2047,7 @@ hipError_t launch_wca(bdk_t *h, hipStream_t st) {
 }
 
 // Deposit tiling: tile side tc (list cells) such that a tile owns at most
-// about 38 x 38 mesh points; fixed-point scales chosen so that no sum of
+// about tile_pts x tile_pts mesh points (38 unless a batch tuned it); fixed-point scales chosen so that no sum of
 // N charges can overflow 63 bits.
 struct DepPlan {
     DepArgs a;
@@ -1999,11 +2055,11 @@ struct DepPlan {
     size_t lds;
 };
 
-static DepPlan dep_plan(const bdk_t *h, int reset) {
+static DepPlan dep_plan(const bdk_t *h, int reset, int tile_pts = 38) {
     const Geo &g = h->g;
     const bool f64 = h->p.precision == BDK_FP64;
     DepPlan pl;
-    int tc = (int)std::floor(38.0 * g.h / g.B);
+    int tc = (int)std::floor(tile_pts * g.h / g.B);
     if (tc < 1) tc = 1;
     if (tc > g.nc) tc = g.nc;
     const int side = (int)std::floor(tc * g.B / g.h) + 2;
@@ -2831,14 +2887,18 @@ int batch_min_ept(int stage, int zmax) {
     return ept_for(8, need);
 }
 
-hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchPlan &bp, std::vector<RepDev> &reps,
-                       std::vector<int2> maps[BS_COUNT]) {
+hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune, BatchPlan &bp,
+                       std::vector<RepDev> &reps, std::vector<int2> maps[BS_COUNT]) {
     const int n = (int)mem.size();
     reps.assign(n, RepDev());
     for (int k = 0; k < BS_COUNT; ++k) maps[k].clear();
     bp.nmem = n;
     bp.f32 = mem.empty() ? 1 : (mem[0]->p.precision != BDK_FP64);
-    for (int k = 0; k < 3; ++k) bp.ept[k] = ept[k];
+    bp.tune = tune;
+    int zmax = 8;
+    for (const bdk_t *h : mem) zmax = std::max(zmax, h->g.Z);
+    for (int k = 0; k < 3; ++k) bp.ept[k] = std::max(tune.ept[k], batch_min_ept(k, zmax));
+    const int *ept = bp.ept;
     bp.dep_lds = bp.l1 = bp.lc = bp.l2 = 0;
     const size_t csz = bp.f32 ? 8 : 16;
     for (int i = 0; i < n; ++i) {
@@ -2846,7 +2906,7 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchP
         RepDev &r = reps[i];
         r.g = h->g;
         r.d = h->d;
-        const DepPlan pl = dep_plan(h, 1);
+        const DepPlan pl = dep_plan(h, 1, tune.dep_tile);
         r.dep = pl.a;
         r.dep_tx = (int)pl.grid.x;
         bp.dep_lds = std::max(bp.dep_lds, pl.lds);
@@ -2863,7 +2923,7 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchP
         const int ad = h->g.adaptive ? 1 : 0;
         const int nb[BS_COUNT] = {(int)nblk(h->g.N, 256), (int)nblk(h->g.ncell, 256), r.scan_nb,
                                   (int)(pl.grid.x * pl.grid.y), Z / r.rb, (Z / 2) / r.rc, Z / r.rb2,
-                                  ad * (int)nblk(h->g.N, 256), ad};
+                                  ad * (int)nblk(h->g.N, 256), ad, (int)nblk(h->g.N, tune.wca_block)};
         for (int k = 0; k < BS_COUNT; ++k)
             for (int b = 0; b < nb[k]; ++b) maps[k].push_back(make_int2(i, b));
     }
@@ -2878,56 +2938,75 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const int ept[3], BatchP
     default: CALL(32); break;    \
     }
 
-template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, hipStream_t st) {
+// blocks of a launch over a map of n entries (the persistent form caps them)
+static unsigned stage_grid(const BatchPlan &bp, int n) {
+    const int p = bp.tune.persist;
+    return (unsigned)std::max(1, p > 0 ? std::min(n, p) : n);
+}
+
+template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, int stage, hipStream_t st) {
     const RepDev *m = bp.reps;
+#define BDK_G(S) dim3(stage_grid(bp, bp.nblk[S]))
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL(kb_wca<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        hipLaunchKernelGGL((kb_wca<R, P>), BDK_G(BS_WCA), dim3(bp.tune.wca_block), 0, st, m, bp.map[BS_WCA],
+                           bp.nblk[BS_WCA]);
         break;
     case 1:  // from the list order
     case 8:  // from the slot arrays, beside the rebuild
-        hipLaunchKernelGGL(kb_deposit<R>, dim3(bp.nblk[BS_DEP]), dim3(DP_THREADS), bp.dep_lds, st, m, bp.map[BS_DEP],
-                           stage == 8 ? 1 : 0);
+        hipLaunchKernelGGL((kb_deposit<R, P>), BDK_G(BS_DEP), dim3(DP_THREADS), bp.dep_lds, st, m, bp.map[BS_DEP],
+                           bp.nblk[BS_DEP], stage == 8 ? 1 : 0);
         break;
     case 2: {
-#define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E>), dim3(bp.nblk[BS_R2C]), dim3(FFT_THREADS), bp.l1, st, m, bp.map[BS_R2C])
+#define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E, P>), BDK_G(BS_R2C), dim3(FFT_THREADS), bp.l1, st, m, bp.map[BS_R2C], \
+                              bp.nblk[BS_R2C])
         BDK_EPT_SWITCH2(bp.ept[0], R2C)
 #undef R2C
         break;
     }
     case 3: {
-#define COL(E) hipLaunchKernelGGL((kb_col<R, E>), dim3(bp.nblk[BS_COL]), dim3(FFT_THREADS), bp.lc, st, m, bp.map[BS_COL])
+#define COL(E) hipLaunchKernelGGL((kb_col<R, E, P>), BDK_G(BS_COL), dim3(FFT_THREADS), bp.lc, st, m, bp.map[BS_COL], \
+                              bp.nblk[BS_COL])
         BDK_EPT_SWITCH2(bp.ept[1], COL)
 #undef COL
         break;
     }
     case 4:
     case 12: {  // 12: also the end of the step
-#define C2R(E) hipLaunchKernelGGL((kb_c2r<R, E>), dim3(bp.nblk[BS_C2R]), dim3(FFT_THREADS), bp.l2, st, m, bp.map[BS_C2R], \
-                              stage == 12 ? 1 : 0)
+#define C2R(E) hipLaunchKernelGGL((kb_c2r<R, E, P>), BDK_G(BS_C2R), dim3(FFT_THREADS), bp.l2, st, m, bp.map[BS_C2R], \
+                              bp.nblk[BS_C2R], stage == 12 ? 1 : 0)
         BDK_EPT_SWITCH2(bp.ept[2], C2R)
 #undef C2R
         break;
     }
     case 5:
-        hipLaunchKernelGGL(kb_update<R>, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
+        hipLaunchKernelGGL((kb_update<R, P>), BDK_G(BS_PART), dim3(256), 0, st, m, bp.map[BS_PART], bp.nblk[BS_PART]);
         break;
     case 13:  // adaptive members: forces, dt candidates and the fold
         if (bp.nblk[BS_GATHER] > 0) {
-            hipLaunchKernelGGL(kb_gather<R>, dim3(bp.nblk[BS_GATHER]), dim3(256), 0, st, m, bp.map[BS_GATHER]);
-            hipLaunchKernelGGL(kb_dt_fold, dim3(bp.nblk[BS_FOLD]), dim3(64), 0, st, m, bp.map[BS_FOLD]);
+            hipLaunchKernelGGL((kb_gather<R, P>), BDK_G(BS_GATHER), dim3(256), 0, st, m, bp.map[BS_GATHER],
+                               bp.nblk[BS_GATHER]);
+            hipLaunchKernelGGL(kb_dt_fold<P>, BDK_G(BS_FOLD), dim3(64), 0, st, m, bp.map[BS_FOLD], bp.nblk[BS_FOLD]);
         }
         break;
     case 6:
-        hipLaunchKernelGGL(kb_rank_scan, dim3(bp.nblk[BS_PART] + bp.nblk[BS_SCAN]), dim3(256), 0, st, m,
-                           bp.map[BS_PART], bp.nblk[BS_PART], bp.map[BS_SCAN]);
+        hipLaunchKernelGGL(kb_rank_scan<P>, dim3(stage_grid(bp, bp.nblk[BS_PART] + bp.nblk[BS_SCAN])), dim3(256), 0, st,
+                           m, bp.map[BS_PART], bp.nblk[BS_PART], bp.map[BS_SCAN], bp.nblk[BS_SCAN]);
         break;
     default: {  // 11: fill, sort
-        hipLaunchKernelGGL(kb_fill, dim3(bp.nblk[BS_PART]), dim3(256), 0, st, m, bp.map[BS_PART]);
-        hipLaunchKernelGGL(kb_sort_cells, dim3(bp.nblk[BS_CELL]), dim3(256), 0, st, m, bp.map[BS_CELL]);
+        hipLaunchKernelGGL(kb_fill<P>, BDK_G(BS_PART), dim3(256), 0, st, m, bp.map[BS_PART], bp.nblk[BS_PART]);
+        hipLaunchKernelGGL(kb_sort_cells<P>, BDK_G(BS_CELL), dim3(256), 0, st, m, bp.map[BS_CELL], bp.nblk[BS_CELL]);
         break;
     }
     }
+#undef BDK_G
+}
+
+template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, hipStream_t st) {
+    if (bp.tune.persist > 0)
+        batch_stage_tp<R, true>(bp, stage, st);
+    else
+        batch_stage_tp<R, false>(bp, stage, st);
 }
 
 // stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update,

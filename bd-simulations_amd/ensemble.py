@@ -88,8 +88,8 @@ def time_plain(sims, steps):
 
 
 def time_batched(sims, steps, tune_reps=5):
-    """All replicas as one batch: particle-steps/s, the tuned FFT tiling and
-    per-stage times."""
+    """All replicas as one batch: particle-steps/s, the autotuner's choice
+    and per-stage times."""
     b = bdk.Batch(sims)
     ept = b.tune(tune_reps)
     b.run_steps(8)

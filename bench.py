@@ -268,13 +268,13 @@ def ensemble_figures(log_fn, cpu=False):
     plain_steps, batch_steps = 100, 500
     log_fn("ensemble: plain path, %d steps per replica, one replica at a time" % plain_steps)
     ps_plain, t_plain = ensemble.time_plain(sims, plain_steps)
-    log_fn("ensemble: batched, %d steps (autotuned FFT tiling)" % batch_steps)
+    log_fn("ensemble: batched, %d steps (autotuned)" % batch_steps)
     ps_b, t_b, ept, stages = ensemble.time_batched(sims, batch_steps)
     for s in sims:
         s.close()
     batched = {"replicas": len(mix), "particles": n_mix, "steps": batch_steps, "value": ps_b,
                "unit": "particle-steps/s", "plain_value": ps_plain, "plain_steps": plain_steps,
-               "speedup_vs_plain": ps_b / ps_plain, "fft_ept": ept,
+               "speedup_vs_plain": ps_b / ps_plain, "tuning": ept,
                "stage_ms": {k: v for k, v in stages.items()},
                "mix": "N log-uniform [2e3, 6.4e4], density 0.5, Z = pow2(L/0.5) in [64, 512], "
                       "lambda U[5,50], CIC/TSC, FP32 forces, uniform start"}

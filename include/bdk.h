@@ -363,9 +363,26 @@ int bdk_batch_sync(bdk_batch_t *b);
    flag stays until the member's positions are set again), or BDK_ERR_HIP for
    an internal inconsistency.  n = the number of members. */
 int bdk_batch_status(bdk_batch_t *b, int32_t n, int32_t *status);
-/* Project autotuner: times the FFT stages for elements per thread 8/16/32
-   (reps launches each) and keeps the fastest; ept_out[3] receives the choice. */
-int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *ept_out);
+/* Project autotuner: times each knob on the batch's own members (reps
+   launches of the stages that leave the particles unchanged) and keeps the
+   fastest value: the FFT elements per thread of each pass (8, 16, 32), the
+   WCA block size (128, 256, 512 threads), the deposit tile (list cells per
+   side such that a tile owns at most 24^2, 32^2, 38^2 or 48^2 mesh points),
+   and the persistent grid (one block per work item, or 4 or 8 blocks per
+   compute unit that loop over the work items).  No choice changes a
+   result.  choice_out[BDK_TUNE_COUNT] (may be NULL) receives the choice. */
+enum bdk_tune_index {
+    BDK_TUNE_EPT_R2C = 0, BDK_TUNE_EPT_COL = 1, BDK_TUNE_EPT_C2R = 2,
+    BDK_TUNE_WCA_BLOCK = 3, BDK_TUNE_DEPOSIT_TILE = 4, BDK_TUNE_PERSIST = 5,
+    BDK_TUNE_COUNT = 6
+};
+int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *choice_out);
+/* Apply a choice (e.g. one bdk_batch_tune made for the same mix): FFT
+   elements per thread 8, 16 or 32 (raised to the members' minimum), WCA
+   block size 128, 256 or 512, deposit tile 8..64 mesh points per side,
+   persistent grid >= 0 (0: one block per work item).  No choice changes a
+   result. */
+int bdk_batch_set_tuning(bdk_batch_t *b, const int32_t *choice);
 /* ms per launch of one stage (0 WCA, 1 deposit, 2 r2c, 3 col, 4 c2r, 7 whole
    step) over reps launches after one warm-up launch.  Stages 0-4 leave the
    particles unchanged; stage 7 advances every member by reps + 1 steps. */

@@ -59,10 +59,11 @@ def test_batch_members_equal_alone(prec):
         assert np.array_equal(ga, gm), "member %d cells differ" % i
         assert np.array_equal(ra, rm), "member %d positions differ (max %.3e)" % (i, np.abs(ra - rm).max())
         assert a.step_counter() == m.step_counter() == 37
-    # the autotuner re-tiles the FFT stages; the arithmetic per row is unchanged
-    ept = b.tune(3)
-    say("tuned ept", ept)
-    assert all(e in (8, 16, 32) for e in ept)
+    # the autotuner re-tiles the stages; no choice changes a result
+    choice = b.tune(3)
+    say("tuned", choice)
+    assert all(choice[k] in (8, 16, 32) for k in ("ept_r2c", "ept_col", "ept_c2r"))
+    assert choice["wca_block"] in (128, 256, 512) and choice["deposit_tile"] in (24, 32, 38, 48)
     b.run_steps(11)
     b.sync()
     for s in alone:
@@ -72,6 +73,43 @@ def test_batch_members_equal_alone(prec):
     # members stay usable through their own handles
     f_alone, f_member = alone[3].berechne_WCA(), members[3].berechne_WCA()
     assert np.array_equal(f_alone, f_member)
+    b.close()
+
+
+@pytest.mark.parametrize("choice", [
+    dict(ept_r2c=32, ept_col=32, ept_c2r=32, wca_block=512, deposit_tile=24, persist=64),
+    dict(ept_r2c=16, ept_col=8, ept_c2r=32, wca_block=128, deposit_tile=48, persist=1),
+    dict(ept_r2c=8, ept_col=16, ept_c2r=16, wca_block=256, deposit_tile=32, persist=0),
+])
+def test_batch_tuning_choices_keep_results(choice):
+    """Every knob of the autotuner (FFT elements per thread, WCA block size,
+    deposit tile, persistent grid down to a single block looping over all
+    work items) leaves each member's trajectory bit-identical to its run
+    alone, FP32, with one adaptive-dt member."""
+    say("case: tuning choice %s" % choice)
+
+    def with_adaptive():
+        out = sims(bdk.FP32)
+        bp, _ = params(n1=500, n2=300, L=40.0, Z=64, scheme=bdk.TSC, lam=6.0, precision=bdk.FP32, seed=19,
+                       T=2.0, dt_max=2e-3, max_travel=0.05)
+        git, rel = positions(bp, 19, 0.9)
+        s = bdk.RunDynamik(bp)
+        s.set_positions(git, rel)
+        return out + [s]
+
+    alone, members = with_adaptive(), with_adaptive()
+    b = bdk.Batch(members)
+    b.set_tuning(**choice)
+    b.run_steps(13)
+    b.sync()
+    for s in alone:
+        s.run_steps(13)
+    for i, (a, m) in enumerate(zip(alone, members)):
+        ga, ra = a.positions()
+        gm, rm = m.positions()
+        assert np.array_equal(ga, gm) and np.array_equal(ra, rm), "member %d" % i
+    with pytest.raises(bdk.BdkError):
+        b.set_tuning(wca_block=96)
     b.close()
 
 
