@@ -87,6 +87,7 @@ def lib():
             "bdk_steps_for_interval": (C.c_int, [dbl, dbl, vp]),
             "bdk_refresh": (C.c_int, [vp]),
             "bdk_sync": (C.c_int, [vp]),
+            "bdk_prepare": (C.c_int, [vp]),
             "bdk_step_counter": (i64, [vp]),
             "bdk_get_cells": (C.c_int, [vp, vp, vp]),
             "bdk_get_mesh": (C.c_int, [vp, C.c_int, vp]),
@@ -130,6 +131,7 @@ def lib():
             "bdk_serve_step": (C.c_int, [vp, vp, vp, vp]),
             "bdk_serve_result": (C.c_int, [vp, i64, vp, vp, vp, vp]),
             "bdk_serve_stats": (C.c_int, [vp, vp, vp, vp]),
+            "bdk_serve_timing": (C.c_int, [vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -143,7 +145,7 @@ def exported_symbols():
     return ["bdk_params_default", "bdk_params_derived", "bdk_create", "bdk_destroy", "bdk_last_error",
             "bdk_set_positions", "bdk_get_positions", "bdk_compute_wca", "bdk_compute_capillary",
             "bdk_set_noise", "bdk_get_noise", "bdk_optimal_dt", "bdk_step", "bdk_run_steps",
-            "bdk_run_interval", "bdk_run_interval_clock", "bdk_set_max_travel", "bdk_steps_for_interval", "bdk_refresh", "bdk_sync", "bdk_step_counter",
+            "bdk_run_interval", "bdk_run_interval_clock", "bdk_set_max_travel", "bdk_steps_for_interval", "bdk_refresh", "bdk_prepare", "bdk_sync", "bdk_step_counter",
             "bdk_get_cells", "bdk_get_mesh", "bdk_footprint", "bdk_profile_steps", "bdk_profile_kernel",
             "bdk_mt_create",
             "bdk_mt_destroy", "bdk_mt_uniform", "bdk_mt_init_positions", "bdk_mt_disc_noise",
@@ -152,7 +154,7 @@ def exported_symbols():
             "bdk_radial_profile", "bdk_rho_k", "bdk_pair_correlation", "bdk_pair_correlation_noalloc", "bdk_density_grid", "bdk_density_stats", "bdk_ftrho_sums", "bdk_batch_create", "bdk_batch_destroy", "bdk_batch_last_error", "bdk_batch_run_steps",
             "bdk_batch_sync", "bdk_batch_status", "bdk_batch_tune", "bdk_batch_set_tuning", "bdk_batch_profile",
             "bdk_serve_create", "bdk_serve_destroy", "bdk_serve_last_error", "bdk_serve_submit",
-            "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats"]
+            "bdk_serve_step", "bdk_serve_result", "bdk_serve_stats", "bdk_serve_timing"]
 
 
 def _p(a):
@@ -374,6 +376,10 @@ class RunDynamik:
 
     def refresh_erwNachbar(self):
         self._chk(self.L.bdk_refresh(self.h))
+
+    def prepare(self):
+        """bdk_prepare: upload, lists and forces enqueued now."""
+        self._chk(self.L.bdk_prepare(self.h))
 
     def sync(self):
         self._chk(self.L.bdk_sync(self.h))
@@ -616,6 +622,12 @@ class Server:
         ps, busy, used = C.c_int64(0), C.c_double(0.0), C.c_uint64(0)
         self._chk(self.L.bdk_serve_stats(self.h, C.byref(ps), C.byref(busy), C.byref(used)))
         return ps.value, busy.value, used.value
+
+    def timing(self):
+        """Host seconds in admission, batch rebuilds, stepping and retirement."""
+        v = [C.c_double(0.0) for _ in range(4)]
+        self._chk(self.L.bdk_serve_timing(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("admit_s", "batch_s", "run_s", "retire_s"), [x.value for x in v]))
 
 
 def has_gpu():

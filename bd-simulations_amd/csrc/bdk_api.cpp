@@ -1045,6 +1045,12 @@ int bdk_refresh(bdk_t *h) {
     return check_nonfinite(h);
 }
 
+int bdk_prepare(bdk_t *h) {
+    if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
+    HIPCHK(h, hipSetDevice(h->dev));
+    return ensure_forces(h);
+}
+
 int bdk_sync(bdk_t *h) {
     if (!h) return fail(nullptr, BDK_ERR_ARG, "null handle");
     HIPCHK(h, hipStreamSynchronize(h->stream));

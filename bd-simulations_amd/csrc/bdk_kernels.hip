@@ -777,8 +777,20 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     __syncthreads();
     R *rho = (R *)d.rho;
     const int nl = lhi - llo, nown = (khi - klo) * nl;
+    // w / nl from a float reciprocal, corrected to the exact quotient
+    // (w < 2^24): an integer division per point cost more than the point
+    const float rnl = nl > 0 ? 1.0f / (float)nl : 0.0f;
     for (int w = tid; w < nown; w += nt) {
-        const int k = klo + w / nl, l = llo + w - (w / nl) * nl;
+        int qk = (int)((float)w * rnl);
+        int rl = w - qk * nl;
+        if (rl >= nl) {
+            ++qk;
+            rl -= nl;
+        } else if (rl < 0) {
+            --qk;
+            rl += nl;
+        }
+        const int k = klo + qk, l = llo + rl;
         double v;
         if (win) {
             v = (double)(long long)acc_hi[((k - kw0) & (g.Z - 1)) * nwy + ((l - lw0) & (g.Z - 1))] * a.iscale_f;
@@ -1333,6 +1345,17 @@ __device__ __forceinline__ void dt_fold_body(const Geo &g, const DevBufs &d) {
 
 __global__ __launch_bounds__(64) void k_dt_fold(Geo g, DevBufs d) { dt_fold_body(g, d); }
 
+// fmod(y, B) for the refresh's y = rel + B (dm:249-250): for |y| < 2B the
+// quotient is 0 or 1 and the remainder y or y - B, which is exact (y and B
+// within a factor of two); larger moves take fmod itself.  Equal to fmod
+// bit for bit everywhere (IEEE fmod is exact too).
+__device__ __forceinline__ double fmod_cell(double y, double B) {
+    if (y >= 0.0 && y < B) return y;
+    if (y >= B && y < 2.0 * B) return y - B;
+    if (y > -B && y < 0.0) return y;
+    return fmod(y, B);
+}
+
 // Euler step (dm:116-157) + refresh (dm:216-325) + cell counting of list
 // position s (written to slot s).  Returns how far the stored new position
 // (cell and offset) lies outside the particle's previous cell (<= 0 inside).
@@ -1414,8 +1437,8 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
         const int jc0 = (int)floor((gy * B + rn.y) / B);
         const int ic = wrapc(ic0 + g.nc, g.nc);
         const int jc = wrapc(jc0 + g.nc, g.nc);
-        rn.x = fmod(rn.x + B, B);
-        rn.y = fmod(rn.y + B, B);
+        rn.x = fmod_cell(rn.x + B, B);
+        rn.y = fmod_cell(rn.y + B, B);
         // a move of more than one cell downwards keeps a negative offset
         // (fmod takes the sign of rel + B, dm:216-325): the stored position
         // then lies one cell below the moved one, so the bound follows the
@@ -1706,9 +1729,22 @@ __global__ void k_zero_int(int *p, int n) {
 __device__ __forceinline__ void sort_cells_body(const Geo &g, const DevBufs &d, int moved_ranks, int blk) {
     const int c = blk * blockDim.x + threadIdx.x;
     if (c >= g.ncell) return;
-    const int b = d.start[c], n = d.start[c + 1] - b;
-    const int lo = moved_ranks ? n - d.arr[c] : 0;
-    d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
+    int lo = 0, n, b;
+    if (moved_ranks) {
+        // after a step most cells had no arrival and one or two had one: only
+        // cells with two or more arrivals read their range and sort
+        const int a = d.arr[c];
+        if (a == 0) return;
+        d.arr[c] = 0;  // arrivals of the step, consumed by k_fill
+        if (a < 2) return;
+        b = d.start[c];
+        n = d.start[c + 1] - b;
+        lo = n - a;
+    } else {
+        b = d.start[c];
+        n = d.start[c + 1] - b;
+        d.arr[c] = 0;
+    }
     if (lo < 0 || n - lo < 2) return;  // (lo < 0: inconsistent counts, flagged by the scan)
     unsigned *l = (unsigned *)d.list + b;
     double2 *rl = d.lrel + b;

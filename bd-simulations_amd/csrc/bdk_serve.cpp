@@ -4,12 +4,13 @@
 // counterpart in the reference, which runs one simulation per process run
 // (main.cpp:130-300); the server is a client of the public C-ABI only.
 //
-// One tick: admit queued jobs in arrival order while the budget and the job
-// limit allow, rebuild the batches if the set of jobs changed (one batch per
-// precision class, side by side on the device), advance every active job by
-// min(tick, smallest remaining step count) steps with one launch per stage
-// and class, then retire the jobs that are done (their positions are copied
-// to the host and their device state is freed).  A job's trajectory
+// One tick: the jobs admitted during the previous tick join (the batches are
+// rebuilt when the set of jobs changed, one batch per precision class, side
+// by side on the device), every active job advances by min(tick, smallest
+// remaining step count) steps with one launch per stage and class, queued
+// jobs are admitted in arrival order while those steps run (as far as the
+// budget and the job limit allow), and the jobs that are done retire (their
+// positions are copied to the host and their device state is freed).  A job's trajectory
 // is bit-identical to running it alone, whatever it shares the device with.
 
 #include <algorithm>
@@ -52,6 +53,7 @@ struct bdk_server {
     int64_t next_id = 1;
     std::deque<Job *> queue;
     std::vector<Job *> active;
+    std::vector<Job *> pending;  // admitted (on the device, forces enqueued), stepping from the next tick
     std::map<int64_t, Job *> finished;
     // one batch per precision class of the active jobs (a batch needs one
     // precision); idx = the jobs' positions in `active`
@@ -63,6 +65,9 @@ struct bdk_server {
     bool dirty = false;
     int64_t particle_steps = 0;
     double busy_s = 0;
+    // host seconds per part of the ticks: admission (create + upload),
+    // batch rebuilds, stepping (enqueue + wait), retirement (read-back + free)
+    double t_admit = 0, t_batch = 0, t_run = 0, t_retire = 0;
     std::string err;
 
     int fail(int code, const std::string &m) {
@@ -94,6 +99,10 @@ void bdk_serve_destroy(bdk_server_t *s) {
     if (!s) return;
     s->drop_batches();
     for (Job *j : s->active) {
+        bdk_destroy(j->h);
+        delete j;
+    }
+    for (Job *j : s->pending) {
         bdk_destroy(j->h);
         delete j;
     }
@@ -143,16 +152,13 @@ static void retire(bdk_server_t *s, Job *j, int status, const std::string &err) 
     s->finished[j->id] = j;
 }
 
-// One tick; *n_active / *n_queued / *n_done receive the counts after it
-// (n_done counts failed jobs too).  Every job is at all times in exactly one
-// of the queue, the active set and the finished map.  A job that cannot be
-// admitted (bdk_create or the upload fails) or whose particles run away is
-// finished with its error code, which bdk_serve_result reports; the other
-// jobs go on.  The tick itself fails only when the device fails.
-int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_t *n_done) {
-    if (!s) return BDK_ERR_ARG;
-    // admission in arrival order
-    while (!s->queue.empty() && (int)s->active.size() < s->max_jobs) {
+// Admission in arrival order while the job limit and the HBM budget allow:
+// each job's handle is created, its positions uploaded and its forces
+// enqueued on its own stream (bdk_prepare), so this overlaps the batches'
+// steps; the job starts stepping with the next tick.  A job that cannot be
+// admitted is finished as failed.
+static void admit(bdk_server_t *s) {
+    while (!s->queue.empty() && (int)(s->active.size() + s->pending.size()) < s->max_jobs) {
         Job *j = s->queue.front();
         if (s->used + j->bytes > s->budget) break;
         s->queue.pop_front();
@@ -167,6 +173,7 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
             rc = bdk_set_positions(h, 1, n1, j->git.data(), j->rel.data());
             if (!rc && n2)
                 rc = bdk_set_positions(h, 2, n2, j->git.data() + 2 * (size_t)n1, j->rel.data() + 2 * (size_t)n1);
+            if (!rc) rc = bdk_prepare(h);
             if (rc) {
                 m = bdk_last_error(h);
                 bdk_destroy(h);
@@ -179,9 +186,35 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
         }
         j->h = h;
         s->used += j->bytes;
-        s->active.push_back(j);
+        s->pending.push_back(j);
+    }
+}
+
+// One tick; *n_active / *n_queued / *n_done receive the counts after it
+// (n_active counts the admitted jobs, n_done the failed ones too).  Every
+// job is at all times in exactly one of the queue, the admitted jobs
+// (pending or active) and the finished map.  The tick: the jobs admitted
+// during the previous tick join the active ones (the batches are rebuilt,
+// one per precision class); the active jobs' steps are enqueued; queued
+// jobs are admitted while the device runs them; then the tick waits,
+// checks every job and retires the finished and the failed ones (a job
+// whose particles ran away fails alone).  The tick itself fails only when
+// the device fails.
+int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_t *n_done) {
+    if (!s) return BDK_ERR_ARG;
+    double tp = now_s();
+    auto lap = [&](double &acc) {
+        const double t = now_s();
+        acc += t - tp;
+        tp = t;
+    };
+    if (s->active.empty() && s->pending.empty()) admit(s);  // nothing runs to overlap with
+    if (!s->pending.empty()) {
+        s->active.insert(s->active.end(), s->pending.begin(), s->pending.end());
+        s->pending.clear();
         s->dirty = true;
     }
+    lap(s->t_admit);
     if (!s->active.empty()) {
         if (s->dirty) {
             s->drop_batches();
@@ -201,6 +234,7 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
             }
             s->dirty = false;
         }
+        lap(s->t_batch);
         int64_t k = s->tick;
         for (Job *j : s->active) k = std::min(k, j->steps - j->done);
         const double t0 = now_s();
@@ -215,6 +249,9 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
             j->done += k;
             s->particle_steps += k * ((int64_t)j->p.n1 + j->p.n2);
         }
+        lap(s->t_run);
+        admit(s);  // beside the enqueued steps
+        lap(s->t_admit);
         const int n = (int)s->active.size();
         std::vector<int32_t> st(n, BDK_OK);
         for (bdk_server::Group &g : s->groups) {
@@ -224,6 +261,7 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
             for (size_t q = 0; q < g.idx.size(); ++q) st[g.idx[q]] = gs[q];
         }
         s->busy_s += now_s() - t0;
+        lap(s->t_run);
         std::vector<Job *> keep, out;
         std::vector<int> out_rc;
         std::vector<std::string> out_err;
@@ -254,8 +292,9 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
             s->active.swap(keep);
             for (size_t i = 0; i < out.size(); ++i) retire(s, out[i], out_rc[i], out_err[i]);
         }
+        lap(s->t_retire);
     }
-    if (n_active) *n_active = (int32_t)s->active.size();
+    if (n_active) *n_active = (int32_t)(s->active.size() + s->pending.size());
     if (n_queued) *n_queued = (int32_t)s->queue.size();
     if (n_done) *n_done = (int32_t)s->finished.size();
     return BDK_OK;
@@ -274,6 +313,15 @@ int bdk_serve_result(bdk_server_t *s, int64_t job_id, int32_t *git, double *rel,
     if (j->status) return s->fail(j->status, "job " + std::to_string(job_id) + " failed: " + j->err);
     if (git) std::memcpy(git, j->git.data(), j->git.size() * sizeof(int32_t));
     if (rel) std::memcpy(rel, j->rel.data(), j->rel.size() * sizeof(double));
+    return BDK_OK;
+}
+
+int bdk_serve_timing(const bdk_server_t *s, double *admit_s, double *batch_s, double *run_s, double *retire_s) {
+    if (!s) return BDK_ERR_ARG;
+    if (admit_s) *admit_s = s->t_admit;
+    if (batch_s) *batch_s = s->t_batch;
+    if (run_s) *run_s = s->t_run;
+    if (retire_s) *retire_s = s->t_retire;
     return BDK_OK;
 }
 

@@ -103,10 +103,12 @@ def time_batched(sims, steps, tune_reps=5):
     return sum(s.N for s in sims) * steps / dt, dt, ept, stages
 
 
-def serve(jobs, rate, hbm_budget=64 << 30, max_jobs=256, tick=256, seed=5):
+def serve(jobs, rate, hbm_budget=64 << 30, max_jobs=256, tick=256, seed=5, timeline=None):
     """Poisson arrivals at `rate` jobs/s into a Server; returns per-job
     latencies (arrival to completion, s), aggregate particle-steps/s over the
-    run and the server handle's results by job index."""
+    run and the server handle's results by job index.  timeline (a list)
+    receives (seconds, active, queued, particle-steps so far, jobs still to
+    arrive) after every tick."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
     srv = bdk.Server(hbm_budget, max_jobs=max_jobs, tick_steps=tick)
@@ -121,6 +123,8 @@ def serve(jobs, rate, hbm_budget=64 << 30, max_jobs=256, tick=256, seed=5):
             submitted[nxt] = time.perf_counter() - t0
             nxt += 1
         active, queued, done = srv.step()
+        if timeline is not None:
+            timeline.append((time.perf_counter() - t0, active, queued, srv.stats()[0], len(jobs) - nxt))
         if active == 0 and queued == 0 and nxt < len(jobs):
             time.sleep(max(0.0, min(0.01, arrivals[nxt] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
@@ -130,5 +134,26 @@ def serve(jobs, rate, hbm_budget=64 << 30, max_jobs=256, tick=256, seed=5):
         lat.append(submitted[j] - arrivals[j] + td)
         results[j] = (git, rel)
     ps, busy, _ = srv.stats()
+    if timeline is not None:
+        timeline.append(srv.timing())
     srv.close()
     return np.array(lat), ps / wall, wall, results
+
+
+def serve_phases(timeline):
+    """Split a serve() timeline where the last job has been admitted (no job
+    left to arrive and none queued): particle-steps/s before that point (the
+    device always had work offered) and in the tail after it (only the
+    admitted jobs left, the longest ones last), with the tail's length and
+    its share of the particle-steps."""
+    timing = timeline[-1] if timeline and isinstance(timeline[-1], dict) else {}
+    timeline = [t for t in timeline if not isinstance(t, dict)]
+    if not timeline:
+        return {}
+    cut = next((k for k, (_, _, q, _, left) in enumerate(timeline) if q == 0 and left == 0), len(timeline) - 1)
+    t_cut, ps_cut = timeline[cut][0], timeline[cut][3]
+    t_end, ps_end = timeline[-1][0], timeline[-1][3]
+    return {"offered_phase_s": t_cut, "offered_phase_value": ps_cut / t_cut if t_cut > 0 else None,
+            "tail_s": t_end - t_cut, "tail_value": (ps_end - ps_cut) / (t_end - t_cut) if t_end > t_cut else None,
+            "tail_share_of_particle_steps": (ps_end - ps_cut) / ps_end if ps_end else None,
+            "active_at_tail_start": timeline[cut][1], "server_host_seconds": timing}

@@ -285,10 +285,12 @@ def ensemble_figures(log_fn, cpu=False):
     work = float(np.mean([(p.n1 + p.n2) * st for p, _, _, st in jobs]))
     rate = 1.5 * ps_b / work
     log_fn("ensemble: serving %d jobs, Poisson arrivals at %.2f jobs/s" % (len(jobs), rate))
-    lat, ps_srv, wall, _ = ensemble.serve(jobs, rate, max_jobs=64)
+    tl = []
+    lat, ps_srv, wall, _ = ensemble.serve(jobs, rate, max_jobs=64, timeline=tl)
     served = {"jobs": len(jobs), "arrival_rate_per_s": rate, "p50_latency_s": float(np.percentile(lat, 50)),
               "p99_latency_s": float(np.percentile(lat, 99)), "aggregate_value": ps_srv,
-              "unit": "particle-steps/s", "wall_s": wall,
+              "unit": "particle-steps/s", "wall_s": wall, "phases": ensemble.serve_phases(tl),
+              "longest_job_particle_steps": max((p.n1 + p.n2) * st for p, _, _, st in jobs),
               "mix": "N log-uniform [1e3, 2.56e5], split U[0.2,0.8], density U[0.3,0.7], lambda U[5,100], "
                      "NGP/CIC/TSC, steps log-uniform [1e3, 1e5], uniform or blob start; offered load 1.5x "
                      "the batched throughput; budget 64 GiB, at most 64 jobs at once, 256-step ticks"}

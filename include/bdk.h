@@ -147,6 +147,12 @@ int bdk_steps_for_interval(double obs_dt, double dt, int64_t *steps_out);
 /* refresh_erwNachbar (dm:216-325) on the current positions. */
 int bdk_refresh(bdk_t *h);
 
+/* Upload the positions set with bdk_set_positions, rebuild the cell lists
+   and enqueue the forces of that state on the handle's stream, without
+   waiting (otherwise the first step or force call does this).  The state and
+   every later result are the same either way; a job server uses it to bring
+   a new job up while other simulations run. */
+int bdk_prepare(bdk_t *h);
 int bdk_sync(bdk_t *h);
 int64_t bdk_step_counter(const bdk_t *h);
 
@@ -418,6 +424,10 @@ int bdk_serve_step(bdk_server_t *s, int32_t *n_active, int32_t *n_queued, int32_
    writes only the times. */
 int bdk_serve_result(bdk_server_t *s, int64_t job_id, int32_t *git, double *rel, double *t_admit, double *t_done);
 int bdk_serve_stats(const bdk_server_t *s, int64_t *particle_steps, double *busy_s, uint64_t *hbm_used);
+/* Host seconds the ticks spent so far in admission (bdk_create and the
+   uploads), batch rebuilds, stepping (enqueue and wait) and retirement
+   (positions read back, device state freed); any pointer may be NULL. */
+int bdk_serve_timing(const bdk_server_t *s, double *admit_s, double *batch_s, double *run_s, double *retire_s);
 
 #ifdef __cplusplus
 }
