@@ -6,10 +6,11 @@
 //               (dm:116-157), refresh (dm:216-325); the cell counts are
 //               updated by the particles that change cell;
 // then on two streams side by side:
-//   k_rank_scan rank of each particle that stayed in its cell (no atomics)
-//               beside the exclusive scan of the cell counts;
+//   k_scan      exclusive scan of the cell counts;
 //   k_fill      cell list = slots grouped by cell, species in bit 31, and
 //               the particle state in list order (lcell, lrel, lpid, lrelf);
+//               particles that stayed in their cell are ranked here (no
+//               atomics);
 //   k_sort_cells  the arrivals of each cell in slot order;
 //   k_wca       WCA forces (dm:617-1059) of the new lists;
 // and
@@ -401,26 +402,75 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
 
 // FP32 forces for boxes of at least 6 cells per side: the displacement is
 // the difference of the in-cell offsets (FP32 copies centred on the cell,
-// lrelf) plus the cell offset, the minimum image by construction.  The
-// list entry and offsets of a candidate are loaded one candidate ahead of
-// the arithmetic (the list arrays carry one padding entry).  UNIT: all sigma
-// and epsilon ratios are 1, so the per-type factors drop out and one sum
-// collects all partners; otherwise same-species and cross-species partners
-// are summed apart and added at the end (the grouping of dm:617-702 +
-// dm:973-1059).  Partners are visited in list order, cell by cell.
-// FP32 WCA of one particle against candidates q0..qe-1 of one neighbour
-// row.  The cell offset is folded into this particle's coordinates: y + B
-// below m1 (yb), y below m2 (ym), y - B from m2 on (ya), x + ox (xo).
-// SELF: the row holds this particle (position qi, skipped).  Contraction
-// into FMAs is allowed in the FP32 force arithmetic, unlike the FP64
-// position arithmetic.  (Scalar arguments: a parameter struct here was
-// placed in LDS by the compiler.)
+// lrelf) plus the cell offset, the minimum image by construction.  UNIT: all
+// sigma and epsilon ratios are 1, so the per-type factors drop out and one
+// sum collects all partners; otherwise same-species and cross-species
+// partners are summed apart and added at the end (the grouping of
+// dm:617-702 + dm:973-1059).  Partners are visited in list order, cell by
+// cell, row by row.
+//
+// A block handles consecutive list positions.  Their neighbour cells in row
+// da (da = -1, 0, 1: the cells (cx + da, cy - 1 .. cy + 1)) form one
+// contiguous range of the list when no particle's row wraps around the box
+// in x; the block copies the three ranges' offsets (and entries) to LDS once
+// and every particle away from the y edge reads its candidates there.  The
+// rest (the first and last cell columns, rows wrapping in y, ranges beyond
+// the LDS capacity) read the list directly.  Both loops evaluate a candidate
+// with the same code, so a particle's force has the same bits either way
+// (and therefore for every block size).
+constexpr int WCA_STAGE = 2048;  // staged candidates per block
+
+// One candidate (entry e, centred offset r) at list position q.  The cell
+// offset is folded into this particle's coordinates: y + B below m1 (yb), y
+// below m2 (ym), y - B from m2 on (ya), x + ox (xo).  SELF: the row holds
+// this particle (position qi, excluded).  Outside the cutoff the force is
+// selected away rather than branched around (every wave holds some partner
+// inside, so the branch would not skip the arithmetic).  Contraction into
+// FMAs is allowed in the FP32 force arithmetic, unlike the FP64 position
+// arithmetic.
+template <bool UNIT, bool SELF>
+__device__ __forceinline__ void wca_f32_cand(unsigned e, float2 r, int q, int qi, bool s2i, int m1, int m2, float xo,
+                                             float yb, float ym, float ya, float cut2, float cap, float sigA,
+                                             float prefA, float sigB, float prefB, float &fsx, float &fsy,
+                                             float &fcx, float &fcy) {
+#pragma clang fp contract(fast)
+    const float yo = q < m1 ? yb : (q < m2 ? ym : ya);
+    const float dx = xo - r.x, dy = yo - r.y;
+    const float r2 = dx * dx + dy * dy;
+    if (UNIT) {
+        const float inv = __builtin_amdgcn_rcpf(r2);
+        const float i2 = inv * inv, i4 = i2 * i2;
+        float absF = fminf(24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
+        absF = (r2 <= cut2 && (!SELF || q != qi)) ? absF : 0.0f;
+        fsx += absF * dx;
+        fsy += absF * dy;
+    } else {
+        const bool s2j = (e >> 31) != 0;
+        const float sig = s2j ? sigB : sigA;
+        const float r2s = r2 * sig * sig;
+        const float inv = __builtin_amdgcn_rcpf(r2s);
+        const float i2 = inv * inv, i4 = i2 * i2;
+        float absF = fminf((s2j ? prefB : prefA) * 24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
+        absF = (r2s <= cut2 && (!SELF || q != qi)) ? absF : 0.0f;
+        const float fx = absF * dx * sig, fy = absF * dy * sig;
+        if (s2i == s2j) {
+            fsx += fx;
+            fsy += fy;
+        } else {
+            fcx += fx;
+            fcy += fy;
+        }
+    }
+}
+
+// candidates q0..qe-1 read from the list, loaded one candidate ahead of the
+// arithmetic (the list arrays carry one padding entry).  (Scalar arguments:
+// a parameter struct here was placed in LDS by the compiler.)
 template <bool UNIT, bool SELF>
 __device__ __forceinline__ void wca_f32_row(const unsigned *__restrict__ list, const float2 *__restrict__ rf, int qi,
                                             bool s2i, int q0, int qe, int m1, int m2, float xo, float yb, float ym,
                                             float ya, float cut2, float cap, float sigA, float prefA, float sigB,
                                             float prefB, float &fsx, float &fsy, float &fcx, float &fcy) {
-#pragma clang fp contract(fast)
     unsigned ent = list[q0];
     float2 rj = rf[q0];
     for (int q = q0; q < qe; ++q) {
@@ -428,45 +478,72 @@ __device__ __forceinline__ void wca_f32_row(const unsigned *__restrict__ list, c
         const float2 r = rj;
         ent = list[q + 1];  // q + 1 <= N: padding entry
         rj = rf[q + 1];
-        const float yo = q < m1 ? yb : (q < m2 ? ym : ya);
-        const float dx = xo - r.x, dy = yo - r.y;
-        const float r2 = dx * dx + dy * dy;
-        if (UNIT) {
-            if (r2 <= cut2 && (!SELF || q != qi)) {
-                const float inv = __builtin_amdgcn_rcpf(r2);
-                const float i2 = inv * inv, i4 = i2 * i2;
-                const float absF = fminf(24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
-                fsx += absF * dx;
-                fsy += absF * dy;
-            }
-        } else {
-            const bool s2j = (e >> 31) != 0;
-            const float sig = s2j ? sigB : sigA;
-            const float r2s = r2 * sig * sig;
-            if (r2s <= cut2 && (!SELF || q != qi)) {
-                const float inv = __builtin_amdgcn_rcpf(r2s);
-                const float i2 = inv * inv, i4 = i2 * i2;
-                const float absF = fminf((s2j ? prefB : prefA) * 24.0f * (2.0f * (i4 * i2 * inv) - i4), cap);
-                const float fx = absF * dx * sig, fy = absF * dy * sig;
-                if (s2i == s2j) {
-                    fsx += fx;
-                    fsy += fy;
-                } else {
-                    fcx += fx;
-                    fcy += fy;
-                }
-            }
-        }
+        wca_f32_cand<UNIT, SELF>(e, r, q, qi, s2i, m1, m2, xo, yb, ym, ya, cut2, cap, sigA, prefA, sigB, prefB, fsx,
+                                 fsy, fcx, fcy);
     }
 }
 
+// the same candidates read from the staged copy (list position q at q + delta)
+template <bool UNIT, bool SELF>
+__device__ __forceinline__ void wca_f32_row_lds(const float2 *sp, const unsigned *se, int delta, int qi, bool s2i,
+                                                int q0, int qe, int m1, int m2, float xo, float yb, float ym,
+                                                float ya, float cut2, float cap, float sigA, float prefA, float sigB,
+                                                float prefB, float &fsx, float &fsy, float &fcx, float &fcy) {
+    float2 rn = sp[q0 + delta];
+    unsigned en = UNIT ? 0u : se[q0 + delta];
+    for (int q = q0; q < qe; ++q) {
+        const float2 r = rn;
+        const unsigned e = en;
+        rn = sp[q + 1 + delta];  // one ahead: the arrays hold WCA_STAGE + 1 entries
+        if (!UNIT) en = se[q + 1 + delta];
+        wca_f32_cand<UNIT, SELF>(e, r, q, qi, s2i, m1, m2, xo, yb, ym, ya, cut2, cap, sigA, prefA, sigB, prefB, fsx,
+                                 fsy, fcx, fcy);
+    }
+}
+
+// LDS of the staged rows: offsets sp[WCA_STAGE + 1], entries
+// se[WCA_STAGE + 1] (not UNIT; the extra entry is read ahead, never used),
+// and hdr[8] (each row's first list position and length, and whether the
+// block stages).  Every thread of the block calls this.
 template <bool UNIT>
-__device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int blk) {
-    const int qi = blk * blockDim.x + threadIdx.x;
-    if (qi >= g.N) return;
+__device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int blk, float2 *sp, unsigned *se,
+                                             int *hdr) {
+    const int nb = blockDim.x, tid = threadIdx.x;
+    const int q_first = blk * nb;
+    const int q_last = min(q_first + nb, g.N) - 1;
     const int nc = g.nc;
     const unsigned *list = (const unsigned *)d.list;
     const float2 *rf = d.lrelf;
+    if (tid < 3) {
+        // lane da + 1: the list range of the cells (cx + da, cy - 1 .. cy + 1)
+        // over the block's cells c_first..c_last (list order is cell order)
+        const int cf = d.lcell[q_first], cl = d.lcell[q_last];
+        const int ok = div_nc(cf, g) >= 1 && div_nc(cl, g) <= nc - 2;
+        int g0 = 0, len = 0;
+        if (ok) {
+            const int da = tid - 1;
+            const int lo = max(cf + da * nc - 1, 0), hi = min(cl + da * nc + 1, g.ncell - 1);
+            g0 = d.start[lo];
+            len = d.start[hi + 1] - g0;
+        }
+        hdr[tid] = g0;
+        hdr[3 + tid] = len;
+        if (tid == 0) hdr[6] = ok;
+    }
+    __syncthreads();
+    const int l0 = hdr[3], l1 = hdr[4], total = l0 + l1 + hdr[5];
+    const bool staged = hdr[6] && total <= WCA_STAGE;
+    if (staged) {
+        for (int t = tid; t < total; t += nb) {
+            const int r = (t >= l0) + (t >= l0 + l1);
+            const int q = hdr[r] + t - (r == 0 ? 0 : (r == 1 ? l0 : l0 + l1));
+            sp[t] = rf[q];
+            if (!UNIT) se[t] = list[q];
+        }
+    }
+    __syncthreads();
+    const int qi = q_first + tid;
+    if (qi > q_last) return;
     const int c = d.lcell[qi];
     const int cx = div_nc(c, g), cy = c - cx * nc;
     const float2 ri = rf[qi];
@@ -477,34 +554,51 @@ __device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int
     const float sigB = s2i ? (float)g.s22 : (float)g.s12, prefB = s2i ? (float)g.e22f : (float)g.e12f;
     const float yb = ri.y + B, ym = ri.y, ya = ri.y - B;
     float fsx = 0, fsy = 0, fcx = 0, fcy = 0;
+    // cells cy-1, cy, cy+1 of a row are adjacent in the list
+    const bool inner = cy >= 1 && cy <= nc - 2;
+    if (staged && inner) {
+#define BDK_WCA_ROW(SELF)                                                                                       \
+    wca_f32_row_lds<UNIT, SELF>(sp, se, delta, qi, s2i, s0, s3, s1, s2, xo, yb, ym, ya, cut2, cap, sigA, prefA, \
+                                sigB, prefB, fsx, fsy, fcx, fcy)
+        for (int da = -1; da <= 1; ++da) {
+            const float xo = ri.x - da * B;
+            const int *st = d.start + (cx + da) * nc + cy - 1;
+            const int s0 = st[0], s1 = st[1], s2 = st[2], s3 = st[3];
+            const int delta = (da < 0 ? 0 : (da == 0 ? l0 : l0 + l1)) - hdr[da + 1];
+            if (da == 0)
+                BDK_WCA_ROW(true);
+            else
+                BDK_WCA_ROW(false);
+        }
+#undef BDK_WCA_ROW
+    } else {
 #define BDK_WCA_ROW(SELF, Q0, QE, M1, M2)                                                                    \
     wca_f32_row<UNIT, SELF>(list, rf, qi, s2i, Q0, QE, M1, M2, xo, yb, ym, ya, cut2, cap, sigA, prefA, sigB, \
                             prefB, fsx, fsy, fcx, fcy)
-    const bool inner = cy >= 1 && cy <= nc - 2;
-    for (int da = -1; da <= 1; ++da) {
-        const int row = wrap1(cx + da, nc) * nc;
-        const float xo = ri.x - da * B;
-        if (inner) {
-            // cells cy-1, cy, cy+1 of a row are adjacent in the list
-            const int *st = d.start + row + cy - 1;
-            const int s0 = st[0], s1 = st[1], s2 = st[2], s3 = st[3];
-            if (da == 0)
-                BDK_WCA_ROW(true, s0, s3, s1, s2);
-            else
-                BDK_WCA_ROW(false, s0, s3, s1, s2);
-        } else {
-            for (int db = -1; db <= 1; ++db) {
-                const int cc = row + wrap1(cy + db, nc);
-                const int q0 = d.start[cc], qe = d.start[cc + 1];
-                const int m1 = db < 0 ? qe : q0, m2 = db > 0 ? q0 : qe;
-                if (da == 0 && db == 0)
-                    BDK_WCA_ROW(true, q0, qe, m1, m2);
+        for (int da = -1; da <= 1; ++da) {
+            const int row = wrap1(cx + da, nc) * nc;
+            const float xo = ri.x - da * B;
+            if (inner) {
+                const int *st = d.start + row + cy - 1;
+                const int s0 = st[0], s1 = st[1], s2 = st[2], s3 = st[3];
+                if (da == 0)
+                    BDK_WCA_ROW(true, s0, s3, s1, s2);
                 else
-                    BDK_WCA_ROW(false, q0, qe, m1, m2);
+                    BDK_WCA_ROW(false, s0, s3, s1, s2);
+            } else {
+                for (int db = -1; db <= 1; ++db) {
+                    const int cc = row + wrap1(cy + db, nc);
+                    const int q0 = d.start[cc], qe = d.start[cc + 1];
+                    const int m1 = db < 0 ? qe : q0, m2 = db > 0 ? q0 : qe;
+                    if (da == 0 && db == 0)
+                        BDK_WCA_ROW(true, q0, qe, m1, m2);
+                    else
+                        BDK_WCA_ROW(false, q0, qe, m1, m2);
+                }
             }
         }
-    }
 #undef BDK_WCA_ROW
+    }
     float2 o;
     o.x = fsx + fcx;
     o.y = fsy + fcy;
@@ -513,7 +607,10 @@ __device__ __forceinline__ void wca_f32_body(const Geo &g, const DevBufs &d, int
 
 template <bool UNIT>
 __global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
-    wca_f32_body<UNIT>(g, d, xcd_block(g.xcd));
+    __shared__ float2 sp[WCA_STAGE + 1];
+    __shared__ unsigned se[UNIT ? 1 : WCA_STAGE + 1];
+    __shared__ int hdr[8];
+    wca_f32_body<UNIT>(g, d, xcd_block(g.xcd), sp, se, hdr);
 }
 
 // ------------------------------------------------------------------ deposit
@@ -1468,7 +1565,8 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
     if (nc_new != c) {
         // a cell change (a few percent of the particles per step): the
         // counts follow, and the rank among the new cell's arrivals comes
-        // from arr; particles that stay are ranked by k_rank without atomics
+        // from arr; particles that stay (rank 0 here) are ranked by the
+        // fill without atomics
         atomicSub(&d.count[c], 1);
         atomicAdd(&d.count[nc_new], 1);
         if ((c >> SCAN_SHIFT) != (nc_new >> SCAN_SHIFT)) {
@@ -1476,6 +1574,8 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
             atomicAdd(&d.bcount[(nc_new >> SCAN_SHIFT) * BCOUNT_STRIDE], 1);
         }
         d.rank[s] = RANK_MOVED | atomicAdd(&d.arr[nc_new], 1);
+    } else {
+        d.rank[s] = 0;
     }
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
     if (ex > 0.0) atomicMax(&d.sc->exc_bits[(step + 1) & 1], bitsd(ex));
@@ -1666,41 +1766,24 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *__restrict__ c
     scan_body(count, bcount, start, ncell, nb, nmax, sc, inc_step, blockIdx.x);
 }
 
-// Rank of a particle that stayed in its cell during the step: its position
-// among the cell's previous entries, minus the entries before it that left.
-// Reads the previous starts from the update's copy, so it runs beside the
-// scan that overwrites start.
-__device__ __forceinline__ void rank_body(const Geo &g, const DevBufs &d, int blk) {
-    const int q = blk * blockDim.x + threadIdx.x;
-    if (q >= g.N) return;
-    const int c = d.lcell[q];
-    if (d.cell[q] != c) return;  // moved: ranked among the arrivals
-    const int b = d.start_old[c];
-    int r = q - b;
-    for (int p = b; p < q; ++p) r -= d.cell[p] != d.lcell[p];
-    d.rank[q] = r;
-}
-
-// the ranks and the scan of the new counts as one launch: blocks [0, nrank)
-// rank, the others scan
-__global__ __launch_bounds__(256) void k_rank_scan(Geo g, DevBufs d, int nrank, int nb, int inc_step) {
-    if ((int)blockIdx.x < nrank)
-        rank_body(g, d, blockIdx.x);
-    else
-        scan_body(d.count, d.bcount, d.start, g.ncell, nb, g.N, d.sc, inc_step, blockIdx.x - nrank);
-}
-
 // List entries and the list-ordered state from the ranks.  After a step
-// (moved_ranks) a cell holds its stayers first (ascending slot, from
-// k_rank), then its arrivals (marked ranks, ordered by k_sort_cells); after
-// a full binning every rank is the position in the cell (ordered by
-// k_sort_cells).
+// (moved_ranks) a cell holds its stayers first (ascending slot), then its
+// arrivals (marked ranks, ordered by k_sort_cells); after a full binning
+// every rank is the position in the cell (ordered by k_sort_cells).  The
+// rank of a stayer (unmarked after a step) is its position among its cell's
+// previous entries (the slots start_old[c]..) minus the entries before it
+// that left: those carry marked ranks.
 __device__ __forceinline__ void fill_body(const Geo &g, const DevBufs &d, int moved_ranks, int blk) {
     const int s = blk * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
     const int pid = d.pid[s];
     const int c = d.cell[s];
-    const int rk = d.rank[s];
+    int rk = d.rank[s];
+    if (moved_ranks && !(rk & RANK_MOVED)) {
+        const int b = d.start_old[c];
+        rk = s - b;
+        for (int p = b; p < s; ++p) rk -= (d.rank[p] >> 30) & 1;  // RANK_MOVED
+    }
     int pos = d.start[c] + (rk & (RANK_MOVED - 1));
     if (moved_ranks && (rk & RANK_MOVED)) pos += d.start[c + 1] - d.start[c] - d.arr[c];
     if ((unsigned)pos >= (unsigned)g.N) {  // inconsistent counts: flagged, nothing written
@@ -1807,14 +1890,18 @@ template <bool P, bool SYNC, typename F> __device__ __forceinline__ void for_ent
 
 template <typename R, bool P>
 __global__ __launch_bounds__(512) void kb_wca(const RepDev *m, const int2 *map, int nmap) {
-    for_entries<P, false>(nmap, [&](int e_) {
+    constexpr int S = std::is_same<R, float>::value ? WCA_STAGE + 1 : 1;  // FP64 members read the list directly
+    __shared__ float2 sp[S];
+    __shared__ unsigned se[S];
+    __shared__ int hdr[8];
+    for_entries<P, true>(nmap, [&](int e_) {
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
         if (std::is_same<R, float>::value && r.g.fast_wca) {
             if (r.unit)
-                wca_f32_body<true>(r.g, r.d, t.y);
+                wca_f32_body<true>(r.g, r.d, t.y, sp, se, hdr);
             else
-                wca_f32_body<false>(r.g, r.d, t.y);
+                wca_f32_body<false>(r.g, r.d, t.y, sp, se, hdr);
         } else {
             wca_ref_body<R>(r.g, r.d, t.y);
         }
@@ -1922,19 +2009,12 @@ __global__ __launch_bounds__(64) void kb_dt_fold(const RepDev *m, const int2 *ma
     });
 }
 
-// entries [0, nrank): ranks (map mp); entries [nrank, nrank + nscan): scans (map ms)
 template <bool P>
-__global__ __launch_bounds__(256) void kb_rank_scan(const RepDev *m, const int2 *mp, int nrank, const int2 *ms,
-                                                    int nscan) {
-    for_entries<P, true>(nrank + nscan, [&](int e_) {
-        if (e_ < nrank) {
-            const int2 t = mp[e_];
-            rank_body(m[t.x].g, m[t.x].d, t.y);
-        } else {
-            const int2 t = ms[e_ - nrank];
-            const RepDev &r = m[t.x];
-            scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
-        }
+__global__ __launch_bounds__(SCAN_THREADS) void kb_scan(const RepDev *m, const int2 *map, int nmap) {
+    for_entries<P, true>(nmap, [&](int e_) {
+        const int2 t = map[e_];
+        const RepDev &r = m[t.x];
+        scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
     });
 }
 
@@ -2838,7 +2918,7 @@ hipError_t enqueue_forces(bdk_t *h, std::vector<std::string> *names, std::vector
 // One step from a state whose forces are ready: the update (gather, noise,
 // Euler step, refresh), then the two halves that only need its output side
 // by side:
-//   main stream: the list rebuild (rank, scan, fill, sort), then the WCA
+//   main stream: the list rebuild (scan, fill, sort), then the WCA
 //                forces of the new lists;
 //   second stream: the deposit from the slot arrays (grouped by the previous
 //                starts, which the update kept; halo widened by the
@@ -2892,11 +2972,11 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         }
         if (fork) m.chk(hipEventRecord(h->ev_join, side));
     }
-    // bcount is current (the update adjusted it with the counts); the ranks
-    // read the previous starts that the update kept
-    const int nrank = (int)nblk(h->g.N, 256);
-    hipLaunchKernelGGL(k_rank_scan, dim3(nrank + nb), dim3(256), 0, h->stream, h->g, h->d, nrank, nb, !kap);
-    m.mark("k_rank_scan");
+    // bcount is current (the update adjusted it with the counts); the fill
+    // ranks the stayers from the previous starts that the update kept
+    hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start,
+                       ncell, nb, h->g.N, h->d.sc, !kap);
+    m.mark("k_scan");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     m.mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
@@ -3026,8 +3106,8 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
         }
         break;
     case 6:
-        hipLaunchKernelGGL(kb_rank_scan<P>, dim3(stage_grid(bp, bp.nblk[BS_PART] + bp.nblk[BS_SCAN])), dim3(256), 0, st,
-                           m, bp.map[BS_PART], bp.nblk[BS_PART], bp.map[BS_SCAN], bp.nblk[BS_SCAN]);
+        hipLaunchKernelGGL(kb_scan<P>, BDK_G(BS_SCAN), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN],
+                           bp.nblk[BS_SCAN]);
         break;
     default: {  // 11: fill, sort
         hipLaunchKernelGGL(kb_fill<P>, BDK_G(BS_PART), dim3(256), 0, st, m, bp.map[BS_PART], bp.nblk[BS_PART]);
@@ -3046,7 +3126,7 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
 }
 
 // stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update,
-// 6 rank + scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end,
+// 6 scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end,
 // 13 gather + dt fold of the adaptive members
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)
