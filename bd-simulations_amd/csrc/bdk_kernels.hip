@@ -6,11 +6,10 @@
 //               (dm:116-157), refresh (dm:216-325); the cell counts are
 //               updated by the particles that change cell;
 // then on two streams side by side:
-//   k_scan      exclusive scan of the cell counts;
+//   k_rank_scan rank of each particle that stayed in its cell (no atomics)
+//               beside the exclusive scan of the cell counts;
 //   k_fill      cell list = slots grouped by cell, species in bit 31, and
 //               the particle state in list order (lcell, lrel, lpid, lrelf);
-//               particles that stayed in their cell are ranked here (no
-//               atomics);
 //   k_sort_cells  the arrivals of each cell in slot order;
 //   k_wca       WCA forces (dm:617-1059) of the new lists;
 // and
@@ -1565,8 +1564,7 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
     if (nc_new != c) {
         // a cell change (a few percent of the particles per step): the
         // counts follow, and the rank among the new cell's arrivals comes
-        // from arr; particles that stay (rank 0 here) are ranked by the
-        // fill without atomics
+        // from arr; particles that stay are ranked by k_rank without atomics
         atomicSub(&d.count[c], 1);
         atomicAdd(&d.count[nc_new], 1);
         if ((c >> SCAN_SHIFT) != (nc_new >> SCAN_SHIFT)) {
@@ -1574,8 +1572,6 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
             atomicAdd(&d.bcount[(nc_new >> SCAN_SHIFT) * BCOUNT_STRIDE], 1);
         }
         d.rank[s] = RANK_MOVED | atomicAdd(&d.arr[nc_new], 1);
-    } else {
-        d.rank[s] = 0;
     }
     double ex = fmax(fmax(-rn.x, rn.x - B), fmax(-rn.y, rn.y - B));
     if (ex > 0.0) atomicMax(&d.sc->exc_bits[(step + 1) & 1], bitsd(ex));
@@ -1766,24 +1762,40 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int *__restrict__ c
     scan_body(count, bcount, start, ncell, nb, nmax, sc, inc_step, blockIdx.x);
 }
 
+// Rank of a particle that stayed in its cell during the step: its position
+// among the cell's previous entries, minus the entries before it that left.
+// Reads the previous starts from the update's copy, so it runs beside the
+// scan that overwrites start.
+__device__ __forceinline__ void rank_body(const Geo &g, const DevBufs &d, int blk) {
+    const int q = blk * blockDim.x + threadIdx.x;
+    if (q >= g.N) return;
+    const int c = d.lcell[q];
+    if (d.cell[q] != c) return;  // moved: ranked among the arrivals
+    const int b = d.start_old[c];
+    int r = q - b;
+    for (int p = b; p < q; ++p) r -= d.cell[p] != d.lcell[p];
+    d.rank[q] = r;
+}
+
+// the ranks and the scan of the new counts as one launch: blocks [0, nrank)
+// rank, the others scan
+__global__ __launch_bounds__(256) void k_rank_scan(Geo g, DevBufs d, int nrank, int nb, int inc_step) {
+    if ((int)blockIdx.x < nrank)
+        rank_body(g, d, blockIdx.x);
+    else
+        scan_body(d.count, d.bcount, d.start, g.ncell, nb, g.N, d.sc, inc_step, blockIdx.x - nrank);
+}
 // List entries and the list-ordered state from the ranks.  After a step
-// (moved_ranks) a cell holds its stayers first (ascending slot), then its
-// arrivals (marked ranks, ordered by k_sort_cells); after a full binning
-// every rank is the position in the cell (ordered by k_sort_cells).  The
-// rank of a stayer (unmarked after a step) is its position among its cell's
-// previous entries (the slots start_old[c]..) minus the entries before it
-// that left: those carry marked ranks.
+// (moved_ranks) a cell holds its stayers first (ascending slot, from
+// k_rank), then its arrivals (marked ranks, ordered by k_sort_cells); after
+// a full binning every rank is the position in the cell (ordered by
+// k_sort_cells).
 __device__ __forceinline__ void fill_body(const Geo &g, const DevBufs &d, int moved_ranks, int blk) {
     const int s = blk * blockDim.x + threadIdx.x;
     if (s >= g.N) return;
     const int pid = d.pid[s];
     const int c = d.cell[s];
-    int rk = d.rank[s];
-    if (moved_ranks && !(rk & RANK_MOVED)) {
-        const int b = d.start_old[c];
-        rk = s - b;
-        for (int p = b; p < s; ++p) rk -= (d.rank[p] >> 30) & 1;  // RANK_MOVED
-    }
+    const int rk = d.rank[s];
     int pos = d.start[c] + (rk & (RANK_MOVED - 1));
     if (moved_ranks && (rk & RANK_MOVED)) pos += d.start[c + 1] - d.start[c] - d.arr[c];
     if ((unsigned)pos >= (unsigned)g.N) {  // inconsistent counts: flagged, nothing written
@@ -2009,12 +2021,19 @@ __global__ __launch_bounds__(64) void kb_dt_fold(const RepDev *m, const int2 *ma
     });
 }
 
+// entries [0, nrank): ranks (map mp); entries [nrank, nrank + nscan): scans (map ms)
 template <bool P>
-__global__ __launch_bounds__(SCAN_THREADS) void kb_scan(const RepDev *m, const int2 *map, int nmap) {
-    for_entries<P, true>(nmap, [&](int e_) {
-        const int2 t = map[e_];
-        const RepDev &r = m[t.x];
-        scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
+__global__ __launch_bounds__(256) void kb_rank_scan(const RepDev *m, const int2 *mp, int nrank, const int2 *ms,
+                                                    int nscan) {
+    for_entries<P, true>(nrank + nscan, [&](int e_) {
+        if (e_ < nrank) {
+            const int2 t = mp[e_];
+            rank_body(m[t.x].g, m[t.x].d, t.y);
+        } else {
+            const int2 t = ms[e_ - nrank];
+            const RepDev &r = m[t.x];
+            scan_body(r.d.count, r.d.bcount, r.d.start, r.g.ncell, r.scan_nb, r.g.N, r.d.sc, 0, t.y);
+        }
     });
 }
 
@@ -2918,7 +2937,7 @@ hipError_t enqueue_forces(bdk_t *h, std::vector<std::string> *names, std::vector
 // One step from a state whose forces are ready: the update (gather, noise,
 // Euler step, refresh), then the two halves that only need its output side
 // by side:
-//   main stream: the list rebuild (scan, fill, sort), then the WCA
+//   main stream: the list rebuild (rank, scan, fill, sort), then the WCA
 //                forces of the new lists;
 //   second stream: the deposit from the slot arrays (grouped by the previous
 //                starts, which the update kept; halo widened by the
@@ -2972,11 +2991,11 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         }
         if (fork) m.chk(hipEventRecord(h->ev_join, side));
     }
-    // bcount is current (the update adjusted it with the counts); the fill
-    // ranks the stayers from the previous starts that the update kept
-    hipLaunchKernelGGL(k_scan, dim3(nb), dim3(SCAN_THREADS), 0, h->stream, h->d.count, h->d.bcount, h->d.start,
-                       ncell, nb, h->g.N, h->d.sc, !kap);
-    m.mark("k_scan");
+    // bcount is current (the update adjusted it with the counts); the ranks
+    // read the previous starts that the update kept
+    const int nrank = (int)nblk(h->g.N, 256);
+    hipLaunchKernelGGL(k_rank_scan, dim3(nrank + nb), dim3(256), 0, h->stream, h->g, h->d, nrank, nb, !kap);
+    m.mark("k_rank_scan");
     hipLaunchKernelGGL(k_fill, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
     m.mark("k_fill");
     hipLaunchKernelGGL(k_sort_cells, dim3(nblk(ncell, 256)), dim3(256), 0, h->stream, h->g, h->d, 1);
@@ -3106,8 +3125,8 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
         }
         break;
     case 6:
-        hipLaunchKernelGGL(kb_scan<P>, BDK_G(BS_SCAN), dim3(SCAN_THREADS), 0, st, m, bp.map[BS_SCAN],
-                           bp.nblk[BS_SCAN]);
+        hipLaunchKernelGGL(kb_rank_scan<P>, dim3(stage_grid(bp, bp.nblk[BS_PART] + bp.nblk[BS_SCAN])), dim3(256), 0, st,
+                           m, bp.map[BS_PART], bp.nblk[BS_PART], bp.map[BS_SCAN], bp.nblk[BS_SCAN]);
         break;
     default: {  // 11: fill, sort
         hipLaunchKernelGGL(kb_fill<P>, BDK_G(BS_PART), dim3(256), 0, st, m, bp.map[BS_PART], bp.nblk[BS_PART]);
@@ -3126,7 +3145,7 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
 }
 
 // stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update,
-// 6 scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end,
+// 6 rank + scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end,
 // 13 gather + dt fold of the adaptive members
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)

@@ -50,7 +50,7 @@ def rows(directory, suffix):
                 yield r
 
 
-STEP_KERNELS = ("k_update", "k_scan", "k_fill", "k_sort_cells", "k_wca_f32", "k_deposit", "k_r2c_rows",
+STEP_KERNELS = ("k_update", "k_rank_scan", "k_fill", "k_sort_cells", "k_wca_f32", "k_deposit", "k_r2c_rows",
                 "k_col", "k_c2r_rows")
 
 

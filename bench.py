@@ -67,7 +67,7 @@ def canonical_step_bytes(N, Z):
 
 
 # the kernels of one graph-replayed step (fixed dt, FP32 forces, TSC)
-STEP_KERNELS = ("k_update", "k_scan", "k_fill", "k_sort_cells", "k_wca_f32", "k_deposit", "k_r2c_rows",
+STEP_KERNELS = ("k_update", "k_rank_scan", "k_fill", "k_sort_cells", "k_wca_f32", "k_deposit", "k_r2c_rows",
                 "k_col", "k_c2r_rows")
 # bench names of the kernels timed one at a time -> names in the profiler's records
 TRACE_NAME = {"k_wca": "k_wca_f32"}
@@ -410,7 +410,7 @@ def main():
                              for k, ms in kernels.items()},
         # measured HBM bytes of the list-rebuild kernels (implementation
         # overhead outside the canonical bytes)
-        "rebuild_traffic": {k: load_traffic(k) for k in ("k_scan", "k_fill", "k_sort_cells")},
+        "rebuild_traffic": {k: load_traffic(k) for k in ("k_rank_scan", "k_fill", "k_sort_cells")},
     }
     sim.close()
     out["adaptive_dt"] = adaptive_figure(bdk, w, git, rel, args.steps, args.warmup, N)
