@@ -417,7 +417,11 @@ __global__ __launch_bounds__(256) void k_wca(Geo g, DevBufs d) {
 // the LDS capacity) read the list directly.  Both loops evaluate a candidate
 // with the same code, so a particle's force has the same bits either way
 // (and therefore for every block size).
-constexpr int WCA_STAGE = 2048;  // staged candidates per block
+// staged candidates per block: 8 KB of offsets, small enough to share a CU
+// with the FFT passes of the other stream (measured in the two-stream step:
+// 2048 entries 185.5 us, 1024 180.7 us, 768 183.4 us per step; a dense
+// 256-particle block stages about 800)
+constexpr int WCA_STAGE = 1024;
 
 // One candidate (entry e, centred offset r) at list position q.  The cell
 // offset is folded into this particle's coordinates: y + B below m1 (yb), y
