@@ -153,6 +153,27 @@ __device__ __forceinline__ int wrapc(int u, int n) {
 }
 // u in [-n, 2n)
 __device__ __forceinline__ int wrap1(int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); }
+// mesh index u in [-Z, 2Z) into [0, Z): a mask when Z is a power of two
+__device__ __forceinline__ int wrapz(int u, int Z) {
+    if ((Z & (Z - 1)) == 0) return u & (Z - 1);
+    return wrap1(u, Z);
+}
+// x / d and x % d for 0 <= x < 2^22: a shift when d is a power of two, else
+// the float reciprocal corrected to the exact quotient
+struct Dv {
+    int d, sh;
+    float inv;
+    __device__ explicit Dv(int d_)
+        : d(d_), sh((d_ & (d_ - 1)) == 0 ? __ffs(d_) - 1 : -1), inv(__builtin_amdgcn_rcpf((float)d_)) {}
+    __device__ int q(int x) const {
+        if (sh >= 0) return x >> sh;
+        int v = (int)((float)x * inv);
+        const int r = x - v * d;
+        v += (r >= d) - (r < 0);
+        return v;
+    }
+    __device__ int r(int x) const { return sh >= 0 ? (x & (d - 1)) : x - q(x) * d; }
+};
 // c / nc for 0 <= c < 2^24 (float reciprocal, corrected to the exact quotient)
 __device__ __forceinline__ int div_nc(int c, const Geo &g) {
     int q = (int)((float)c * g.ncinv);
@@ -660,7 +681,7 @@ __device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs
     float wx[3], wy[3];
     weights_r<float, SCHEME>(x, g, kp, wx);
     weights_r<float, SCHEME>(y, g, lp, wy);
-    const int kb = (kp - 1 - kw0) & (g.Z - 1), lb = (lp - 1 - lw0) & (g.Z - 1);
+    const int kb = wrapz(kp - 1 - kw0, g.Z), lb = wrapz(lp - 1 - lw0, g.Z);
     if (kb > nwx - 3 || lb > nwy - 3) {  // outside the window: not reachable, flagged
         atomicOr(&d.sc->nonfinite, 4);
         return;
@@ -696,13 +717,13 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
     for (int ia = 0; ia < 3; ++ia) {
         if (SCHEME == 0 && ia != 1) continue;
         if (wx[ia] == 0.0) continue;
-        const int k = (kp + ia - 1) & (Z - 1);
+        const int k = wrapz(kp + ia - 1, Z);
         if (k < klo || k >= khi) continue;
 #pragma unroll
         for (int ib = 0; ib < 3; ++ib) {
             if (SCHEME == 0 && ib != 1) continue;
             if (wy[ib] == 0.0) continue;
-            const int l = (lp + ib - 1) & (Z - 1);
+            const int l = wrapz(lp + ib - 1, Z);
             if (l < llo || l >= lhi) continue;
             const int w = (k - klo) * nl + (l - llo);
             if (std::is_same<R, float>::value) {
@@ -893,7 +914,7 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
         const int k = klo + qk, l = llo + rl;
         double v;
         if (win) {
-            v = (double)(long long)acc_hi[((k - kw0) & (g.Z - 1)) * nwy + ((l - lw0) & (g.Z - 1))] * a.iscale_f;
+            v = (double)(long long)acc_hi[wrapz(k - kw0, g.Z) * nwy + wrapz(l - lw0, g.Z)] * a.iscale_f;
         } else if (std::is_same<R, float>::value) {
             v = (double)(long long)acc_hi[w] * a.iscale_f;
         } else {
@@ -931,7 +952,7 @@ __global__ void k_rearm(Geo g, DevScalars *sc) {
     sc->exc_bits[(step + 1) & 1] = 0ull;
 }
 // ------------------------------------------------------------------ FFT in LDS
-// tw[k] = exp(-2 pi i k / 2^logZ); the inverse transforms use the conjugates.
+// tw[k] = exp(-2 pi i k / Z), k < Z; the inverse transforms use the conjugates.
 // radix-4 butterfly in place, y = DFT4(v) (inverse: conjugate kernel)
 template <typename C>
 __device__ __forceinline__ void dft4(C *v, bool inv) {
@@ -1076,10 +1097,128 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
     __syncthreads();
 }
 
+// radix-3 butterfly, y = DFT3(v): W = exp(-+2 pi i / 3) = -1/2 -+ i sqrt(3)/2
+template <typename C>
+__device__ __forceinline__ void dft3(const C *v, C *y, bool inv) {
+    using R = decltype(C::x);
+    const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+    const C s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    const C m = cm<C>(v[0].x - (R)0.5 * s.x, v[0].y - (R)0.5 * s.y);
+    const C r = inv ? cm<C>(-h * d.y, h * d.x) : cm<C>(h * d.y, -h * d.x);  // (+-i) sqrt(3)/2 (v1 - v2)
+    y[0] = cadd(v[0], s);
+    y[1] = cadd(m, r);
+    y[2] = csub(m, r);
+}
+
+// One Stockham pass of radix RAD (2, 3, 4 or 8) over rows of any length n
+// divisible by RAD (the meshes whose size has a factor 3).  The same
+// butterflies as fft_pass, with per-element LDS addresses (lp of every
+// index), exact integer division for the butterfly and row indices, and
+// every twiddle power loaded from the table: tw[t k Z / (Ns RAD)].  Inputs
+// are read before the barrier, outputs written after it (in place).
+template <typename R, int EPT, int RAD>
+__device__ __forceinline__ void fft_pass_g(typename V2<R>::T *buf, int n, int nrows, int stride, int Ns,
+                                           const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
+    using C = typename V2<R>::T;
+    constexpr int NB = (EPT + RAD - 1) / RAD;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nb = n / RAD, tot = nb * nrows;
+    const int twm = Z / (Ns * RAD);
+    const Dv dnb(nb), dns(Ns);
+    C v[NB][RAD];
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        const int b = tid + e * nt;
+        if (b < tot) {
+            const int row = dnb.q(b), j = b - row * nb;
+            const C *p = buf + row * stride;
+#pragma unroll
+            for (int t = 0; t < RAD; ++t) v[e][t] = p[lp(j + t * nb)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < NB; ++e) {
+        const int b = tid + e * nt;
+        if (b < tot) {
+            const int row = dnb.q(b), j = b - row * nb;
+            const int k = dns.r(j);
+#pragma unroll
+            for (int t = 1; t < RAD; ++t) {
+                C ww = tw[t * k * twm];
+                if (inv) ww.y = -ww.y;
+                v[e][t] = cmul(v[e][t], ww);
+            }
+            C y[RAD];
+            if (RAD == 2) {
+                y[0] = cadd(v[e][0], v[e][1]);
+                y[1] = csub(v[e][0], v[e][1]);
+            } else if (RAD == 3) {
+                dft3(v[e], y, inv);
+            } else if (RAD == 4) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) y[t] = v[e][t];
+                dft4(y, inv);
+            } else {
+                dft8(v[e], y, inv);
+            }
+            C *q = buf + row * stride;
+            const int base = (j - k) * RAD + k;
+#pragma unroll
+            for (int t = 0; t < RAD; ++t) q[lp(base + t * Ns)] = y[t];
+        }
+    }
+    __syncthreads();
+}
+
+// Transform of rows of length n = 2^p 3^c (c <= 2): radix-8 passes over the
+// factor 2^p, one radix-4 or radix-2 pass for its remaining bits, then one
+// radix-3 pass per factor 3.  A power-of-two n takes fft_pass (below).
+template <typename R, int EPT>
+__device__ __forceinline__ void lds_fft_g(typename V2<R>::T *buf, int n, int nrows, int stride,
+                                          const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
+    int m = n, c = 0;
+    while (m % 3 == 0) {
+        m /= 3;
+        ++c;
+    }
+    int Ns = 1;
+    while (m >= 8) {
+        fft_pass_g<R, EPT, 8>(buf, n, nrows, stride, Ns, tw, Z, inv);
+        Ns *= 8;
+        m /= 8;
+    }
+    if (m == 4) {
+        fft_pass_g<R, EPT, 4>(buf, n, nrows, stride, Ns, tw, Z, inv);
+        Ns *= 4;
+    } else if (m == 2) {
+        fft_pass_g<R, EPT, 2>(buf, n, nrows, stride, Ns, tw, Z, inv);
+        Ns *= 2;
+    }
+    for (; c > 0; --c) {
+        fft_pass_g<R, EPT, 3>(buf, n, nrows, stride, Ns, tw, Z, inv);
+        Ns *= 3;
+    }
+}
+
+// rows of length n: the power-of-two passes below, or lds_fft_g when n has a
+// factor 3 (tw holds exp(-2 pi i k / Z), k < Z; n divides Z)
+template <typename R, int EPT>
+__device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
+                                             const typename V2<R>::T *__restrict__ tw, int logZ, bool inv);
+template <typename R, int EPT>
+__device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int n, int nrows, int stride,
+                                        const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
+    if ((n & (n - 1)) == 0 && (Z & (Z - 1)) == 0)
+        lds_fft_pow2<R, EPT>(buf, __ffs(n) - 1, nrows, stride, tw, __ffs(Z) - 1, inv);
+    else
+        lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
+}
+
 // radix-8 passes, then one radix-4 or radix-2 pass for the remaining bits
 template <typename R, int EPT>
-__device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int logn, int nrows, int stride,
-                        const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
+__device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
+                                             const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     int logNs = 0;
     const bool small8 = logn - 3 < 3;  // nb = n/8 below 8
     while (logn - logNs >= 3) {
@@ -1109,20 +1248,21 @@ constexpr int FFT_THREADS = 256;
 // ht[m*Z + k].  Length-Z/2 complex FFT of (x[2n], x[2n+1]) + split.
 template <typename R, int EPT>
 __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
-                                              const typename V2<R>::T *__restrict__ tw, int Z, int logZ, int RB,
+                                              const typename V2<R>::T *__restrict__ tw, int Z, int RB,
                                               int stride, int blk) {
     using C = typename V2<R>::T;
     extern __shared__ unsigned char smem_raw[];
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int n = Z >> 1, logn = logZ - 1;
+    const int n = Z >> 1;
+    const Dv dn(n), drb(RB);
     const int k0 = blk * RB;
     for (int idx = tid; idx < RB * n; idx += nt) {
-        const int row = idx >> logn, m = idx & (n - 1);
+        const int row = dn.q(idx), m = idx - row * n;
         s[row * stride + lp(m)] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, logn, RB, stride, tw, logZ, false);
+    lds_fft<R, EPT>(s, n, RB, stride, tw, Z, false);
     const int half = n >> 1;
     for (int idx = tid; idx < RB * (half + 1); idx += nt) {
         const int row = idx / (half + 1), m = idx - row * (half + 1);
@@ -1141,16 +1281,16 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
     }
     __syncthreads();
     for (int idx = tid; idx < RB * n; idx += nt) {
-        const int row = idx % RB, m = idx / RB;
+        const int m = drb.q(idx), row = idx - m * RB;
         ht[(size_t)m * Z + k0 + row] = s[row * stride + lp(m)];
     }
 }
 
 template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
-                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
+                                                          const typename V2<R>::T *__restrict__ tw, int Z,
                                                           int RB, int stride, int remap) {
-    r2c_rows_body<R, EPT>(rho, ht, tw, Z, logZ, RB, stride, xcd_block(remap));
+    r2c_rows_body<R, EPT>(rho, ht, tw, Z, RB, stride, xcd_block(remap));
 }
 
 // spectral factors of one (j, m') pair: gx = sq[j] G / Z / Z, gy = sq[m'] G / Z / Z,
@@ -1182,21 +1322,22 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
     C *s = (C *)smem_raw;
     C *sB = s + RC * stride;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int Z = g.Z, logZ = g.logZ, Zh = Z >> 1;
+    const int Z = g.Z, Zh = Z >> 1;
+    const Dv dz(Z);
     const int m0 = blk * RC;
     for (int idx = tid; idx < RC * Z; idx += nt) {
-        const int row = idx >> logZ, k = idx & (Z - 1);
+        const int row = dz.q(idx), k = idx - row * Z;
         s[row * stride + lp(k)] = ht[(size_t)(m0 + row) * Z + k];
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, logZ, RC, stride, tw, logZ, false);
+    lds_fft<R, EPT>(s, Z, RC, stride, tw, Z, false);
     for (int idx = tid; idx < RC * Z; idx += nt) {
-        const int row = idx >> logZ, j = idx & (Z - 1);
+        const int row = dz.q(idx), j = idx - row * Z;
         const int m = m0 + row;
         if (m == 0) {
             // packed (DC, Nyquist) column: pairs (j, Z-j) by one thread
             if (j > Zh) continue;
-            const int jn = (Z - j) & (Z - 1);
+            const int jn = j == 0 ? 0 : Z - j;
             C P = s[lp(j)], Q = s[lp(jn)];
             C Sdc = cscale(cadd(P, cconj(Q)), (R)0.5);
             C Sny = cmul_mi(cscale(csub(P, cconj(Q)), (R)0.5));
@@ -1226,9 +1367,9 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
         }
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, logZ, 2 * RC, stride, tw, logZ, true);
+    lds_fft<R, EPT>(s, Z, 2 * RC, stride, tw, Z, true);
     for (int idx = tid; idx < RC * Z; idx += nt) {
-        const int row = idx >> logZ, k = idx & (Z - 1);
+        const int row = dz.q(idx), k = idx - row * Z;
         const int so = row * stride + lp(k);
         C a = s[so], b = sB[so];
         C4 o;
@@ -1254,7 +1395,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__
 template <typename R, int EPT>
 __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restrict__ gt,
                                               typename V2<R>::T *__restrict__ fm,
-                                              const typename V2<R>::T *__restrict__ tw, int Z, int logZ, int RB2,
+                                              const typename V2<R>::T *__restrict__ tw, int Z, int RB2,
                                               int stride, int blk) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
@@ -1262,9 +1403,10 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int Zh = Z >> 1;
+    const Dv dz(Z), drb(RB2);
     const int k0 = blk * RB2;
     for (int idx = tid; idx < RB2 * Zh; idx += nt) {
-        const int row = idx % RB2, m = idx / RB2;
+        const int m = drb.q(idx), row = idx - m * RB2;
         C4 v = gt[(size_t)m * Z + k0 + row];
         C *p = s + row * stride;
         if (m == 0) {
@@ -1276,9 +1418,9 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
         }
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, logZ, RB2, stride, tw, logZ, true);
+    lds_fft<R, EPT>(s, Z, RB2, stride, tw, Z, true);
     for (int idx = tid; idx < RB2 * Z; idx += nt) {
-        const int row = idx >> logZ, l = idx & (Z - 1);
+        const int row = dz.q(idx), l = idx - row * Z;
         fm[(size_t)(k0 + row) * Z + l] = s[row * stride + lp(l)];
     }
 }
@@ -1286,11 +1428,11 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
 template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
                                                           typename V2<R>::T *__restrict__ fm,
-                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
+                                                          const typename V2<R>::T *__restrict__ tw, int Z,
                                                           int RB2, int stride, int remap, Geo g, DevBufs d,
                                                           int end_step) {
     if (end_step && blockIdx.x == 0) step_end_body(g, d);
-    c2r_rows_body<R, EPT>(gt, fm, tw, Z, logZ, RB2, stride, xcd_block(remap));
+    c2r_rows_body<R, EPT>(gt, fm, tw, Z, RB2, stride, xcd_block(remap));
 }
 
 // ------------------------------------------------------------------ gather, update
@@ -1303,8 +1445,8 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
     R wx[3], wy[3];
     weights_r<R, SCHEME>(x, g, k, wx);
     weights_r<R, SCHEME>(y, g, l, wy);
-    k &= Z - 1;  // Z is a power of two
-    l &= Z - 1;
+    k = wrapz(k, Z);
+    l = wrapz(l, Z);
     if (SCHEME == 0) {
         R2 f = fm[(size_t)k * Z + l];
         if (!s2) {
@@ -1318,8 +1460,8 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
         }
         return;
     }
-    const int kx[3] = {(k - 1 + Z) & (Z - 1), k, (k + 1) & (Z - 1)};
-    const int ly[3] = {(l - 1 + Z) & (Z - 1), l, (l + 1) & (Z - 1)};
+    const int kx[3] = {wrapz(k - 1, Z), k, wrapz(k + 1, Z)};
+    const int ly[3] = {wrapz(l - 1, Z), l, wrapz(l + 1, Z)};
     R ax = 0, ay = 0;
     // weighted sums of the mesh forces with explicit fused multiply-adds
     // (the weights and indices above follow the reference's rounding)
@@ -1948,7 +2090,7 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int
     for_entries<P, true>(nmap, [&](int e_) {
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
-        r2c_rows_body<R, EPT>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb, r.sR1, t.y);
+        r2c_rows_body<R, EPT>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.rb, r.sR1, t.y);
     });
 }
 
@@ -1973,7 +2115,7 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
         if (end_step && t.y == 0) step_end_body(r.g, r.d);
-        c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.g.logZ, r.rb2, r.sR2,
+        c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.rb2, r.sR2,
                               t.y);
     });
 }
@@ -2098,6 +2240,22 @@ static int ept_for(int want, int need) {
     return e;
 }
 
+// the largest divisor of m not above cap (at least 1)
+static int div_at_most(int m, int cap) {
+    cap = std::max(1, std::min(m, cap));
+    while (m % cap) --cap;
+    return cap;
+}
+
+// FFT rows per block for elements-per-thread (e1, e2, e3) at mesh size Z:
+// as many rows as the block's elements hold, a divisor of the row count
+// (for a power-of-two Z the element count itself)
+static void fft_rows(int Z, const int e[3], int &rb, int &rc, int &rb2) {
+    rb = div_at_most(Z, e[0] * FFT_THREADS / (Z / 2));
+    rc = div_at_most(Z / 2, e[1] * FFT_THREADS / (2 * Z));
+    rb2 = div_at_most(Z, e[2] * FFT_THREADS / Z);
+}
+
 template <typename R> static FftCfg fft_cfg(int Z) {
     using C = typename V2<R>::T;
     const int *w = fft_ept_default();
@@ -2105,9 +2263,8 @@ template <typename R> static FftCfg fft_cfg(int Z) {
     c.e1 = ept_for(w[0], Z / 2);
     c.e2 = ept_for(w[1], 2 * Z);
     c.e3 = ept_for(w[2], Z);
-    c.RB = std::min(Z, c.e1 * FFT_THREADS / (Z / 2));
-    c.RC = std::max(1, std::min(Z / 2, c.e2 * FFT_THREADS / (2 * Z)));
-    c.RB2 = std::min(Z, c.e3 * FFT_THREADS / Z);
+    const int e[3] = {c.e1, c.e2, c.e3};
+    fft_rows(Z, e, c.RB, c.RC, c.RB2);
     c.sR1 = lds_row(Z / 2);
     c.sC = lds_row(Z);
     c.sR2 = lds_row(Z);
@@ -2272,19 +2429,19 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStrea
     if (!st) st = h->stream;
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
-    const int Z = h->g.Z, logZ = h->g.logZ;
+    const int Z = h->g.Z;
     const FftCfg c = fft_cfg<R>(Z);
     const R *sq = std::is_same<R, float>::value ? (const R *)h->d.sqf : (const R *)h->d.sq;
     const R *lap = std::is_same<R, float>::value ? (const R *)h->d.lapf : (const R *)h->d.lap;
 #define R2C(E)                                                                                              \
     hipLaunchKernelGGL((k_r2c_rows<R, E>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,             \
-                       (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, logZ, c.RB, c.sR1, h->g.xcd)
+                       (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, c.RB, c.sR1, h->g.xcd)
 #define COL(E)                                                                                              \
     hipLaunchKernelGGL((k_col<R, E>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,            \
                        (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC)
 #define C2R(E)                                                                                              \
     hipLaunchKernelGGL((k_c2r_rows<R, E>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,            \
-                       (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, logZ, c.RB2, c.sR2, h->g.xcd, h->g, h->d,  \
+                       (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, c.RB2, c.sR2, h->g.xcd, h->g, h->d,  \
                        end_step)
     if (only < 0 || only == 0) BDK_EPT_SWITCH(c.e1, R2C)
     if (only < 0 || only == 1) BDK_EPT_SWITCH(c.e2, COL)
@@ -2396,7 +2553,7 @@ hipError_t launch_noise_out(bdk_t *h) {
 // partial row; k_rhok_final adds the partial rows in order.
 template <typename R, int EPT>
 __global__ __launch_bounds__(FFT_THREADS) void k_col_rhok(const typename V2<R>::T *__restrict__ ht,
-                                                          const typename V2<R>::T *__restrict__ tw, int Z, int logZ,
+                                                          const typename V2<R>::T *__restrict__ tw, int Z,
                                                           double dqr, int nbins, double *part) {
     using C = typename V2<R>::T;
     extern __shared__ unsigned char smem_raw[];
@@ -2409,16 +2566,16 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col_rhok(const typename V2<R>::
         if (m == 0) pn[b] = 0.0;
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, logZ, 1, lds_row(Z), tw, logZ, false);
+    lds_fft<R, EPT>(s, Z, 1, lds_row(Z), tw, Z, false);
     if (tid != 0) return;
     for (int j = 0; j < Z; ++j) {
-        const double xq = (double)(((j + Zh) & (Z - 1)) - Zh);
+        const double xq = (double)(wrapz(j + Zh, Z) - Zh);
         const double sg = ((j + m) & 1) ? -1.0 : 1.0;
         if (m > 0) {
             const int bin = (int)(dqr * sqrt(xq * xq + (double)m * m) + 0.5);
             if (bin < nbins) p0[bin] += 2.0 * sg * (double)s[lp(j)].x;
         } else {
-            const C P = s[lp(j)], Q = s[lp((Z - j) & (Z - 1))];
+            const C P = s[lp(j)], Q = s[lp(j == 0 ? 0 : Z - j)];
             const double dc = 0.5 * ((double)P.x + (double)Q.x);  // Re of the DC column
             const double ny = 0.5 * ((double)P.y + (double)Q.y);  // Re of the Nyquist column
             int bin = (int)(dqr * sqrt(xq * xq) + 0.5);
@@ -2439,14 +2596,14 @@ __global__ void k_rhok_final(const double *part, int ncol, int nbins, double *ou
 
 template <typename R> static void launch_rhok_t(bdk_t *h, double dqr, int nbins, double *part, double *out) {
     using C = typename V2<R>::T;
-    const int Z = h->g.Z, logZ = h->g.logZ;
+    const int Z = h->g.Z;
     const int ept = ept_for(8, Z);
     const size_t lds = (size_t)lds_row(Z) * sizeof(C);
 #define RHOK(E)                                                                                              \
     (void)hipFuncSetAttribute((const void *)k_col_rhok<R, E>, hipFuncAttributeMaxDynamicSharedMemorySize,      \
                               160 * 1024);                                                                     \
     hipLaunchKernelGGL((k_col_rhok<R, E>), dim3(Z / 2), dim3(FFT_THREADS), lds, h->stream, (const C *)h->d.ht, \
-                       (const C *)h->d.tw, Z, logZ, dqr, nbins, part)
+                       (const C *)h->d.tw, Z, dqr, nbins, part)
     BDK_EPT_SWITCH(ept, RHOK)
 #undef RHOK
     hipLaunchKernelGGL(k_rhok_final, dim3(nblk(nbins, 256)), dim3(256), 0, h->stream, part, Z / 2 + 1, nbins, out);
@@ -3014,12 +3171,6 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
 }
 
 // ------------------------------------------------------------------ batch plans
-// FFT rows per block for elements-per-thread (e1, e2, e3) at mesh size Z.
-static void fft_rows(int Z, const int e[3], int &rb, int &rc, int &rb2) {
-    rb = std::min(Z, e[0] * FFT_THREADS / (Z / 2));
-    rc = std::max(1, std::min(Z / 2, e[1] * FFT_THREADS / (2 * Z)));
-    rb2 = std::min(Z, e[2] * FFT_THREADS / Z);
-}
 
 int batch_min_ept(int stage, int zmax) {
     const int need = stage == 0 ? zmax / 2 : (stage == 1 ? 2 * zmax : zmax);

@@ -134,11 +134,42 @@ void bdo_philox_noise(uint64_t seed, uint32_t pid, uint64_t step, int gauss,
 }
 
 /* ------------------------------------------------------------------ */
+/* in-place complex DFT of any length n, unnormalised, sign -1 / +1 (FFTW
+   sign convention used at dm:600-602): the definition summed directly,
+   y[k] = sum_j x[j] w^(jk mod n), w = cs[1] + sign i sn[1].  For the meshes
+   with a factor 3 (the reference's default FFT_samples = 384,
+   parameter.h:24); O(n^2) per transform. */
+static void dft1d(double *re, double *im, int n, int stride, int sign,
+                  const double *cs, const double *sn) {
+    double *yr = (double *)malloc(sizeof(double) * (size_t)n);
+    double *yi = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const int e = (int)(((long long)j * k) % n);
+            const double wr = cs[e], wi = sign * sn[e];
+            const double xr = re[j * stride], xi = im[j * stride];
+            ar += xr * wr - xi * wi;
+            ai += xr * wi + xi * wr;
+        }
+        yr[k] = ar;
+        yi[k] = ai;
+    }
+    for (int k = 0; k < n; ++k) { re[k * stride] = yr[k]; im[k * stride] = yi[k]; }
+    free(yr);
+    free(yi);
+}
+
 /* radix-2 in-place complex FFT, unnormalised, sign -1 / +1 (FFTW sign
    convention used at dm:600-602).  The reference calls FFTW3 (external,
-   unpinned); this is the DFT definition evaluated by Cooley-Tukey. */
+   unpinned); this is the DFT definition evaluated by Cooley-Tukey.  A
+   length that is not a power of two goes to dft1d. */
 static void fft1d(double *re, double *im, int n, int stride, int sign,
                   const double *cs, const double *sn) {
+    if (n & (n - 1)) {
+        dft1d(re, im, n, stride, sign, cs, sn);
+        return;
+    }
     int j = 0;
     for (int i = 0; i < n - 1; ++i) {
         if (i < j) {

@@ -59,7 +59,8 @@ def test_derived_constants_match_oracle():
 
 
 @pytest.mark.parametrize("bad,msg", [
-    (dict(Z=100), "power of two"), (dict(Z=8192), "power of two"), (dict(scheme=3), "Schema"),
+    (dict(Z=100), "FFT_samples"), (dict(Z=8192), "FFT_samples"), (dict(Z=54), "FFT_samples"),
+    (dict(Z=6), "FFT_samples"), (dict(scheme=3), "Schema"),
     (dict(L=3.0), "3 list cells"), (dict(n1=0, n2=0), "n1+n2"), (dict(lam=0.0), "lambda"),
     (dict(dt_max=0.0), "dt_max"), (dict(precision=5), "precision"), (dict(noise_mode=9), "noise_mode"),
     (dict(max_travel=0.0), "max_travel"),
@@ -71,6 +72,14 @@ def test_create_rejects_invalid_parameters(bad, msg):
     assert e.value.code == bdk.ERR_ARG and msg in str(e.value)
     with pytest.raises(bdk.BdkError):
         bdk.footprint(bp)
+
+
+@pytest.mark.parametrize("Z", [8, 12, 18, 24, 36, 96, 384, 576, 1152, 3072, 4096])
+def test_mesh_sizes_with_a_factor_three_are_accepted(Z):
+    """Z = 2^a 3^b (a >= 1, b <= 2): the reference's default FFT_samples =
+    384 (parameter.h:24) included; footprint() validates without a GPU."""
+    bp, _ = params(Z=Z, L=max(40.0, 0.6 * Z))
+    assert bdk.footprint(bp) > 0
 
 
 def test_footprint_scales():

@@ -1,0 +1,97 @@
+"""GPU parity on meshes whose size has a factor 3 (Z = 2^a 3^b, b <= 2),
+among them the reference's default FFT_samples = 384 (parameter.h:24),
+against the FP64 oracle (whose DFT of these lengths is pinned against numpy
+in test_oracle.py::test_fft2d_lengths_with_a_factor_three).
+
+Tolerances as in test_gpu_parity.py: FP64 forces and meshes within 1e-12
+relative RMS, FP32 within 1e-5; FP64 trajectories keep identical cells and
+positions within 1e-10.
+"""
+import numpy as np
+import pytest
+
+import bdk
+from common import pair
+from helpers import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def check_forces(g, o, tol):
+    fg = g.berechne_kapkraefte()
+    fo = o.kap()
+    assert rel_rms(fg, fo) < tol, rel_rms(fg, fo)
+    assert rel_rms(g.mesh(bdk.MESH_RHO), o.mesh(0)) < tol
+    assert rel_rms(g.mesh(bdk.MESH_FX), o.mesh(1)) < tol
+    assert rel_rms(g.mesh(bdk.MESH_FY), o.mesh(2)) < tol
+
+
+# 12 = 4*3 and 18 = 2*9 have no radix-8 pass; 24 = 8*3; 96 = 32*3 (radix 8,
+# 4, 3); 144 = 16*9 (two radix-3 passes); 384 = 128*3 (8, 8, 2, 3)
+@pytest.mark.parametrize("scheme", [bdk.NGP, bdk.CIC, bdk.TSC])
+@pytest.mark.parametrize("prec,tol", [(bdk.FP64, TOL64), (bdk.FP32, TOL32)])
+@pytest.mark.parametrize("Z,L", [(12, 10.0), (18, 12.0), (24, 16.0), (96, 60.0), (144, 90.0), (384, 200.0)])
+def test_capillary_parity_factor3(scheme, prec, tol, Z, L):
+    say("case: kap Z=%d scheme=%d prec=%d" % (Z, scheme, prec))
+    n = int(0.2 * L * L)
+    g, o, git, rel = pair(n1=n, n2=n, L=L, Z=Z, scheme=scheme, precision=prec, seed=31, lam=6.0, kap=1.3,
+                          f2_f1=-0.7)
+    check_forces(g, o, tol)
+
+
+@pytest.mark.parametrize("prec", [bdk.FP64, bdk.FP32])
+def test_reference_default_setup(prec):
+    """The reference's own parameter.h setup (N1 = N2 = 2000, L = 300, TSC on
+    384^2, lambda = 25, kap = 1.111111111, T = 24.3055555, dt_max = 0.005,
+    max_reisedistanz = 0.24, parameter.h:9-44): forces against the oracle,
+    then 8 adaptive-dt steps (dt = optimaler_zeitschritt, dm:1420-1491) in
+    FP64 with cells and positions against the oracle's."""
+    say("case: reference default setup prec=%d" % prec)
+    g, o, git, rel = pair(n1=2000, n2=2000, L=300.0, Z=384, scheme=bdk.TSC, lam=25.0, kap=1.111111111,
+                          T=24.3055555, dt_max=0.005, max_travel=0.24, precision=prec, seed=37)
+    check_forces(g, o, TOL64 if prec == bdk.FP64 else TOL32)
+    if prec != bdk.FP64:
+        return
+    for _ in range(8):
+        dg = g.zeitschritt(1.0)
+        do = o.step(1.0)
+        assert dg == pytest.approx(do, rel=1e-10)
+    gg, gr = g.positions()
+    og, orl = o.all_positions()
+    assert np.array_equal(gg, og)
+    assert np.abs(gr - orl).max() < 1e-10
+
+
+def test_factor3_graph_replay_and_batch():
+    """A 384^2 simulation: graph-replayed steps equal single steps bit for bit,
+    and the same simulation as a batch member (beside a power-of-two member)
+    equals its run alone."""
+    say("case: 384^2 graph replay and batch")
+    kw = dict(n1=3000, n2=3000, L=150.0, Z=384, scheme=bdk.TSC, precision=bdk.FP32, lam=20.0, seed=41)
+    a, _, git, rel = pair(**kw)
+    b = bdk.RunDynamik(a.p)
+    b.set_positions(git, rel)
+    a.run_steps(20)
+    for _ in range(20):
+        b.zeitschritt(1.0)
+    ga, ra = a.positions()
+    gb, rb = b.positions()
+    assert np.array_equal(ga, gb) and np.array_equal(ra, rb)
+    c = bdk.RunDynamik(a.p)
+    c.set_positions(git, rel)
+    d, _, _, _ = pair(n1=800, n2=800, L=50.0, Z=64, scheme=bdk.CIC, precision=bdk.FP32, seed=43)
+    batch = bdk.Batch([c, d])
+    batch.run_steps(20)
+    batch.sync()
+    gc, rc = c.positions()
+    assert np.array_equal(gc, ga) and np.array_equal(rc, ra)
+    batch.close()
+    for s in (a, b, c, d):
+        s.close()
