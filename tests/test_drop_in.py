@@ -34,6 +34,23 @@ def test_reference_call_driver_compiles(tmp_path):
     assert out.returncode == 0, out.stderr
 
 
+REF_PARAM = "/root/reference/parameter.h"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PARAM), reason="the reference tree is not on this machine")
+def test_reference_call_driver_compiles_against_reference_parameter_h(tmp_path):
+    """The same driver against the reference's own parameter.h as it lies in
+    the reference tree (N1 = N2 = 2000, L = 300, TSC, FFT_samples = 384,
+    adaptive dt): the header a reference user already has configures the
+    build, and its mesh size is one the library accepts."""
+    exe = str(tmp_path / "refcalls_ref")
+    cmd = B.driver_cmd(SRC, exe, param_header=REF_PARAM)
+    if not os.path.exists(B.LIB):
+        cmd = [c for c in cmd if not c.startswith(("-L", "-l", "-Wl", "-o")) and c != exe] + ["-fsyntax-only"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+
+
 @pytest.mark.gpu
 def test_reference_call_driver_equals_bdk_sim(tmp_path):
     print("case: reference-call driver vs bdk_sim", flush=True)
