@@ -153,11 +153,13 @@ __device__ __forceinline__ int wrapc(int u, int n) {
 }
 // u in [-n, 2n)
 __device__ __forceinline__ int wrap1(int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); }
-// mesh index u in [-Z, 2Z) into [0, Z): a mask when Z is a power of two
-__device__ __forceinline__ int wrapz(int u, int Z) {
-    if ((Z & (Z - 1)) == 0) return u & (Z - 1);
-    return wrap1(u, Z);
-}
+// Mesh code is compiled for two kinds of mesh size: P2 (Z a power of two:
+// masks and shifts) and the sizes with a factor 3.  Kernels pick the kind
+// once, at their top, so the power-of-two instruction stream carries no
+// test of the other.
+__device__ __forceinline__ bool zpow2(int Z) { return (Z & (Z - 1)) == 0; }
+// mesh index u in [-Z, 2Z) into [0, Z)
+template <bool P2> __device__ __forceinline__ int wrapz(int u, int Z) { return P2 ? (u & (Z - 1)) : wrap1(u, Z); }
 // x / d and x % d for 0 <= x < 2^22: a shift when d is a power of two, else
 // the float reciprocal corrected to the exact quotient
 struct Dv {
@@ -173,6 +175,16 @@ struct Dv {
         return v;
     }
     __device__ int r(int x) const { return sh >= 0 ? (x & (d - 1)) : x - q(x) * d; }
+};
+// the same with the kind of d known at compile time (P2: a power of two)
+template <bool P2> struct DvT : Dv {
+    __device__ explicit DvT(int d_) : Dv(d_) {}
+};
+template <> struct DvT<true> {
+    int d, sh;
+    __device__ explicit DvT(int d_) : d(d_), sh(__ffs(d_) - 1) {}
+    __device__ int q(int x) const { return x >> sh; }
+    __device__ int r(int x) const { return x & (d - 1); }
 };
 // c / nc for 0 <= c < 2^24 (float reciprocal, corrected to the exact quotient)
 __device__ __forceinline__ int div_nc(int c, const Geo &g) {
@@ -672,7 +684,7 @@ __global__ void k_own(Geo g, int *own) {
 // charges are rounded to 32-bit fixed point (scale 2^S, |charge| 2^S < 2^31,
 // far below FP32 resolution) and summed in 64 bits, so the sums stay exact
 // and independent of the order of the atomics.
-template <int SCHEME>
+template <int SCHEME, bool P2>
 __device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs &d, const DepArgs &a, int gx,
                                                      int gy, double2 rr, bool s2, int kw0, int lw0, int nwx,
                                                      int nwy, unsigned long long *acc) {
@@ -681,7 +693,7 @@ __device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs
     float wx[3], wy[3];
     weights_r<float, SCHEME>(x, g, kp, wx);
     weights_r<float, SCHEME>(y, g, lp, wy);
-    const int kb = wrapz(kp - 1 - kw0, g.Z), lb = wrapz(lp - 1 - lw0, g.Z);
+    const int kb = wrapz<P2>(kp - 1 - kw0, g.Z), lb = wrapz<P2>(lp - 1 - lw0, g.Z);
     if (kb > nwx - 3 || lb > nwy - 3) {  // outside the window: not reachable, flagged
         atomicOr(&d.sc->nonfinite, 4);
         return;
@@ -703,7 +715,7 @@ __device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs
         }
 }
 
-template <typename R, int SCHEME>
+template <typename R, int SCHEME, bool P2>
 __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int gx, int gy,
                                                  double2 rr, bool s2, int klo, int khi, int llo, int lhi,
                                                  unsigned long long *acc_hi, unsigned long long *acc_lo) {
@@ -717,13 +729,13 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
     for (int ia = 0; ia < 3; ++ia) {
         if (SCHEME == 0 && ia != 1) continue;
         if (wx[ia] == 0.0) continue;
-        const int k = wrapz(kp + ia - 1, Z);
+        const int k = wrapz<P2>(kp + ia - 1, Z);
         if (k < klo || k >= khi) continue;
 #pragma unroll
         for (int ib = 0; ib < 3; ++ib) {
             if (SCHEME == 0 && ib != 1) continue;
             if (wy[ib] == 0.0) continue;
-            const int l = wrapz(lp + ib - 1, Z);
+            const int l = wrapz<P2>(lp + ib - 1, Z);
             if (l < llo || l >= lhi) continue;
             const int w = (k - klo) * nl + (l - llo);
             if (std::is_same<R, float>::value) {
@@ -752,7 +764,7 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
 }
 
 // lb: tile index, tiles numbered x fastest over tx tiles per row
-template <typename R, int SCHEME>
+template <typename R, int SCHEME, bool P2>
 __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, const DepArgs &a, int lb, int tx) {
     extern __shared__ unsigned char smem_raw[];
     unsigned long long *acc_hi = (unsigned long long *)smem_raw;
@@ -865,9 +877,9 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
                 s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
             }
             if (win)
-                deposit_particle_win<SCHEME>(g, d, a, gx, gy, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
+                deposit_particle_win<SCHEME, P2>(g, d, a, gx, gy, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
             else
-                deposit_particle<R, SCHEME>(g, d, a, gx, gy, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
+                deposit_particle<R, SCHEME, P2>(g, d, a, gx, gy, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
         }
     } else {
         for (int idx = tid; idx < SX * SY; idx += nt) {
@@ -889,9 +901,9 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
                     s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
                 }
                 if (win)
-                    deposit_particle_win<SCHEME>(g, d, a, px, py, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
+                    deposit_particle_win<SCHEME, P2>(g, d, a, px, py, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
                 else
-                    deposit_particle<R, SCHEME>(g, d, a, px, py, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
+                    deposit_particle<R, SCHEME, P2>(g, d, a, px, py, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
             }
         }
     }
@@ -914,7 +926,7 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
         const int k = klo + qk, l = llo + rl;
         double v;
         if (win) {
-            v = (double)(long long)acc_hi[wrapz(k - kw0, g.Z) * nwy + wrapz(l - lw0, g.Z)] * a.iscale_f;
+            v = (double)(long long)acc_hi[wrapz<P2>(k - kw0, g.Z) * nwy + wrapz<P2>(l - lw0, g.Z)] * a.iscale_f;
         } else if (std::is_same<R, float>::value) {
             v = (double)(long long)acc_hi[w] * a.iscale_f;
         } else {
@@ -925,9 +937,11 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
 }
 
 // hardware block order: measured faster than per-XCD ranges for this kernel
-template <typename R, int SCHEME>
+// P2: the mesh kind (chosen by the launcher; one body per kernel keeps the
+// scalar registers of the power-of-two path as they were)
+template <typename R, int SCHEME, bool P2>
 __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArgs a) {
-    deposit_body<R, SCHEME>(g, d, a, blockIdx.x + blockIdx.y * gridDim.x, gridDim.x);
+    deposit_body<R, SCHEME, P2>(g, d, a, blockIdx.x + blockIdx.y * gridDim.x, gridDim.x);
 }
 
 // End of a step: step counter + 1, and the per-step scalars of the next
@@ -1214,6 +1228,15 @@ __device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int n, int nrows
     else
         lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
 }
+// the same with the mesh kind known at compile time (P2: n and Z powers of two)
+template <typename R, int EPT, bool P2>
+__device__ __forceinline__ void lds_fft_t(typename V2<R>::T *buf, int n, int nrows, int stride,
+                                          const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
+    if (P2)
+        lds_fft_pow2<R, EPT>(buf, __ffs(n) - 1, nrows, stride, tw, __ffs(Z) - 1, inv);
+    else
+        lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
+}
 
 // radix-8 passes, then one radix-4 or radix-2 pass for the remaining bits
 template <typename R, int EPT>
@@ -1246,7 +1269,7 @@ constexpr int FFT_THREADS = 256;
 // Row r2c: rows k0..k0+RB-1 of the real mesh -> half spectrum X[m], m<Z/2,
 // with X[0] packing (DC, Nyquist) (both real), written transposed:
 // ht[m*Z + k].  Length-Z/2 complex FFT of (x[2n], x[2n+1]) + split.
-template <typename R, int EPT>
+template <typename R, int EPT, bool P2>
 __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
                                               const typename V2<R>::T *__restrict__ tw, int Z, int RB,
                                               int stride, int blk) {
@@ -1255,14 +1278,14 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int n = Z >> 1;
-    const Dv dn(n), drb(RB);
+    const DvT<P2> dn(n), drb(RB);  // RB divides Z: a power of two with Z
     const int k0 = blk * RB;
     for (int idx = tid; idx < RB * n; idx += nt) {
-        const int row = dn.q(idx), m = idx - row * n;
+        const int row = dn.q(idx), m = dn.r(idx);
         s[row * stride + lp(m)] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, n, RB, stride, tw, Z, false);
+    lds_fft_t<R, EPT, P2>(s, n, RB, stride, tw, Z, false);
     const int half = n >> 1;
     for (int idx = tid; idx < RB * (half + 1); idx += nt) {
         const int row = idx / (half + 1), m = idx - row * (half + 1);
@@ -1281,16 +1304,16 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
     }
     __syncthreads();
     for (int idx = tid; idx < RB * n; idx += nt) {
-        const int m = drb.q(idx), row = idx - m * RB;
+        const int m = drb.q(idx), row = drb.r(idx);
         ht[(size_t)m * Z + k0 + row] = s[row * stride + lp(m)];
     }
 }
 
-template <typename R, int EPT>
+template <typename R, int EPT, bool P2>
 __global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
                                                           const typename V2<R>::T *__restrict__ tw, int Z,
                                                           int RB, int stride, int remap) {
-    r2c_rows_body<R, EPT>(rho, ht, tw, Z, RB, stride, xcd_block(remap));
+    r2c_rows_body<R, EPT, P2>(rho, ht, tw, Z, RB, stride, xcd_block(remap));
 }
 
 // spectral factors of one (j, m') pair: gx = sq[j] G / Z / Z, gy = sq[m'] G / Z / Z,
@@ -1312,7 +1335,7 @@ __device__ __forceinline__ void symbols(int j, int mm, const Geo &g, const R *sq
 
 // Column pass: rows m0..m0+RC-1 of ht (length Z along x), forward FFT,
 // multiply by i*gx and i*gy, inverse FFT of both, out gt[m*Z+k] = (Fx, Fy).
-template <typename R, int EPT>
+template <typename R, int EPT, bool P2>
 __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ ht, typename V4<R>::T *__restrict__ gt,
                                          const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
                                          const R *__restrict__ lap, const Geo &g, int RC, int stride, int blk) {
@@ -1323,21 +1346,21 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
     C *sB = s + RC * stride;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int Z = g.Z, Zh = Z >> 1;
-    const Dv dz(Z);
+    const DvT<P2> dz(Z);
     const int m0 = blk * RC;
     for (int idx = tid; idx < RC * Z; idx += nt) {
-        const int row = dz.q(idx), k = idx - row * Z;
+        const int row = dz.q(idx), k = dz.r(idx);
         s[row * stride + lp(k)] = ht[(size_t)(m0 + row) * Z + k];
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, Z, RC, stride, tw, Z, false);
+    lds_fft_t<R, EPT, P2>(s, Z, RC, stride, tw, Z, false);
     for (int idx = tid; idx < RC * Z; idx += nt) {
-        const int row = dz.q(idx), j = idx - row * Z;
+        const int row = dz.q(idx), j = dz.r(idx);
         const int m = m0 + row;
         if (m == 0) {
             // packed (DC, Nyquist) column: pairs (j, Z-j) by one thread
             if (j > Zh) continue;
-            const int jn = j == 0 ? 0 : Z - j;
+            const int jn = P2 ? ((Z - j) & (Z - 1)) : (j == 0 ? 0 : Z - j);
             C P = s[lp(j)], Q = s[lp(jn)];
             C Sdc = cscale(cadd(P, cconj(Q)), (R)0.5);
             C Sny = cmul_mi(cscale(csub(P, cconj(Q)), (R)0.5));
@@ -1367,9 +1390,9 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
         }
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, Z, 2 * RC, stride, tw, Z, true);
+    lds_fft_t<R, EPT, P2>(s, Z, 2 * RC, stride, tw, Z, true);
     for (int idx = tid; idx < RC * Z; idx += nt) {
-        const int row = dz.q(idx), k = idx - row * Z;
+        const int row = dz.q(idx), k = dz.r(idx);
         const int so = row * stride + lp(k);
         C a = s[so], b = sB[so];
         C4 o;
@@ -1381,18 +1404,18 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
     }
 }
 
-template <typename R, int EPT>
+template <typename R, int EPT, bool P2>
 __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__restrict__ ht,
                                                      typename V4<R>::T *__restrict__ gt,
                                                      const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
                                                      const R *__restrict__ lap, Geo g, int RC, int stride) {
-    col_body<R, EPT>(ht, gt, tw, sq, lap, g, RC, stride, blockIdx.x);
+    col_body<R, EPT, P2>(ht, gt, tw, sq, lap, g, RC, stride, blockIdx.x);
 }
 
 // Row c2r of the two real signals Fx, Fy: rows k0..k0+RB2-1, spectrum
 // C[m] = X[m] + i Y[m] completed by Hermitian symmetry, inverse FFT of
 // length Z, out fmesh[k*Z + l] = (Re, Im) = (Fx, Fy).
-template <typename R, int EPT>
+template <typename R, int EPT, bool P2>
 __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restrict__ gt,
                                               typename V2<R>::T *__restrict__ fm,
                                               const typename V2<R>::T *__restrict__ tw, int Z, int RB2,
@@ -1403,10 +1426,10 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
     C *s = (C *)smem_raw;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int Zh = Z >> 1;
-    const Dv dz(Z), drb(RB2);
+    const DvT<P2> dz(Z), drb(RB2);  // RB2 divides Z: a power of two with Z
     const int k0 = blk * RB2;
     for (int idx = tid; idx < RB2 * Zh; idx += nt) {
-        const int m = drb.q(idx), row = idx - m * RB2;
+        const int m = drb.q(idx), row = drb.r(idx);
         C4 v = gt[(size_t)m * Z + k0 + row];
         C *p = s + row * stride;
         if (m == 0) {
@@ -1418,25 +1441,25 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
         }
     }
     __syncthreads();
-    lds_fft<R, EPT>(s, Z, RB2, stride, tw, Z, true);
+    lds_fft_t<R, EPT, P2>(s, Z, RB2, stride, tw, Z, true);
     for (int idx = tid; idx < RB2 * Z; idx += nt) {
-        const int row = dz.q(idx), l = idx - row * Z;
+        const int row = dz.q(idx), l = dz.r(idx);
         fm[(size_t)(k0 + row) * Z + l] = s[row * stride + lp(l)];
     }
 }
 
-template <typename R, int EPT>
+template <typename R, int EPT, bool P2>
 __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
                                                           typename V2<R>::T *__restrict__ fm,
                                                           const typename V2<R>::T *__restrict__ tw, int Z,
                                                           int RB2, int stride, int remap, Geo g, DevBufs d,
                                                           int end_step) {
     if (end_step && blockIdx.x == 0) step_end_body(g, d);
-    c2r_rows_body<R, EPT>(gt, fm, tw, Z, RB2, stride, xcd_block(remap));
+    c2r_rows_body<R, EPT, P2>(gt, fm, tw, Z, RB2, stride, xcd_block(remap));
 }
 
 // ------------------------------------------------------------------ gather, update
-template <typename R, int SCHEME>
+template <typename R, int SCHEME, bool P2>
 __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const typename V2<R>::T *__restrict__ fm,
                                            const Geo &g, double &fx, double &fy) {
     using R2 = typename V2<R>::T;
@@ -1445,8 +1468,8 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
     R wx[3], wy[3];
     weights_r<R, SCHEME>(x, g, k, wx);
     weights_r<R, SCHEME>(y, g, l, wy);
-    k = wrapz(k, Z);
-    l = wrapz(l, Z);
+    k = wrapz<P2>(k, Z);
+    l = wrapz<P2>(l, Z);
     if (SCHEME == 0) {
         R2 f = fm[(size_t)k * Z + l];
         if (!s2) {
@@ -1460,8 +1483,8 @@ __device__ __forceinline__ void gather_kap(double x, double y, bool s2, const ty
         }
         return;
     }
-    const int kx[3] = {wrapz(k - 1, Z), k, wrapz(k + 1, Z)};
-    const int ly[3] = {wrapz(l - 1, Z), l, wrapz(l + 1, Z)};
+    const int kx[3] = {wrapz<P2>(k - 1, Z), k, wrapz<P2>(k + 1, Z)};
+    const int ly[3] = {wrapz<P2>(l - 1, Z), l, wrapz<P2>(l + 1, Z)};
     R ax = 0, ay = 0;
     // weighted sums of the mesh forces with explicit fused multiply-adds
     // (the weights and indices above follow the reference's rounding)
@@ -1511,7 +1534,7 @@ __device__ __forceinline__ double2 particle_noise(const Geo &g, const DevBufs &d
 
 // forces only: fkap (and dt candidates when with_dt) of block blk of list
 // positions; the block minimum of the candidates goes to dts slot `slot`
-template <typename R, int SCHEME>
+template <typename R, int SCHEME, bool P2>
 __device__ __forceinline__ void gather_body(const Geo &g, const DevBufs &d, int with_dt, int blk, int slot) {
     using R2 = typename V2<R>::T;
     const int s = blk * blockDim.x + threadIdx.x;  // list position
@@ -1522,7 +1545,7 @@ __device__ __forceinline__ void gather_body(const Geo &g, const DevBufs &d, int 
         const double2 r = d.lrel[s];
         const int pid = d.lpid[s];
         double fx = 0.0, fy = 0.0;
-        if (!g.idgas) gather_kap<R, SCHEME>(gx * g.B + r.x, gy * g.B + r.y, pid >= g.n1, (const R2 *)d.fmesh, g, fx, fy);
+        if (!g.idgas) gather_kap<R, SCHEME, P2>(gx * g.B + r.x, gy * g.B + r.y, pid >= g.n1, (const R2 *)d.fmesh, g, fx, fy);
         R2 o;
         o.x = (R)fx;
         o.y = (R)fy;
@@ -1560,9 +1583,9 @@ __device__ __forceinline__ void gather_body(const Geo &g, const DevBufs &d, int 
     }
 }
 
-template <typename R, int SCHEME>
+template <typename R, int SCHEME, bool P2>
 __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
-    gather_body<R, SCHEME>(g, d, with_dt, xcd_block(g.xcd), blockIdx.x);
+    gather_body<R, SCHEME, P2>(g, d, with_dt, xcd_block(g.xcd), blockIdx.x);
 }
 
 // dt of the step: dt_max and the 64 slot minima of k_gather folded into
@@ -1602,7 +1625,7 @@ __device__ __forceinline__ double fmod_cell(double y, double B) {
 // position s (written to slot s).  Returns how far the stored new position
 // (cell and offset) lies outside the particle's previous cell (<= 0 inside).
 // FUSED: capillary force gathered here (fixed dt); else read from fkap.
-template <typename R, int SCHEME, bool FUSED>
+template <typename R, int SCHEME, bool FUSED, bool P2>
 __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int s) {
     using R2 = typename V2<R>::T;
     const unsigned long long step = d.sc->step;
@@ -1617,7 +1640,7 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
         fwx = (double)fw.x;
         fwy = (double)fw.y;
         if (FUSED) {
-            gather_kap<R, SCHEME>(gx * g.B + r.x, gy * g.B + r.y, s2, (const R2 *)d.fmesh, g, fkx, fky);
+            gather_kap<R, SCHEME, P2>(gx * g.B + r.x, gy * g.B + r.y, s2, (const R2 *)d.fmesh, g, fkx, fky);
         } else {
             const R2 fk = ((const R2 *)d.fkap)[s];
             fkx = (double)fk.x;
@@ -1730,7 +1753,7 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
 // particles when it stages them by their previous cells (during the list
 // rebuild): reduced per wave, then one atomicMax per wave into one of 64
 // slots (128 bytes apart), skipped when the slot already holds as much.
-template <typename R, int SCHEME, bool FUSED>
+template <typename R, int SCHEME, bool FUSED, bool P2>
 __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int blk) {
     const int s = blk * blockDim.x + threadIdx.x;  // list position in, slot out
     // the starts that group the slot arrays after this update (the previous
@@ -1738,7 +1761,7 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     const int stride = ((g.N + blockDim.x - 1) / blockDim.x) * blockDim.x;
     for (int c = s; c <= g.ncell; c += stride) d.start_old[c] = d.start[c];
     double m = 0.0;
-    if (s < g.N) m = update_one<R, SCHEME, FUSED>(g, d, s);
+    if (s < g.N) m = update_one<R, SCHEME, FUSED, P2>(g, d, s);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0 && m > 0.0) {
@@ -1748,9 +1771,10 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     }
 }
 
-template <typename R, int SCHEME, bool FUSED>
+// (without the fused gather the mesh kind plays no part: P2 = true)
+template <typename R, int SCHEME, bool FUSED, bool P2>
 __global__ __launch_bounds__(256) void k_update(Geo g, DevBufs d) {
-    update_body<R, SCHEME, FUSED>(g, d, xcd_block(g.xcd));
+    update_body<R, SCHEME, FUSED, P2>(g, d, xcd_block(g.xcd));
 }
 
 // refresh (dm:216-325) without motion + counting; used by bdk_refresh
@@ -2066,7 +2090,7 @@ __global__ __launch_bounds__(512) void kb_wca(const RepDev *m, const int2 *map, 
     });
 }
 
-template <typename R, bool P>
+template <typename R, bool P, bool P2>
 __global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const int2 *map, int nmap, int slot) {
     for_entries<P, true>(nmap, [&](int e_) {
         const int2 t = map[e_];
@@ -2077,24 +2101,24 @@ __global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const 
             a.reset = 0;
         }
         switch (r.g.scheme) {
-        case 0: deposit_body<R, 0>(r.g, r.d, a, t.y, r.dep_tx); break;
-        case 1: deposit_body<R, 1>(r.g, r.d, a, t.y, r.dep_tx); break;
-        default: deposit_body<R, 2>(r.g, r.d, a, t.y, r.dep_tx); break;
+        case 0: deposit_body<R, 0, P2>(r.g, r.d, a, t.y, r.dep_tx); break;
+        case 1: deposit_body<R, 1, P2>(r.g, r.d, a, t.y, r.dep_tx); break;
+        default: deposit_body<R, 2, P2>(r.g, r.d, a, t.y, r.dep_tx); break;
         }
     });
 }
 
-template <typename R, int EPT, bool P>
+template <typename R, int EPT, bool P, bool P2>
 __global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int2 *map, int nmap) {
     using C = typename V2<R>::T;
     for_entries<P, true>(nmap, [&](int e_) {
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
-        r2c_rows_body<R, EPT>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.rb, r.sR1, t.y);
+        r2c_rows_body<R, EPT, P2>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.rb, r.sR1, t.y);
     });
 }
 
-template <typename R, int EPT, bool P>
+template <typename R, int EPT, bool P, bool P2>
 __global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int2 *map, int nmap) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
@@ -2103,11 +2127,11 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int
         const RepDev &r = m[t.x];
         const R *sq = std::is_same<R, float>::value ? (const R *)r.d.sqf : (const R *)r.d.sq;
         const R *lap = std::is_same<R, float>::value ? (const R *)r.d.lapf : (const R *)r.d.lap;
-        col_body<R, EPT>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
+        col_body<R, EPT, P2>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
     });
 }
 
-template <typename R, int EPT, bool P>
+template <typename R, int EPT, bool P, bool P2>
 __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map, int nmap, int end_step) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
@@ -2115,30 +2139,30 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
         if (end_step && t.y == 0) step_end_body(r.g, r.d);
-        c2r_rows_body<R, EPT>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.rb2, r.sR2,
-                              t.y);
+        c2r_rows_body<R, EPT, P2>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.rb2, r.sR2,
+                                  t.y);
     });
 }
 
 // Members with a fixed dt gather the mesh force inside the update; members
 // with adaptive dt read the forces and the dt of kb_gather / kb_dt_fold, as
 // their single-simulation steps do.
-template <typename R, bool P>
+template <typename R, bool P, bool P2>
 __global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *map, int nmap) {
     for_entries<P, false>(nmap, [&](int e_) {
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
         if (r.g.adaptive) {
             switch (r.g.scheme) {
-            case 0: update_body<R, 0, false>(r.g, r.d, t.y); break;
-            case 1: update_body<R, 1, false>(r.g, r.d, t.y); break;
-            default: update_body<R, 2, false>(r.g, r.d, t.y); break;
+            case 0: update_body<R, 0, false, true>(r.g, r.d, t.y); break;
+            case 1: update_body<R, 1, false, true>(r.g, r.d, t.y); break;
+            default: update_body<R, 2, false, true>(r.g, r.d, t.y); break;
             }
         } else {
             switch (r.g.scheme) {
-            case 0: update_body<R, 0, true>(r.g, r.d, t.y); break;
-            case 1: update_body<R, 1, true>(r.g, r.d, t.y); break;
-            default: update_body<R, 2, true>(r.g, r.d, t.y); break;
+            case 0: update_body<R, 0, true, P2>(r.g, r.d, t.y); break;
+            case 1: update_body<R, 1, true, P2>(r.g, r.d, t.y); break;
+            default: update_body<R, 2, true, P2>(r.g, r.d, t.y); break;
             }
         }
     });
@@ -2146,15 +2170,15 @@ __global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *ma
 
 // capillary forces and dt candidates of the adaptive members (their
 // particle blocks only), then one wave per adaptive member folds its dt
-template <typename R, bool P>
+template <typename R, bool P, bool P2>
 __global__ __launch_bounds__(256) void kb_gather(const RepDev *m, const int2 *map, int nmap) {
     for_entries<P, true>(nmap, [&](int e_) {
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
         switch (r.g.scheme) {
-        case 0: gather_body<R, 0>(r.g, r.d, 1, t.y, e_); break;
-        case 1: gather_body<R, 1>(r.g, r.d, 1, t.y, e_); break;
-        default: gather_body<R, 2>(r.g, r.d, 1, t.y, e_); break;
+        case 0: gather_body<R, 0, P2>(r.g, r.d, 1, t.y, e_); break;
+        case 1: gather_body<R, 1, P2>(r.g, r.d, 1, t.y, e_); break;
+        default: gather_body<R, 2, P2>(r.g, r.d, 1, t.y, e_); break;
         }
     });
 }
@@ -2279,45 +2303,56 @@ hipError_t set_kernel_attributes() {
     hipError_t e = hipSuccess;
 #define BDK_ATTR(K)                                                                            \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-#define BDK_ATTR_E(E)                \
-    BDK_ATTR((k_r2c_rows<float, E>))  \
-    BDK_ATTR((k_r2c_rows<double, E>)) \
-    BDK_ATTR((k_col<float, E>))       \
-    BDK_ATTR((k_col<double, E>))      \
-    BDK_ATTR((k_c2r_rows<float, E>))  \
-    BDK_ATTR((k_c2r_rows<double, E>))
-    BDK_ATTR_E(8)
-    BDK_ATTR_E(16)
-    BDK_ATTR_E(32)
+#define BDK_ATTR_E(E, Q)                \
+    BDK_ATTR((k_r2c_rows<float, E, Q>))  \
+    BDK_ATTR((k_r2c_rows<double, E, Q>)) \
+    BDK_ATTR((k_col<float, E, Q>))       \
+    BDK_ATTR((k_col<double, E, Q>))      \
+    BDK_ATTR((k_c2r_rows<float, E, Q>))  \
+    BDK_ATTR((k_c2r_rows<double, E, Q>))
+    BDK_ATTR_E(8, true)
+    BDK_ATTR_E(16, true)
+    BDK_ATTR_E(32, true)
+    BDK_ATTR_E(8, false)
+    BDK_ATTR_E(16, false)
+    BDK_ATTR_E(32, false)
 #undef BDK_ATTR_E
-#define BDK_ATTR_B(E, P)            \
-    BDK_ATTR((kb_r2c<float, E, P>))  \
-    BDK_ATTR((kb_r2c<double, E, P>)) \
-    BDK_ATTR((kb_col<float, E, P>))  \
-    BDK_ATTR((kb_col<double, E, P>)) \
-    BDK_ATTR((kb_c2r<float, E, P>))  \
-    BDK_ATTR((kb_c2r<double, E, P>))
-    BDK_ATTR_B(8, false)
-    BDK_ATTR_B(16, false)
-    BDK_ATTR_B(32, false)
-    BDK_ATTR_B(8, true)
-    BDK_ATTR_B(16, true)
-    BDK_ATTR_B(32, true)
+#define BDK_ATTR_B(E, P, Q)            \
+    BDK_ATTR((kb_r2c<float, E, P, Q>))  \
+    BDK_ATTR((kb_r2c<double, E, P, Q>)) \
+    BDK_ATTR((kb_col<float, E, P, Q>))  \
+    BDK_ATTR((kb_col<double, E, P, Q>)) \
+    BDK_ATTR((kb_c2r<float, E, P, Q>))  \
+    BDK_ATTR((kb_c2r<double, E, P, Q>))
+#define BDK_ATTR_BQ(Q)           \
+    BDK_ATTR_B(8, false, Q)  \
+    BDK_ATTR_B(16, false, Q) \
+    BDK_ATTR_B(32, false, Q) \
+    BDK_ATTR_B(8, true, Q)   \
+    BDK_ATTR_B(16, true, Q)  \
+    BDK_ATTR_B(32, true, Q)
+    BDK_ATTR_BQ(true)
+    BDK_ATTR_BQ(false)
+#undef BDK_ATTR_BQ
 #undef BDK_ATTR_B
 #undef BDK_ATTR
     // the deposit also has a little static LDS: ask for 128 KB of dynamic LDS
 #define BDK_ATTR_D(K)                                                                          \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
-    BDK_ATTR_D((k_deposit<float, 0>))
-    BDK_ATTR_D((k_deposit<float, 1>))
-    BDK_ATTR_D((k_deposit<float, 2>))
-    BDK_ATTR_D((k_deposit<double, 0>))
-    BDK_ATTR_D((k_deposit<double, 1>))
-    BDK_ATTR_D((k_deposit<double, 2>))
-    BDK_ATTR_D((kb_deposit<float, false>))
-    BDK_ATTR_D((kb_deposit<double, false>))
-    BDK_ATTR_D((kb_deposit<float, true>))
-    BDK_ATTR_D((kb_deposit<double, true>))
+#define BDK_ATTR_DQ(Q)                      \
+    BDK_ATTR_D((k_deposit<float, 0, Q>))    \
+    BDK_ATTR_D((k_deposit<float, 1, Q>))    \
+    BDK_ATTR_D((k_deposit<float, 2, Q>))    \
+    BDK_ATTR_D((k_deposit<double, 0, Q>))   \
+    BDK_ATTR_D((k_deposit<double, 1, Q>))   \
+    BDK_ATTR_D((k_deposit<double, 2, Q>))   \
+    BDK_ATTR_D((kb_deposit<float, false, Q>))  \
+    BDK_ATTR_D((kb_deposit<double, false, Q>)) \
+    BDK_ATTR_D((kb_deposit<float, true, Q>))   \
+    BDK_ATTR_D((kb_deposit<double, true, Q>))
+    BDK_ATTR_DQ(true)
+    BDK_ATTR_DQ(false)
+#undef BDK_ATTR_DQ
 #undef BDK_ATTR_D
     return e;
 }
@@ -2396,8 +2431,13 @@ static DepPlan dep_plan(const bdk_t *h, int reset, int tile_pts = 38) {
 
 size_t deposit_lds_bytes(const bdk_t *h) { return dep_plan(h, 0).lds; }
 
+static bool zp2(int Z) { return (Z & (Z - 1)) == 0; }
+
 template <typename R, int SCHEME> static void launch_deposit_t(bdk_t *h, const DepPlan &pl, hipStream_t st) {
-    hipLaunchKernelGGL((k_deposit<R, SCHEME>), pl.grid, dim3(DP_THREADS), pl.lds, st, h->g, h->d, pl.a);
+    if (zp2(h->g.Z))
+        hipLaunchKernelGGL((k_deposit<R, SCHEME, true>), pl.grid, dim3(DP_THREADS), pl.lds, st, h->g, h->d, pl.a);
+    else
+        hipLaunchKernelGGL((k_deposit<R, SCHEME, false>), pl.grid, dim3(DP_THREADS), pl.lds, st, h->g, h->d, pl.a);
 }
 
 hipError_t launch_deposit(bdk_t *h, int reset, int slot, hipStream_t st) {
@@ -2425,8 +2465,7 @@ hipError_t launch_rearm(bdk_t *h) {
     default: CALL(32); break;   \
     }
 
-template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStream_t st = nullptr, int end_step = 0) {
-    if (!st) st = h->stream;
+template <typename R, bool Q> static void launch_fft_tq(bdk_t *h, int only, hipStream_t st, int end_step) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     const int Z = h->g.Z;
@@ -2434,13 +2473,13 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStrea
     const R *sq = std::is_same<R, float>::value ? (const R *)h->d.sqf : (const R *)h->d.sq;
     const R *lap = std::is_same<R, float>::value ? (const R *)h->d.lapf : (const R *)h->d.lap;
 #define R2C(E)                                                                                              \
-    hipLaunchKernelGGL((k_r2c_rows<R, E>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,             \
+    hipLaunchKernelGGL((k_r2c_rows<R, E, Q>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,          \
                        (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, c.RB, c.sR1, h->g.xcd)
 #define COL(E)                                                                                              \
-    hipLaunchKernelGGL((k_col<R, E>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,            \
+    hipLaunchKernelGGL((k_col<R, E, Q>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,         \
                        (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC)
 #define C2R(E)                                                                                              \
-    hipLaunchKernelGGL((k_c2r_rows<R, E>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,            \
+    hipLaunchKernelGGL((k_c2r_rows<R, E, Q>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,         \
                        (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, c.RB2, c.sR2, h->g.xcd, h->g, h->d,  \
                        end_step)
     if (only < 0 || only == 0) BDK_EPT_SWITCH(c.e1, R2C)
@@ -2449,6 +2488,14 @@ template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStrea
 #undef R2C
 #undef COL
 #undef C2R
+}
+
+template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStream_t st = nullptr, int end_step = 0) {
+    if (!st) st = h->stream;
+    if (zp2(h->g.Z))
+        launch_fft_tq<R, true>(h, only, st, end_step);
+    else
+        launch_fft_tq<R, false>(h, only, st, end_step);
 }
 
 hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st, int end_step) {
@@ -2468,8 +2515,12 @@ hipError_t launch_fft(bdk_t *h) {
 }
 
 template <typename R, int SCHEME> static void launch_gather_t(bdk_t *h, int with_dt) {
-    hipLaunchKernelGGL((k_gather<R, SCHEME>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d,
-                       with_dt);
+    if (zp2(h->g.Z))
+        hipLaunchKernelGGL((k_gather<R, SCHEME, true>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d,
+                           with_dt);
+    else
+        hipLaunchKernelGGL((k_gather<R, SCHEME, false>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g,
+                           h->d, with_dt);
     if (with_dt) hipLaunchKernelGGL(k_dt_fold, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
 }
 
@@ -2484,12 +2535,15 @@ hipError_t launch_gather(bdk_t *h, int with_dt) {
 }
 
 template <typename R, int SCHEME> static void launch_update_t(bdk_t *h, int fused) {
-    if (fused)
-        hipLaunchKernelGGL((k_update<R, SCHEME, true>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g,
-                           h->d);
+    if (fused && zp2(h->g.Z))
+        hipLaunchKernelGGL((k_update<R, SCHEME, true, true>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream,
+                           h->g, h->d);
+    else if (fused)
+        hipLaunchKernelGGL((k_update<R, SCHEME, true, false>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream,
+                           h->g, h->d);
     else
-        hipLaunchKernelGGL((k_update<R, SCHEME, false>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g,
-                           h->d);
+        hipLaunchKernelGGL((k_update<R, SCHEME, false, true>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream,
+                           h->g, h->d);
 }
 
 hipError_t launch_update(bdk_t *h, int fused) {
@@ -2569,7 +2623,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_col_rhok(const typename V2<R>::
     lds_fft<R, EPT>(s, Z, 1, lds_row(Z), tw, Z, false);
     if (tid != 0) return;
     for (int j = 0; j < Z; ++j) {
-        const double xq = (double)(wrapz(j + Zh, Z) - Zh);
+        const double xq = (double)(wrap1(j + Zh, Z) - Zh);
         const double sg = ((j + m) & 1) ? -1.0 : 1.0;
         if (m > 0) {
             const int bin = (int)(dqr * sqrt(xq * xq + (double)m * m) + 0.5);
@@ -3181,7 +3235,10 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
                        std::vector<RepDev> &reps, std::vector<int2> maps[BS_COUNT]) {
     const int n = (int)mem.size();
     reps.assign(n, RepDev());
-    for (int k = 0; k < BS_COUNT; ++k) maps[k].clear();
+    for (int k = 0; k < BS_COUNT; ++k) {
+        maps[k].clear();
+        bp.nblk2[k] = 0;
+    }
     bp.nmem = n;
     bp.f32 = mem.empty() ? 1 : (mem[0]->p.precision != BDK_FP64);
     bp.tune = tune;
@@ -3191,7 +3248,14 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
     const int *ept = bp.ept;
     bp.dep_lds = bp.l1 = bp.lc = bp.l2 = 0;
     const size_t csz = bp.f32 ? 8 : 16;
-    for (int i = 0; i < n; ++i) {
+    // entries of the members with a power-of-two mesh first: the mesh stages
+    // launch once per mesh kind present (the kind is a template parameter)
+    std::vector<int> order;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < n; ++i)
+            if (zp2(mem[i]->g.Z) == (pass == 0)) order.push_back(i);
+    for (int oi = 0; oi < n; ++oi) {
+        const int i = order[oi];
         const bdk_t *h = mem[i];
         RepDev &r = reps[i];
         r.g = h->g;
@@ -3216,6 +3280,8 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
                                   ad * (int)nblk(h->g.N, 256), ad, (int)nblk(h->g.N, tune.wca_block)};
         for (int k = 0; k < BS_COUNT; ++k)
             for (int b = 0; b < nb[k]; ++b) maps[k].push_back(make_int2(i, b));
+        if (zp2(h->g.Z))
+            for (int k = 0; k < BS_COUNT; ++k) bp.nblk2[k] = (int)maps[k].size();
     }
     for (int k = 0; k < BS_COUNT; ++k) bp.nblk[k] = (int)maps[k].size();
     return hipSuccess;
@@ -3236,6 +3302,12 @@ static unsigned stage_grid(const BatchPlan &bp, int n) {
 
 template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, int stage, hipStream_t st) {
     const RepDev *m = bp.reps;
+    // f(kind, map, entries) once per mesh kind present in stage S's map
+    auto kinds = [&](int S, auto &&f) {
+        const int n2 = bp.nblk2[S], nall = bp.nblk[S];
+        if (n2 > 0) f(std::true_type(), (const int2 *)bp.map[S], n2);
+        if (nall > n2) f(std::false_type(), (const int2 *)bp.map[S] + n2, nall - n2);
+    };
 #define BDK_G(S) dim3(stage_grid(bp, bp.nblk[S]))
     switch (stage) {
     case 0:
@@ -3244,38 +3316,51 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
         break;
     case 1:  // from the list order
     case 8:  // from the slot arrays, beside the rebuild
-        hipLaunchKernelGGL((kb_deposit<R, P>), BDK_G(BS_DEP), dim3(DP_THREADS), bp.dep_lds, st, m, bp.map[BS_DEP],
-                           bp.nblk[BS_DEP], stage == 8 ? 1 : 0);
+        kinds(BS_DEP, [&](auto q, const int2 *mp, int nm) {
+            constexpr bool Q = decltype(q)::value;
+            hipLaunchKernelGGL((kb_deposit<R, P, Q>), dim3(stage_grid(bp, nm)), dim3(DP_THREADS), bp.dep_lds, st, m,
+                               mp, nm, stage == 8 ? 1 : 0);
+        });
         break;
-    case 2: {
-#define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E, P>), BDK_G(BS_R2C), dim3(FFT_THREADS), bp.l1, st, m, bp.map[BS_R2C], \
-                              bp.nblk[BS_R2C])
-        BDK_EPT_SWITCH2(bp.ept[0], R2C)
+    case 2:
+        kinds(BS_R2C, [&](auto q, const int2 *mp, int nm) {
+            constexpr bool Q = decltype(q)::value;
+#define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E, P, Q>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.l1, st, m, mp, nm)
+            BDK_EPT_SWITCH2(bp.ept[0], R2C)
 #undef R2C
+        });
         break;
-    }
-    case 3: {
-#define COL(E) hipLaunchKernelGGL((kb_col<R, E, P>), BDK_G(BS_COL), dim3(FFT_THREADS), bp.lc, st, m, bp.map[BS_COL], \
-                              bp.nblk[BS_COL])
-        BDK_EPT_SWITCH2(bp.ept[1], COL)
+    case 3:
+        kinds(BS_COL, [&](auto q, const int2 *mp, int nm) {
+            constexpr bool Q = decltype(q)::value;
+#define COL(E) hipLaunchKernelGGL((kb_col<R, E, P, Q>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.lc, st, m, mp, nm)
+            BDK_EPT_SWITCH2(bp.ept[1], COL)
 #undef COL
+        });
         break;
-    }
     case 4:
-    case 12: {  // 12: also the end of the step
-#define C2R(E) hipLaunchKernelGGL((kb_c2r<R, E, P>), BDK_G(BS_C2R), dim3(FFT_THREADS), bp.l2, st, m, bp.map[BS_C2R], \
-                              bp.nblk[BS_C2R], stage == 12 ? 1 : 0)
-        BDK_EPT_SWITCH2(bp.ept[2], C2R)
+    case 12:  // 12: also the end of the step
+        kinds(BS_C2R, [&](auto q, const int2 *mp, int nm) {
+            constexpr bool Q = decltype(q)::value;
+#define C2R(E)                                                                                                  \
+    hipLaunchKernelGGL((kb_c2r<R, E, P, Q>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.l2, st, m, mp, nm, \
+                       stage == 12 ? 1 : 0)
+            BDK_EPT_SWITCH2(bp.ept[2], C2R)
 #undef C2R
+        });
         break;
-    }
     case 5:
-        hipLaunchKernelGGL((kb_update<R, P>), BDK_G(BS_PART), dim3(256), 0, st, m, bp.map[BS_PART], bp.nblk[BS_PART]);
+        kinds(BS_PART, [&](auto q, const int2 *mp, int nm) {
+            constexpr bool Q = decltype(q)::value;
+            hipLaunchKernelGGL((kb_update<R, P, Q>), dim3(stage_grid(bp, nm)), dim3(256), 0, st, m, mp, nm);
+        });
         break;
     case 13:  // adaptive members: forces, dt candidates and the fold
         if (bp.nblk[BS_GATHER] > 0) {
-            hipLaunchKernelGGL((kb_gather<R, P>), BDK_G(BS_GATHER), dim3(256), 0, st, m, bp.map[BS_GATHER],
-                               bp.nblk[BS_GATHER]);
+            kinds(BS_GATHER, [&](auto q, const int2 *mp, int nm) {
+                constexpr bool Q = decltype(q)::value;
+                hipLaunchKernelGGL((kb_gather<R, P, Q>), dim3(stage_grid(bp, nm)), dim3(256), 0, st, m, mp, nm);
+            });
             hipLaunchKernelGGL(kb_dt_fold<P>, BDK_G(BS_FOLD), dim3(64), 0, st, m, bp.map[BS_FOLD], bp.nblk[BS_FOLD]);
         }
         break;
