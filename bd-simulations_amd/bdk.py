@@ -42,7 +42,7 @@ class Params(C.Structure):
         ("idgas_only", C.c_int32), ("no_wca", C.c_int32), ("restrict_radial", C.c_int32),
         ("ngp_fix", C.c_int32), ("precision", C.c_int32), ("noise_mode", C.c_int32),
         ("green_mode", C.c_int32), ("device", C.c_int32),
-        ("noise_var", C.c_double), ("seed", C.c_uint64),
+        ("noise_var", C.c_double), ("seed", C.c_uint64), ("ftrho_fix", C.c_int32),
     ]
 
 

@@ -2950,28 +2950,30 @@ hipError_t launch_snap(bdk_t *h, int m, unsigned long long *acc, double *rho, in
 // ------------------------------------------------------------------ direct rho(k)
 // record_ftrho_unkorrigiert (au:307-360): for each wave-vector index pair
 // (i, j) the sums over one species of cos and sin(dq (x i + y j)).  For
-// species 1 the reference takes y = git_y B + rel_x (au:333); species 2 uses
-// rel_y.  Block (chunk, pair): a fixed-order tree sum over the chunk's
-// particles into partial[pair][chunk]; k_ftrho_final adds the chunks in order.
-constexpr int FT_THREADS = 256, FT_PER_THREAD = 16;
+// species 1 the reference takes y = git_y B + rel_x (au:333), unless the
+// parameters ask for rel_y (ftrho_fix); species 2 uses rel_y.  Block (chunk,
+// pair): a fixed-order tree sum over the chunk's particles into
+// partial[pair][chunk]; k_ftrho_final adds the chunks in order.
+constexpr int FT_THREADS = 256, FT_PER_THREAD = 16, FT_MAX_CHUNKS = 64;
 
-__global__ __launch_bounds__(FT_THREADS) void k_ftrho(Geo g, DevBufs d, const int2 *ij, int species, double dq,
-                                                      double2 *partial, int nchunk) {
+// chunk `chunk` of nchunk covers list positions [chunk per, (chunk + 1) per);
+// per is a multiple of FT_THREADS, and nchunk is at most FT_MAX_CHUNKS, so
+// the partial sums stay small at any N.  fix: species 1 takes rel_y.
+__global__ __launch_bounds__(FT_THREADS) void k_ftrho(Geo g, DevBufs d, const int2 *ij, int species, int fix,
+                                                      double dq, double2 *partial, int nchunk, int per) {
     __shared__ double sc[FT_THREADS], ss[FT_THREADS];
     const int p = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
     const int2 q = ij[p];
     double c = 0.0, s = 0.0;
-    const int q0 = chunk * FT_THREADS * FT_PER_THREAD;
-    for (int k = 0; k < FT_PER_THREAD; ++k) {
-        const int t = q0 + k * FT_THREADS + tid;
-        if (t >= g.N) break;
+    const int q0 = chunk * per, q1 = min(q0 + per, g.N);
+    for (int t = q0 + tid; t < q1; t += FT_THREADS) {
         const bool s2 = d.lpid[t] >= g.n1;
         if (s2 != (species == 2)) continue;
         const int cell = d.lcell[t];
         const int gx = div_nc(cell, g), gy = cell - gx * g.nc;
         const double2 r = d.lrel[t];
         const double x = gx * g.B + r.x;
-        const double y = gy * g.B + (species == 1 ? r.x : r.y);
+        const double y = gy * g.B + ((species == 1 && !fix) ? r.x : r.y);
         double sn, cs;
         sincos(dq * (x * q.x + y * q.y), &sn, &cs);
         c += cs;
@@ -3003,17 +3005,29 @@ __global__ void k_ftrho_final(const double2 *partial, int nchunk, int npairs, do
     im[p] = s;
 }
 
+// chunks of at least FT_THREADS * FT_PER_THREAD positions, at most FT_MAX_CHUNKS
+static void ftrho_split(const bdk_t *h, int &nchunk, int &per) {
+    nchunk = std::min((int)nblk(h->g.N, FT_THREADS * FT_PER_THREAD), FT_MAX_CHUNKS);
+    per = (int)nblk(h->g.N, nchunk * FT_THREADS) * FT_THREADS;
+    nchunk = std::max(1, (int)nblk(h->g.N, per));
+}
+
 hipError_t launch_ftrho(bdk_t *h, const int2 *ij, int npairs, int species, double dq, double2 *partial, double *re,
                         double *im) {
-    const int nchunk = (int)nblk(h->g.N, FT_THREADS * FT_PER_THREAD);
-    hipLaunchKernelGGL(k_ftrho, dim3(nchunk, npairs), dim3(FT_THREADS), 0, h->stream, h->g, h->d, ij, species, dq,
-                       partial, nchunk);
+    int nchunk, per;
+    ftrho_split(h, nchunk, per);
+    hipLaunchKernelGGL(k_ftrho, dim3(nchunk, npairs), dim3(FT_THREADS), 0, h->stream, h->g, h->d, ij, species,
+                       h->p.ftrho_fix ? 1 : 0, dq, partial, nchunk, per);
     hipLaunchKernelGGL(k_ftrho_final, dim3(nblk(npairs, 256)), dim3(256), 0, h->stream, partial, nchunk, npairs, re,
                        im);
     return hipGetLastError();
 }
 
-int ftrho_chunks(const bdk_t *h) { return (int)nblk(h->g.N, FT_THREADS * FT_PER_THREAD); }
+int ftrho_chunks(const bdk_t *h) {
+    int nchunk, per;
+    ftrho_split(h, nchunk, per);
+    return nchunk;
+}
 
 // ------------------------------------------------------------------ step counters
 // TimestepInfo (signaturen2.h:71-145, filled by zeitschritt_debug,

@@ -76,6 +76,8 @@ typedef struct bd_params {
     int32_t device;          /* HIP device ordinal */
     double noise_var;        /* variance per component for BDK_NOISE_GAUSS */
     uint64_t seed;           /* Philox key */
+    int32_t ftrho_fix;       /* 0: bdk_ftrho_sums takes the reference's species-1 y = git_y B + rel_x
+                                (au:333); 1: rel_y, the species' actual y */
 } bd_params;
 
 typedef struct bdk bdk_t;
@@ -332,9 +334,11 @@ int bdk_density_stats(bdk_t *h, int32_t m, int32_t nbins, double dbin, const dou
    (i, j), i, j < qbins, of the reference's loop (in loop order; the inner
    loop stops at the first j with i^2 + j^2 > qbins^2), the sums over one
    species of cos and sin(dq_ft (x i + y j)), x = git_x B + rel_x.  For
-   species 1 the reference takes y = git_y B + rel_x (au:333), kept here;
-   species 2 uses rel_y.  re/im receive *npairs (<= qbins^2) entries; the
-   shells by i^2 + j^2 are the caller's (init_ftrho, au:262-300). */
+   species 1 the reference takes y = git_y B + rel_x (au:333), kept here
+   unless bd_params.ftrho_fix is set (then rel_y, as species 2 uses).
+   re/im receive *npairs (<= qbins^2) entries; the shells by i^2 + j^2 are
+   the caller's (init_ftrho, au:262-300).  Device scratch: npairs x at most
+   64 partial sums (16 B each), allocated for the call. */
 int bdk_ftrho_sums(bdk_t *h, int32_t species, int32_t qbins, double dq_ft, double *re, double *im, int32_t *npairs);
 
 /* ---------------------------------------------------------------- batches

@@ -168,3 +168,36 @@ def test_ftrho_sums_match_direct_formula():
             assert abs(re[k] - np.cos(a).sum()) < 1e-9 and abs(im[k] - np.sin(a).sum()) < 1e-9
         assert re[0] == (900 if sp == 1 else 600) and im[0] == 0.0
     s.close()
+    # ftrho_fix: species 1 takes its own rel_y
+    bf, _ = params(n1=900, n2=600, L=L, Z=64, scheme=bdk.CIC, precision=bdk.FP64, seed=73, ftrho_fix=1)
+    f = bdk.RunDynamik(bf)
+    f.set_positions(g, r)
+    re, im = f.ftrho_sums(1, qb, dq)
+    x = g[:900, 0] * B + r[:900, 0]
+    y = g[:900, 1] * B + r[:900, 1]
+    for k, (i, j) in enumerate(pairs):
+        a = dq * (x * i + y * j)
+        assert abs(re[k] - np.cos(a).sum()) < 1e-9 and abs(im[k] - np.sin(a).sum()) < 1e-9
+    f.close()
+
+
+def test_ftrho_sums_many_particles():
+    """300k particles: more particles than 64 chunks of the minimum size, so
+    the chunks grow instead of multiplying (the partial sums stay at 64 per
+    pair); against numpy."""
+    print("case: ftrho sums at 300k particles", flush=True)
+    L, qb = 800.0, 5
+    bp, _ = params(n1=150000, n2=150000, L=L, Z=256, scheme=bdk.CIC, precision=bdk.FP64, seed=79)
+    git, rel = positions(bp, 79)
+    s = bdk.RunDynamik(bp)
+    s.set_positions(git, rel)
+    g, r = s.positions()
+    B, dq = s.B, 2.0 * np.pi / L
+    pairs = [(i, j) for i in range(qb) for j in range(qb) if i * i + j * j <= qb * qb]
+    re, im = s.ftrho_sums(2, qb, dq)
+    x = g[150000:, 0] * B + r[150000:, 0]
+    y = g[150000:, 1] * B + r[150000:, 1]
+    for k, (i, j) in enumerate(pairs):
+        a = dq * (x * i + y * j)
+        assert abs(re[k] - np.cos(a).sum()) < 1e-7 and abs(im[k] - np.sin(a).sum()) < 1e-7
+    s.close()
