@@ -188,7 +188,7 @@ def test_ftrho_sums_many_particles():
     print("case: ftrho sums at 300k particles", flush=True)
     L, qb = 800.0, 5
     bp, _ = params(n1=150000, n2=150000, L=L, Z=256, scheme=bdk.CIC, precision=bdk.FP64, seed=79)
-    git, rel = positions(bp, 79)
+    git, rel = bdk.init_uniform(bp, 79, 0.9)
     s = bdk.RunDynamik(bp)
     s.set_positions(git, rel)
     g, r = s.positions()
