@@ -48,17 +48,14 @@ int hipfail(bdk_t *h, hipError_t e, const char *where) {
 
 bool is_pow2(int z) { return z > 0 && (z & (z - 1)) == 0; }
 
-// Z = 2^a 3^b with a >= 1 and b <= 2 (the FFT passes have radices 2, 4, 8
-// and 3); the reference takes any even Z through FFTW (parameter.h:24, its
+// Z = 2^a 3^b 5^c 7^d with a >= 1 (the FFT passes have radices 2, 4, 8, 3,
+// 5 and 7); the reference takes any even Z through FFTW (parameter.h:24, its
 // default 384 = 2^7 3)
 bool mesh_size_ok(int z) {
     if (z < 8 || z > 4096 || (z & 1)) return false;
-    int b = 0;
-    while (z % 3 == 0) {
-        z /= 3;
-        ++b;
-    }
-    return b <= 2 && is_pow2(z);
+    for (int f : {3, 5, 7})
+        while (z % f == 0) z /= f;
+    return is_pow2(z);
 }
 
 int ilog2(int z) {
@@ -89,7 +86,7 @@ std::string validate(const bd_params &p) {
     if ((int64_t)p.n1 + p.n2 > (1LL << 30)) return "n1+n2 too large";
     if (!(p.L > 0.0) || !std::isfinite(p.L)) return "L must be positive";
     if (p.scheme < 0 || p.scheme > 2) return "densGrid_Schema must be 0, 1 or 2 (NGP/CIC/TSC)";
-    if (!mesh_size_ok(p.Z)) return "Z (FFT_samples) must be 2^a 3^b (a >= 1, b <= 2) in [8, 4096]";
+    if (!mesh_size_ok(p.Z)) return "Z (FFT_samples) must be even, 2^a 3^b 5^c 7^d, in [8, 4096]";
     Derived d = derive(p);
     if (d.nc < 3) return "the box must hold at least 3 list cells per side (L >= 3*2^(1/6))";
     if ((int64_t)d.nc * d.nc > 4096LL * 4096LL) return "too many list cells (nc^2 > 4096^2)";

@@ -154,7 +154,7 @@ __device__ __forceinline__ int wrapc(int u, int n) {
 // u in [-n, 2n)
 __device__ __forceinline__ int wrap1(int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); }
 // Mesh code is compiled for two kinds of mesh size: P2 (Z a power of two:
-// masks and shifts) and the sizes with a factor 3.  Kernels pick the kind
+// masks and shifts) and the other even sizes.  Kernels pick the kind
 // once, at their top, so the power-of-two instruction stream carries no
 // test of the other.
 __device__ __forceinline__ bool zpow2(int Z) { return (Z & (Z - 1)) == 0; }
@@ -1124,12 +1124,33 @@ __device__ __forceinline__ void dft3(const C *v, C *y, bool inv) {
     y[2] = csub(m, r);
 }
 
-// One Stockham pass of radix RAD (2, 3, 4 or 8) over rows of any length n
-// divisible by RAD (the meshes whose size has a factor 3).  The same
-// butterflies as fft_pass, with per-element LDS addresses (lp of every
-// index), exact integer division for the butterfly and row indices, and
-// every twiddle power loaded from the table: tw[t k Z / (Ns RAD)].  Inputs
-// are read before the barrier, outputs written after it (in place).
+// radix-P butterfly for P = 5, 7 as the P-point DFT, y[u] = sum_t v[t] W^(tu
+// mod P), W = exp(-+2 pi i / P) = tw[Z / P] (conjugated for the inverse);
+// P divides Z.
+template <typename C, int P>
+__device__ __forceinline__ void dftp(const C *v, C *y, const C *__restrict__ tw, int Z, bool inv) {
+    const int zp = Z / P;
+    C w[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        w[k] = tw[k * zp];
+        if (inv) w[k].y = -w[k].y;
+    }
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+        C acc = v[0];
+#pragma unroll
+        for (int t = 1; t < P; ++t) acc = cadd(acc, cmul(v[t], w[(t * u) % P]));
+        y[u] = acc;
+    }
+}
+
+// One Stockham pass of radix RAD (2, 3, 4, 5, 7 or 8) over rows of any
+// length n divisible by RAD (the meshes whose size is not a power of two).
+// The same butterflies as fft_pass, with per-element LDS addresses (lp of
+// every index), exact integer division for the butterfly and row indices,
+// and every twiddle power loaded from the table: tw[t k Z / (Ns RAD)].
+// Inputs are read before the barrier, outputs written after it (in place).
 template <typename R, int EPT, int RAD>
 __device__ __forceinline__ void fft_pass_g(typename V2<R>::T *buf, int n, int nrows, int stride, int Ns,
                                            const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
@@ -1164,17 +1185,19 @@ __device__ __forceinline__ void fft_pass_g(typename V2<R>::T *buf, int n, int nr
                 v[e][t] = cmul(v[e][t], ww);
             }
             C y[RAD];
-            if (RAD == 2) {
+            if constexpr (RAD == 2) {
                 y[0] = cadd(v[e][0], v[e][1]);
                 y[1] = csub(v[e][0], v[e][1]);
-            } else if (RAD == 3) {
+            } else if constexpr (RAD == 3) {
                 dft3(v[e], y, inv);
-            } else if (RAD == 4) {
+            } else if constexpr (RAD == 4) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) y[t] = v[e][t];
                 dft4(y, inv);
-            } else {
+            } else if constexpr (RAD == 8) {
                 dft8(v[e], y, inv);
+            } else {
+                dftp<C, RAD>(v[e], y, tw, Z, inv);
             }
             C *q = buf + row * stride;
             const int base = (j - k) * RAD + k;
@@ -1185,16 +1208,25 @@ __device__ __forceinline__ void fft_pass_g(typename V2<R>::T *buf, int n, int nr
     __syncthreads();
 }
 
-// Transform of rows of length n = 2^p 3^c (c <= 2): radix-8 passes over the
+// Transform of rows of length n = 2^p 3^c 5^d 7^f: radix-8 passes over the
 // factor 2^p, one radix-4 or radix-2 pass for its remaining bits, then one
-// radix-3 pass per factor 3.  A power-of-two n takes fft_pass (below).
+// radix-3, -5 or -7 pass per odd factor.  A power-of-two n takes fft_pass
+// (below).
 template <typename R, int EPT>
 __device__ __forceinline__ void lds_fft_g(typename V2<R>::T *buf, int n, int nrows, int stride,
                                           const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
-    int m = n, c = 0;
+    int m = n, c3 = 0, c5 = 0, c7 = 0;
     while (m % 3 == 0) {
         m /= 3;
-        ++c;
+        ++c3;
+    }
+    while (m % 5 == 0) {
+        m /= 5;
+        ++c5;
+    }
+    while (m % 7 == 0) {
+        m /= 7;
+        ++c7;
     }
     int Ns = 1;
     while (m >= 8) {
@@ -1209,14 +1241,22 @@ __device__ __forceinline__ void lds_fft_g(typename V2<R>::T *buf, int n, int nro
         fft_pass_g<R, EPT, 2>(buf, n, nrows, stride, Ns, tw, Z, inv);
         Ns *= 2;
     }
-    for (; c > 0; --c) {
+    for (; c3 > 0; --c3) {
         fft_pass_g<R, EPT, 3>(buf, n, nrows, stride, Ns, tw, Z, inv);
         Ns *= 3;
     }
+    for (; c5 > 0; --c5) {
+        fft_pass_g<R, EPT, 5>(buf, n, nrows, stride, Ns, tw, Z, inv);
+        Ns *= 5;
+    }
+    for (; c7 > 0; --c7) {
+        fft_pass_g<R, EPT, 7>(buf, n, nrows, stride, Ns, tw, Z, inv);
+        Ns *= 7;
+    }
 }
 
-// rows of length n: the power-of-two passes below, or lds_fft_g when n has a
-// factor 3 (tw holds exp(-2 pi i k / Z), k < Z; n divides Z)
+// rows of length n: the power-of-two passes below, or lds_fft_g when n has an
+// odd factor (tw holds exp(-2 pi i k / Z), k < Z; n divides Z)
 template <typename R, int EPT>
 __device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
                                              const typename V2<R>::T *__restrict__ tw, int logZ, bool inv);

@@ -56,7 +56,7 @@ enum bdk_mesh { BDK_MESH_RHO = 0, BDK_MESH_FX = 1, BDK_MESH_FY = 2 };
 typedef struct bd_params {
     int32_t n1, n2;          /* N1, N2 (ph:15-16) */
     int32_t scheme;          /* densGrid_Schema (ph:22) */
-    int32_t Z;               /* FFT_samples, 2^a 3^b (a >= 1, b <= 2), 8..4096 (ph:24) */
+    int32_t Z;               /* FFT_samples: even, 2^a 3^b 5^c 7^d, 8..4096 (ph:24) */
     double L;                /* box side (ph:19) */
     double lambda;           /* lambda_kapillar (ph:29) */
     double kap;              /* kapillar_vorfaktor (ph:32) */

@@ -37,7 +37,7 @@ int main() {
     CHECK(bdk_steps_for_interval(0.1, 1e-4, &k) == BDK_OK && k == 1001);
     // validation failures (before any device call)
     bd_params bad = p;
-    bad.Z = 40;  // a factor 5
+    bad.Z = 44;  // a factor 11
     bdk_t *h = nullptr;
     CHECK(bdk_create(&bad, &h) == BDK_ERR_ARG && h == nullptr && std::strlen(bdk_last_error(nullptr)) > 0);
     bad = p;

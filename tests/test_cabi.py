@@ -59,8 +59,8 @@ def test_derived_constants_match_oracle():
 
 
 @pytest.mark.parametrize("bad,msg", [
-    (dict(Z=100), "FFT_samples"), (dict(Z=8192), "FFT_samples"), (dict(Z=54), "FFT_samples"),
-    (dict(Z=6), "FFT_samples"), (dict(scheme=3), "Schema"),
+    (dict(Z=22), "FFT_samples"), (dict(Z=8192), "FFT_samples"), (dict(Z=26), "FFT_samples"),
+    (dict(Z=6), "FFT_samples"), (dict(Z=105), "FFT_samples"), (dict(scheme=3), "Schema"),
     (dict(L=3.0), "3 list cells"), (dict(n1=0, n2=0), "n1+n2"), (dict(lam=0.0), "lambda"),
     (dict(dt_max=0.0), "dt_max"), (dict(precision=5), "precision"), (dict(noise_mode=9), "noise_mode"),
     (dict(max_travel=0.0), "max_travel"),
@@ -74,10 +74,10 @@ def test_create_rejects_invalid_parameters(bad, msg):
         bdk.footprint(bp)
 
 
-@pytest.mark.parametrize("Z", [8, 12, 18, 24, 36, 96, 384, 576, 1152, 3072, 4096])
-def test_mesh_sizes_with_a_factor_three_are_accepted(Z):
-    """Z = 2^a 3^b (a >= 1, b <= 2): the reference's default FFT_samples =
-    384 (parameter.h:24) included; footprint() validates without a GPU."""
+@pytest.mark.parametrize("Z", [8, 10, 12, 14, 18, 24, 36, 54, 96, 100, 250, 384, 576, 1000, 1400, 3072, 4096])
+def test_even_mesh_sizes_are_accepted(Z):
+    """Z even and 2^a 3^b 5^c 7^d: the reference's default FFT_samples = 384
+    (parameter.h:24) included; footprint() validates without a GPU."""
     bp, _ = params(Z=Z, L=max(40.0, 0.6 * Z))
     assert bdk.footprint(bp) > 0
 

@@ -1,7 +1,7 @@
-"""GPU parity on meshes whose size has a factor 3 (Z = 2^a 3^b, b <= 2),
-among them the reference's default FFT_samples = 384 (parameter.h:24),
-against the FP64 oracle (whose DFT of these lengths is pinned against numpy
-in test_oracle.py::test_fft2d_lengths_with_a_factor_three).
+"""GPU parity on meshes whose size is not a power of two (even Z = 2^a 3^b
+5^c 7^d), among them the reference's default FFT_samples = 384
+(parameter.h:24), against the FP64 oracle (whose DFT of these lengths is
+pinned against numpy in test_oracle.py::test_fft2d_lengths_not_powers_of_two).
 
 Tolerances as in test_gpu_parity.py: FP64 forces and meshes within 1e-12
 relative RMS, FP32 within 1e-5; FP64 trajectories keep identical cells and
@@ -34,10 +34,13 @@ def check_forces(g, o, tol):
 
 
 # 12 = 4*3 and 18 = 2*9 have no radix-8 pass; 24 = 8*3; 96 = 32*3 (radix 8,
-# 4, 3); 144 = 16*9 (two radix-3 passes); 384 = 128*3 (8, 8, 2, 3)
+# 4, 3); 144 = 16*9 (two radix-3 passes); 384 = 128*3 (8, 8, 2, 3); 10 = 2*5
+# and 14 = 2*7; 50 = 2*25 (two radix-5 passes); 98 = 2*49; 210 = 2*3*5*7
+# (every odd radix); 250 = 2*125
 @pytest.mark.parametrize("scheme", [bdk.NGP, bdk.CIC, bdk.TSC])
 @pytest.mark.parametrize("prec,tol", [(bdk.FP64, TOL64), (bdk.FP32, TOL32)])
-@pytest.mark.parametrize("Z,L", [(12, 10.0), (18, 12.0), (24, 16.0), (96, 60.0), (144, 90.0), (384, 200.0)])
+@pytest.mark.parametrize("Z,L", [(12, 10.0), (18, 12.0), (24, 16.0), (96, 60.0), (144, 90.0), (384, 200.0),
+                                 (10, 8.0), (14, 10.0), (50, 30.0), (98, 60.0), (210, 120.0), (250, 150.0)])
 def test_capillary_parity_factor3(scheme, prec, tol, Z, L):
     say("case: kap Z=%d scheme=%d prec=%d" % (Z, scheme, prec))
     n = int(0.2 * L * L)

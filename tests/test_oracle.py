@@ -141,19 +141,19 @@ def test_fft2d_numpy():
         assert rel_rms(O.fft2d(a, +1), np.fft.ifft2(a) * n * n) < 1e-14
 
 
-def test_fft2d_lengths_with_a_factor_three():
-    """The direct DFT of lengths 2^a 3^b (the reference's default mesh is
-    384): against the matrix definition at small n and numpy.fft up to 384.
-    A wrong sign or exponent (jk instead of jk mod n is harmless, j + k is
-    not) fails either check."""
+def test_fft2d_lengths_not_powers_of_two():
+    """The direct DFT of even lengths with factors 3, 5, 7 (the reference's
+    default mesh is 384): against the matrix definition at small n and
+    numpy.fft up to 384.  A wrong sign or exponent (jk instead of jk mod n is
+    harmless, j + k is not) fails either check."""
     rng = np.random.default_rng(5)
-    for n in (6, 12, 18):
+    for n in (6, 10, 12, 14, 18):
         a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
         j = np.arange(n)
         W = np.exp(-2j * np.pi * np.outer(j, j) / n)
         want = W @ a @ W.T
         assert np.allclose(O.fft2d(a, -1), want, rtol=0, atol=1e-12 * np.abs(want).max())
-    for n in (24, 96, 384):
+    for n in (24, 50, 96, 98, 384):
         a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
         assert rel_rms(O.fft2d(a, -1), np.fft.fft2(a)) < 1e-13
         assert rel_rms(O.fft2d(a, +1), np.fft.ifft2(a) * n * n) < 1e-13

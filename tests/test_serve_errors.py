@@ -40,7 +40,7 @@ def test_submit_validation():
     with pytest.raises(bdk.BdkError):
         srv.submit(p, git, rel, 10)  # footprint above the budget
     srv.close()
-    bad = bdk.make_params(n1=60, n2=40, L=20.0, Z=20, precision=bdk.FP32)  # Z has a factor 5
+    bad = bdk.make_params(n1=60, n2=40, L=20.0, Z=22, precision=bdk.FP32)  # Z has a factor 11
     srv = bdk.Server(hbm_budget=1 << 34, max_jobs=2, tick_steps=10, device=99)
     with pytest.raises(bdk.BdkError):
         srv.submit(bad, git, rel, 10)
