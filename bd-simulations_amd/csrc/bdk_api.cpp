@@ -2100,7 +2100,7 @@ int bdk_batch_set_tuning(bdk_batch_t *b, const int32_t *choice) {
     if (t.wca_block != 128 && t.wca_block != 256 && t.wca_block != 512)
         return b->fail(BDK_ERR_ARG, "WCA block size must be 128, 256 or 512");
     t.dep_tile = choice[BDK_TUNE_DEPOSIT_TILE];
-    if (t.dep_tile < 8 || t.dep_tile > 64) return b->fail(BDK_ERR_ARG, "deposit tile must be 8..64 mesh points");
+    if (t.dep_tile < 8 || t.dep_tile > 96) return b->fail(BDK_ERR_ARG, "deposit tile must be 8..96 mesh points");
     t.persist = choice[BDK_TUNE_PERSIST];
     if (t.persist < 0) return b->fail(BDK_ERR_ARG, "persistent grid must be >= 0");
     hipError_t e = hipSetDevice(b->dev);
@@ -2114,7 +2114,8 @@ int bdk_batch_set_tuning(bdk_batch_t *b, const int32_t *choice) {
 // with the stages that leave the particles unchanged, one knob at a time in
 // this order, keeping the fastest value: the FFT elements per thread of each
 // pass (8, 16, 32; stages 2-4), the WCA block size (128, 256, 512; stage 0),
-// the deposit tile (mesh points per tile side 24, 32, 38, 48; stage 1), and
+// the deposit tile (mesh points per tile side 24, 32, 38, 48, 64, 80; stage 1:
+// fine meshes put few particles in a small tile), and
 // the persistent grid (one block per map entry, or 4 or 8 blocks per compute
 // unit for every stage; stages 0-4 together).  No choice changes a result.
 int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *out) {
@@ -2158,7 +2159,7 @@ int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *out) {
         rc = pick([&](bdkx::BatchTuning &t, int v) { return v >= b->ept_min[stage] ? (t.ept[stage] = v, true) : false; },
                   {8, 16, 32}, {2 + stage});
     if (!rc) rc = pick([](bdkx::BatchTuning &t, int v) { return (t.wca_block = v, true); }, {128, 256, 512}, {0});
-    if (!rc) rc = pick([](bdkx::BatchTuning &t, int v) { return (t.dep_tile = v, true); }, {24, 32, 38, 48}, {1});
+    if (!rc) rc = pick([](bdkx::BatchTuning &t, int v) { return (t.dep_tile = v, true); }, {24, 32, 38, 48, 64, 80}, {1});
     if (!rc)
         rc = pick([](bdkx::BatchTuning &t, int v) { return (t.persist = v, true); }, {0, 4 * cus, 8 * cus},
                   {0, 1, 2, 3, 4});

@@ -377,7 +377,7 @@ int bdk_batch_status(bdk_batch_t *b, int32_t n, int32_t *status);
    launches of the stages that leave the particles unchanged) and keeps the
    fastest value: the FFT elements per thread of each pass (8, 16, 32), the
    WCA block size (128, 256, 512 threads), the deposit tile (list cells per
-   side such that a tile owns at most 24^2, 32^2, 38^2 or 48^2 mesh points),
+   side such that a tile owns at most 24^2, 32^2, 38^2, 48^2, 64^2 or 80^2 mesh points),
    and the persistent grid (one block per work item, or 4 or 8 blocks per
    compute unit that loop over the work items).  No choice changes a
    result.  choice_out[BDK_TUNE_COUNT] (may be NULL) receives the choice. */
@@ -389,7 +389,7 @@ enum bdk_tune_index {
 int bdk_batch_tune(bdk_batch_t *b, int32_t reps, int32_t *choice_out);
 /* Apply a choice (e.g. one bdk_batch_tune made for the same mix): FFT
    elements per thread 8, 16 or 32 (raised to the members' minimum), WCA
-   block size 128, 256 or 512, deposit tile 8..64 mesh points per side,
+   block size 128, 256 or 512, deposit tile 8..96 mesh points per side,
    persistent grid >= 0 (0: one block per work item).  No choice changes a
    result. */
 int bdk_batch_set_tuning(bdk_batch_t *b, const int32_t *choice);

@@ -63,7 +63,7 @@ def test_batch_members_equal_alone(prec):
     choice = b.tune(3)
     say("tuned", choice)
     assert all(choice[k] in (8, 16, 32) for k in ("ept_r2c", "ept_col", "ept_c2r"))
-    assert choice["wca_block"] in (128, 256, 512) and choice["deposit_tile"] in (24, 32, 38, 48)
+    assert choice["wca_block"] in (128, 256, 512) and choice["deposit_tile"] in (24, 32, 38, 48, 64, 80)
     b.run_steps(11)
     b.sync()
     for s in alone:
@@ -80,6 +80,7 @@ def test_batch_members_equal_alone(prec):
     dict(ept_r2c=32, ept_col=32, ept_c2r=32, wca_block=512, deposit_tile=24, persist=64),
     dict(ept_r2c=16, ept_col=8, ept_c2r=32, wca_block=128, deposit_tile=48, persist=1),
     dict(ept_r2c=8, ept_col=16, ept_c2r=16, wca_block=256, deposit_tile=32, persist=0),
+    dict(ept_r2c=8, ept_col=8, ept_c2r=8, wca_block=256, deposit_tile=80, persist=0),
 ])
 def test_batch_tuning_choices_keep_results(choice):
     """Every knob of the autotuner (FFT elements per thread, WCA block size,
