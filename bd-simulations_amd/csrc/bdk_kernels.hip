@@ -154,9 +154,9 @@ __device__ __forceinline__ int wrapc(int u, int n) {
 // u in [-n, 2n)
 __device__ __forceinline__ int wrap1(int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); }
 // Mesh code is compiled for two kinds of mesh size: P2 (Z a power of two:
-// masks and shifts) and the other even sizes.  Kernels pick the kind
-// once, at their top, so the power-of-two instruction stream carries no
-// test of the other.
+// masks and shifts) and the other even sizes.  The kind is a template
+// parameter of every mesh kernel, chosen by its launcher, so the
+// power-of-two instruction stream carries no test of the other.
 __device__ __forceinline__ bool zpow2(int Z) { return (Z & (Z - 1)) == 0; }
 // mesh index u in [-Z, 2Z) into [0, Z)
 template <bool P2> __device__ __forceinline__ int wrapz(int u, int Z) { return P2 ? (u & (Z - 1)) : wrap1(u, Z); }
