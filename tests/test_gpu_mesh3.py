@@ -74,8 +74,10 @@ def test_reference_default_setup(prec):
 
 def test_factor3_graph_replay_and_batch():
     """A 384^2 simulation: graph-replayed steps equal single steps bit for bit,
-    and the same simulation as a batch member (beside a power-of-two member)
-    equals its run alone."""
+    and the same simulation as a batch member (beside a power-of-two member
+    and an adaptive-dt member on a 120^2 mesh, so every mesh stage and the
+    adaptive gather run once per mesh kind) equals its run alone, as does the
+    adaptive member."""
     say("case: 384^2 graph replay and batch")
     kw = dict(n1=3000, n2=3000, L=150.0, Z=384, scheme=bdk.TSC, precision=bdk.FP32, lam=20.0, seed=41)
     a, _, git, rel = pair(**kw)
@@ -90,11 +92,21 @@ def test_factor3_graph_replay_and_batch():
     c = bdk.RunDynamik(a.p)
     c.set_positions(git, rel)
     d, _, _, _ = pair(n1=800, n2=800, L=50.0, Z=64, scheme=bdk.CIC, precision=bdk.FP32, seed=43)
-    batch = bdk.Batch([c, d])
+    # an adaptive-dt member on a 3 * 5 * 2^3 mesh, alone and in the batch
+    kw_e = dict(n1=700, n2=500, L=60.0, Z=120, scheme=bdk.TSC, precision=bdk.FP32, seed=47, max_travel=0.05,
+                dt_max=5e-3, T=2.0)
+    e_alone, _, git_e, rel_e = pair(**kw_e)
+    e = bdk.RunDynamik(e_alone.p)
+    e.set_positions(git_e, rel_e)
+    batch = bdk.Batch([c, d, e])
     batch.run_steps(20)
     batch.sync()
     gc, rc = c.positions()
     assert np.array_equal(gc, ga) and np.array_equal(rc, ra)
+    e_alone.run_steps(20)
+    ge, re_ = e.positions()
+    gw, rw = e_alone.positions()
+    assert np.array_equal(ge, gw) and np.array_equal(re_, rw)
     batch.close()
-    for s in (a, b, c, d):
+    for s in (a, b, c, d, e, e_alone):
         s.close()
