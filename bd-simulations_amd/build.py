@@ -41,8 +41,10 @@ def build_lib():
 
 def driver_cmd(main_src, exe, param_header=None, extra=()):
     """g++ line of a driver over the host shim, compiled against the constants
-    of a reference-format parameter.h (default: host/parameter.h)."""
-    inc = os.path.dirname(os.path.abspath(param_header)) if param_header else HOST
+    of a reference-format parameter.h (default: host/small_run/parameter.h).
+    host/ itself holds no parameter.h, so the quoted #include "parameter.h" of
+    every shim file falls through to the -I directory of the header given."""
+    inc = os.path.dirname(os.path.abspath(param_header)) if param_header else os.path.join(HOST, "small_run")
     return (["g++", "-O2", "-std=c++17", "-Wall", "-I" + inc, "-I" + HOST, "-I" + os.path.join(ROOT, "include")]
             + list(extra)
             + [os.path.join(HOST, "run_zustand.cpp"), os.path.join(HOST, "params_from_header.cpp"), main_src,
@@ -50,13 +52,17 @@ def driver_cmd(main_src, exe, param_header=None, extra=()):
 
 
 def build_driver(param_header=None):
-    """bdk_sim: main.cpp's run loop (main.cpp:130-300) over the C-ABI; and
+    """bdk_sim: main.cpp's run loop (main.cpp:130-300) over the C-ABI;
     bdk_refcalls: the driver of tests/drop_in that uses only the reference's
-    calls and parameter.h globals."""
+    calls and parameter.h globals; and bdk_refdefault: the same driver
+    against the reference's default values (tests/drop_in/refdefault)."""
     run(driver_cmd(os.path.join(HOST, "bdk_sim_main.cpp"), os.path.join(HERE, "bdk_sim"), param_header))
     ref_main = os.path.join(ROOT, "tests", "drop_in", "reference_calls_main.cpp")
     if os.path.exists(ref_main):
         run(driver_cmd(ref_main, os.path.join(HERE, "bdk_refcalls"), param_header))
+        ref_hdr = os.path.join(ROOT, "tests", "drop_in", "refdefault", "parameter.h")
+        if os.path.exists(ref_hdr):
+            run(driver_cmd(ref_main, os.path.join(HERE, "bdk_refdefault"), ref_hdr))
 
 
 if __name__ == "__main__":

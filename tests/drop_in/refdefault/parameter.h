@@ -1,7 +1,9 @@
 // Run constants in the format of the reference's parameter.h (same names,
-// same meaning, parameter.h:14-99 and 167-191).  Values: the CPU reference
-// configuration -- L = 64, N1 = N2 = 1000, CIC on 128^2, lambda = 8,
-// q2/q1 = 0.5, kT = 1, dt = 1e-4 fixed, 10 checkpoints of 0.1.
+// same meaning, parameter.h:14-99 and 167-191).  Values: the reference's own
+// defaults (parameter.h:9-101) -- L = 300, N1 = N2 = 2000, TSC on 384^2,
+// lambda = 25, kapillar_vorfaktor = 1.111111111, q2/q1 = -1, kT = 24.3055555,
+// dt_max = 0.005 with max_reisedistanz = 0.24 (adaptive dt), 12 checkpoints
+// of 0.2.
 #pragma once
 
 #define _USE_MATH_DEFINES
@@ -10,22 +12,22 @@
 
 using std::string;
 
-const int N1 = 1000;
-const int N2 = 1000;
-const double L = 64.0;
-const int densGrid_Schema = 1;      // 0 NGP, 1 CIC, 2 TSC
-const int FFT_samples = 128;        // power of two
-const double lambda_kapillar = 8.0;
-const double kapillar_vorfaktor = 1.0;
-const double T = 1.0;
-const double dt_max = 1.0e-4;
-const double max_reisedistanz = HUGE_VAL;  // unbounded travel: dt = dt_max every step
+const int N1 = 2000;
+const int N2 = 2000;
+const double L = 300.0;
+const int densGrid_Schema = 2;      // 0 NGP, 1 CIC, 2 TSC
+const int FFT_samples = 384;        // 2^7 * 3
+const double lambda_kapillar = 25.0;
+const double kapillar_vorfaktor = 1.111111111;
+const double T = 24.3055555;
+const double dt_max = 0.005;
+const double max_reisedistanz = 0.24;
 const double sigma11_22 = 1.0;
 const double sigma11_12 = 0.5 * (1.0 + sigma11_22);
 const double Gamma2_1 = sigma11_22;
 const double eps22_11 = 1.0;
 const double eps12_11 = 1.0;
-const double f2_f1 = 0.5;
+const double f2_f1 = -1.0;
 const bool idgas_only = false;
 const bool noWCA = false;
 const bool noRNG = false;
@@ -36,8 +38,8 @@ const bool krafttestmode = false;
 const double maxPairWCA = 1.0e5;
 const int max_write_interval = 900;
 const int runs = 1;
-const double obs_dt = 0.1;
-const int obs_anzahl = 11;
+const double obs_dt = 0.2;
+const int obs_anzahl = 12;
 const int startpos_methode = 1;
 const bool auswerten_animation = true;
 const bool auswerten_korrfunk = false;        // g11, g22 at every checkpoint

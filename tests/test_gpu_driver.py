@@ -1,5 +1,5 @@
 """The C++ drop-in driver (bd-simulations_amd/bdk_sim: main.cpp:130-300's run
-loop over RunZustand, built against host/parameter.h: N1 = N2 = 1000, L = 64,
+loop over RunZustand, built against host/small_run/parameter.h: N1 = N2 = 1000, L = 64,
 CIC 128^2, lambda 8, dt_max 1e-4, obs_dt 0.1, 11 checkpoints) writes the same
 pos1_1.txt / pos2_1.txt bytes as the same run driven through the Python
 binding: reference start positions from the seeded mt19937_64 (init_zufallspos,
