@@ -660,6 +660,7 @@ __global__ __launch_bounds__(256) void k_wca_f32(Geo g, DevBufs d) {
 // charge into a coarse and a fine integer part (two accumulators), which
 // keeps the sum exact to ~2^-(S+40).
 constexpr int DP_THREADS = 256;
+constexpr int DP_SEG_TAB = 1536;  // staged particles with a segment table entry
 
 __device__ int first_owned(int c, const Geo &g) {
     if (c <= 0) return 0;
@@ -681,13 +682,16 @@ __global__ void k_own(Geo g, int *own) {
 // FP32 deposit into a window that holds the whole stencil of every staged
 // particle (mesh rows kw0 .. kw0 + nwx - 1, columns lw0 .., unwrapped): one
 // base index per particle and nine fixed offsets, no per-point bounds.  The
-// charges are rounded to 32-bit fixed point (scale 2^S, |charge| 2^S < 2^31,
-// far below FP32 resolution) and summed in 64 bits, so the sums stay exact
-// and independent of the order of the atomics.
+// charges are truncated to 32-bit fixed point (scale 2^S, |charge| 2^S <
+// 2^31: a truncation error of 2^-31 of the largest charge, far below FP32
+// resolution) and summed in 64 bits, so the sums stay exact and independent
+// of the order of the atomics.  dqs: the particle's charge drho q times 2^S
+// (exact: a power of two), so a point's fixed-point value is
+// (int)((wx wy) dqs).
 template <int SCHEME, bool P2>
-__device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs &d, const DepArgs &a, int gx,
-                                                     int gy, double2 rr, bool s2, int kw0, int lw0, int nwx,
-                                                     int nwy, unsigned long long *acc) {
+__device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs &d, float dqs, int gx, int gy,
+                                                     double2 rr, int kw0, int lw0, int nwx, int nwy,
+                                                     unsigned long long *acc) {
     const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
     int kp, lp;
     float wx[3], wy[3];
@@ -698,26 +702,23 @@ __device__ __forceinline__ void deposit_particle_win(const Geo &g, const DevBufs
         atomicOr(&d.sc->nonfinite, 4);
         return;
     }
-    const float dq = (float)g.drho * (float)(s2 ? a.q2 : a.q1);
     unsigned long long *p = acc + kb * nwy + lb;
     if (SCHEME == 0) {
-        const int X = __float2int_rn(dq * a.scale_f);
-        atomicAdd(&p[nwy + 1], (unsigned long long)(long long)X);
+        atomicAdd(&p[nwy + 1], (unsigned long long)(long long)(int)dqs);
         return;
     }
 #pragma unroll
     for (int ia = 0; ia < 3; ++ia)
 #pragma unroll
         for (int ib = 0; ib < 3; ++ib) {
-            const float cf = (wx[ia] * wy[ib]) * dq;
-            const int X = __float2int_rn(cf * a.scale_f);
+            const int X = (int)((wx[ia] * wy[ib]) * dqs);
             atomicAdd(&p[ia * nwy + ib], (unsigned long long)(long long)X);
         }
 }
 
 template <typename R, int SCHEME, bool P2>
-__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, int gx, int gy,
-                                                 double2 rr, bool s2, int klo, int khi, int llo, int lhi,
+__device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d, const DepArgs &a, float dqs, int gx,
+                                                 int gy, double2 rr, bool s2, int klo, int khi, int llo, int lhi,
                                                  unsigned long long *acc_hi, unsigned long long *acc_lo) {
     const double x = gx * g.B + rr.x, y = gy * g.B + rr.y;
     int kp, lp;
@@ -741,10 +742,8 @@ __device__ __forceinline__ void deposit_particle(const Geo &g, const DevBufs &d,
             if (std::is_same<R, float>::value) {
                 // the charge and its fixed point exactly as in the window path,
                 // so the mesh does not depend on which path a tile takes
-                const float dq = (float)g.drho * (float)(s2 ? a.q2 : a.q1);
-                const float cf = SCHEME == 0 ? dq : ((float)wx[ia] * (float)wy[ib]) * dq;
-                const int X = __float2int_rn(cf * a.scale_f);
-                atomicAdd(&acc_hi[w], (unsigned long long)(long long)X);
+                const float cf = SCHEME == 0 ? dqs : ((float)wx[ia] * (float)wy[ib]) * dqs;
+                atomicAdd(&acc_hi[w], (unsigned long long)(long long)(int)cf);
             } else {
                 double cd;
                 if (SCHEME == 0)
@@ -773,6 +772,7 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     int *seg_q = (int *)(acc_hi + acc_n);
     int *seg_pre = seg_q + a.seg_cap;       // [seg_cap + 1]
     int *seg_row = seg_pre + a.seg_cap + 1;
+    unsigned short *seg_of = (unsigned short *)(seg_row + a.seg_cap);  // [DP_SEG_TAB]
     __shared__ int info[4];
     __shared__ int wsum[16];
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -803,6 +803,8 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
     if (tid < 4) info[tid] = d.own[tid == 0 ? cx0 : tid == 1 ? cx0 + nx : tid == 2 ? cy0 : cy0 + ny];
     __syncthreads();
     const int klo = info[0], khi = info[1], llo = info[2], lhi = info[3];
+    // FP32: the species charges in fixed point, drho q 2^S (2^S exact)
+    const float dqs1 = ((float)g.drho * (float)a.q1) * a.scale_f, dqs2 = ((float)g.drho * (float)a.q2) * a.scale_f;
     const double e = e_sh + 1e-9 * g.L;
     int hx = (int)floor((1.5 * g.h + e) / g.B) + 1;
     if (hx < 1) hx = 1;
@@ -853,13 +855,24 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
         const int total = block_exscan4(seg_pre, nseg, wsum);
         if (tid == 0) seg_pre[nseg] = total;
         __syncthreads();
+        // the segment of every staged particle, written once per segment
+        // (a table lookup per particle instead of a binary search)
+        const bool tab = total <= DP_SEG_TAB;
+        if (tab) {
+            for (int i = tid; i < nseg; i += nt)
+                for (int t = seg_pre[i], te = seg_pre[i + 1]; t < te; ++t) seg_of[t] = (unsigned short)i;
+            __syncthreads();
+        }
         for (int t = tid; t < total; t += nt) {
             int lo = 0, hi = nseg - 1;  // last segment with seg_pre <= t
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (seg_pre[mid] <= t) lo = mid;
-                else hi = mid - 1;
-            }
+            if (tab)
+                lo = seg_of[t];
+            else
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (seg_pre[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
             const int q = seg_q[lo] + (t - seg_pre[lo]);
             int gx, gy;
             double2 rr;
@@ -876,10 +889,11 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
                 rr = d.lrel[q];
                 s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
             }
+            const float dqs = s2 ? dqs2 : dqs1;
             if (win)
-                deposit_particle_win<SCHEME, P2>(g, d, a, gx, gy, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
+                deposit_particle_win<SCHEME, P2>(g, d, dqs, gx, gy, rr, kw0, lw0, nwx, nwy, acc_hi);
             else
-                deposit_particle<R, SCHEME, P2>(g, d, a, gx, gy, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
+                deposit_particle<R, SCHEME, P2>(g, d, a, dqs, gx, gy, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
         }
     } else {
         for (int idx = tid; idx < SX * SY; idx += nt) {
@@ -900,10 +914,12 @@ __device__ __forceinline__ void deposit_body(const Geo &g, const DevBufs &d, con
                     rr = d.lrel[q];
                     s2 = (((const unsigned *)d.list)[q] >> 31) != 0;
                 }
+                const float dqs = s2 ? dqs2 : dqs1;
                 if (win)
-                    deposit_particle_win<SCHEME, P2>(g, d, a, px, py, rr, s2, kw0, lw0, nwx, nwy, acc_hi);
+                    deposit_particle_win<SCHEME, P2>(g, d, dqs, px, py, rr, kw0, lw0, nwx, nwy, acc_hi);
                 else
-                    deposit_particle<R, SCHEME, P2>(g, d, a, px, py, rr, s2, klo, khi, llo, lhi, acc_hi, acc_lo);
+                    deposit_particle<R, SCHEME, P2>(g, d, a, dqs, px, py, rr, s2, klo, khi, llo, lhi, acc_hi,
+                                                    acc_lo);
             }
         }
     }
@@ -2465,7 +2481,8 @@ static DepPlan dep_plan(const bdk_t *h, int reset, int tile_pts = 38) {
         pl.a.iscale_f = std::ldexp(1.0, -sf);
     }
     pl.grid = dim3(nblk(g.nc, tc), nblk(g.nc, tc));
-    pl.lds = (size_t)std::max(pl.a.win_cap * (f64 ? 2 : 1), pl.a.win_fast) * 8 + (size_t)(3 * pl.a.seg_cap + 1) * 4;
+    pl.lds = (size_t)std::max(pl.a.win_cap * (f64 ? 2 : 1), pl.a.win_fast) * 8 + (size_t)(3 * pl.a.seg_cap + 1) * 4 +
+             (size_t)DP_SEG_TAB * 2;
     return pl;
 }
 
