@@ -11,8 +11,14 @@ The replica mixes follow the reference's use of `runs` and parameter sweeps
   served jobs - total N log-uniform in [1000, 256000], species split uniform
                 in [0.2, 0.8], density uniform in [0.3, 0.7], lambda uniform
                 in [5, 100], NGP/CIC/TSC, steps log-uniform in [1e3, 1e5],
-                uniform or Gaussian-blob start; Poisson arrivals.
-All in FP32 forces with FP64 positions.
+                uniform or Gaussian-blob start, the reference's default
+                travel cap max_reisedistanz = 0.24 (parameter.h:43: the
+                adaptive dt of dm:1420-1491, at most dt_max = 1e-4);
+                Poisson arrivals.
+All in FP32 forces with FP64 positions.  Without the travel cap a blob
+start at dt = 1e-4 turns unstable after a few thousand steps (a close pair
+under the explicit Euler step), and the jobs of up to 1e5 steps then spend
+most of their steps in a blown-up state that costs many times a normal step.
 """
 import math
 import time
@@ -43,9 +49,10 @@ def batched_mix(nrep=256, seed=3):
     return out
 
 
-def served_jobs(njobs=64, seed=4, max_steps=100000, step_scale=1.0):
+def served_jobs(njobs=64, seed=4, max_steps=100000, step_scale=1.0, max_travel=0.24):
     """[(params, git, rel, steps)] of the served job stream; step_scale
-    multiplies every job's step count (at least 1 step)."""
+    multiplies every job's step count (at least 1 step); max_travel is every
+    job's travel cap (inf: fixed dt)."""
     rng = np.random.default_rng(seed)
     out = []
     for j in range(njobs):
@@ -58,7 +65,7 @@ def served_jobs(njobs=64, seed=4, max_steps=100000, step_scale=1.0):
         steps = max(1, int(round(steps * step_scale)))
         p = bdk.make_params(n1=n1, n2=N - n1, L=L, Z=mesh_for(L), scheme=int(rng.choice([bdk.NGP, bdk.CIC, bdk.TSC])),
                             lam=float(rng.uniform(5, 100)), kap=1.0, f2_f1=0.5, T=1.0, dt_max=1e-4,
-                            precision=bdk.FP32, seed=int(seed * 100003 + j))
+                            max_travel=max_travel, precision=bdk.FP32, seed=int(seed * 100003 + j))
         if rng.random() < 0.5:
             git, rel = bdk.init_uniform(p, int(seed * 7919 + j), 0.9)
         else:

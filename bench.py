@@ -292,7 +292,8 @@ def ensemble_figures(log_fn, cpu=False):
               "unit": "particle-steps/s", "wall_s": wall, "phases": ensemble.serve_phases(tl),
               "longest_job_particle_steps": max((p.n1 + p.n2) * st for p, _, _, st in jobs),
               "mix": "N log-uniform [1e3, 2.56e5], split U[0.2,0.8], density U[0.3,0.7], lambda U[5,100], "
-                     "NGP/CIC/TSC, steps log-uniform [1e3, 1e5], uniform or blob start; offered load 1.5x "
+                     "NGP/CIC/TSC, steps log-uniform [1e3, 1e5], uniform or blob start, travel cap 0.24 "
+                     "(the reference's adaptive dt, dt_max 1e-4); offered load 1.5x "
                      "the batched throughput; budget 64 GiB, at most 64 jobs at once, 256-step ticks"}
     return batched, served
 

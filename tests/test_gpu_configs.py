@@ -16,9 +16,11 @@
   CIC/TSC): 100 batched steps; every member's invariants, and 16 sampled
   members equal to their runs alone bit for bit.
 - Served stream (64 jobs of the served mix with step counts scaled by
-  1/100, plus three FP64 jobs, Poisson arrivals): every FP32 job equals its
-  run alone bit for bit; the FP64 jobs follow the oracle's trajectory (the
-  same Philox noise) with identical cells and positions within 1e-10.
+  1/100 and the reference's travel cap 0.24, i.e. each with its own
+  adaptive dt, plus three fixed-dt FP64 jobs, Poisson arrivals): every FP32
+  job equals its run alone bit for bit; the FP64 jobs follow the oracle's
+  trajectory (the same Philox noise) with identical cells and positions
+  within 1e-10.
 """
 import math
 import multiprocessing as mp
