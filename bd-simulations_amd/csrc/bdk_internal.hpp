@@ -98,8 +98,9 @@ struct DevBufs {
     float2 *lcomp = nullptr;  // pos_mode 2: the same in list order (written by k_fill)
     unsigned long long *exo = nullptr; // [64 x 16] max distance of a new position outside its previous cell (double bits)
     unsigned long long *dts = nullptr; // [64 x 16] adaptive dt: minima of the candidates by block (double bits;
-                                       // all ones = none), folded into sc->dt_bits by k_dt_fold
-    double *dtlog = nullptr;  // [kDtLog] adaptive dt of step s at s mod kDtLog (written by k_dt_fold)
+                                       // all ones = none), reduced by the step's update (by k_dt_fold for
+                                       // bdk_optimal_dt), re-armed at the end of the step
+    double *dtlog = nullptr;  // [kDtLog] adaptive dt of step s at s mod kDtLog (written by the update)
     int *list = nullptr;      // [N] slots grouped by cell (slot | species << 31)
     // the state in list order (k_fill): cell, rel and particle id of list entry q
     int *lcell = nullptr;
@@ -147,10 +148,9 @@ struct RepDev {
 };
 
 // Block maps of the batched stages: blockIdx.x -> (member, block of member).
-// BS_GATHER: particle blocks of the adaptive-dt members; BS_FOLD: one entry per adaptive member;
+// BS_GATHER: particle blocks of the adaptive-dt members;
 // BS_WCA: particle blocks of the WCA stage (its own block size).
-enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_GATHER, BS_FOLD, BS_WCA,
-                  BS_COUNT };
+enum BatchStage { BS_PART = 0, BS_CELL, BS_SCAN, BS_DEP, BS_R2C, BS_COL, BS_C2R, BS_GATHER, BS_WCA, BS_COUNT };
 
 // The choices of the batch autotuner (bdk_batch_tune); none changes a result.
 struct BatchTuning {
@@ -213,7 +213,7 @@ size_t deposit_lds_bytes(const bdk_t *h);          // LDS of one deposit block
 hipError_t launch_fft(bdk_t *h);                  // rho -> fmesh (r2c, multiply, c2r)
 // 0 r2c rows, 1 columns, 2 c2r rows (end_step: the c2r pass also ends the step)
 hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st = nullptr, int end_step = 0);
-hipError_t launch_gather(bdk_t *h, int with_dt);  // fkap (+ dt min-reduction)
+hipError_t launch_gather(bdk_t *h, int with_dt, int fold = 1);  // fkap (+ dt candidates; + their fold)
 hipError_t launch_update(bdk_t *h, int fused);    // noise, Euler step, refresh, counting
 hipError_t launch_bin(bdk_t *h);                  // counting after an upload/refresh
 hipError_t launch_own(bdk_t *h);                  // owned-mesh-index table of the deposit tiles

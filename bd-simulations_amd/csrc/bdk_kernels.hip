@@ -961,12 +961,16 @@ __global__ __launch_bounds__(DP_THREADS) void k_deposit(Geo g, DevBufs d, DepArg
 }
 
 // End of a step: step counter + 1, and the per-step scalars of the next
-// update re-armed (dt minimum, excursion slot, displacement bound).  Run by
-// threads 0..63 of one block of the step's last mesh pass (k_c2r_rows), which
-// follows every reader of these scalars in the step.
+// update re-armed (dt minimum and the 64 slots of dt candidates, excursion
+// slot, displacement bound).  Run by threads 0..63 of one block of the
+// step's last mesh pass (k_c2r_rows), which follows every reader of these
+// scalars in the step.
 __device__ __forceinline__ void step_end_body(const Geo &g, const DevBufs &d) {
     const int t = threadIdx.x;
-    if (t < 64) d.exo[t * 16] = 0ull;
+    if (t < 64) {
+        d.exo[t * 16] = 0ull;
+        d.dts[t * 16] = ~0ull;
+    }
     if (t == 0) {
         const unsigned long long s = d.sc->step;
         d.sc->dt_bits = bitsd(g.dt_max);
@@ -975,11 +979,15 @@ __device__ __forceinline__ void step_end_body(const Geo &g, const DevBufs &d) {
     }
 }
 
-// per-step scalar re-arm when no deposit runs (ideal-gas runs)
-__global__ void k_rearm(Geo g, DevScalars *sc) {
-    const unsigned long long step = sc->step;
-    sc->dt_bits = bitsd(g.dt_max);
-    sc->exc_bits[(step + 1) & 1] = 0ull;
+// per-step scalar re-arm when no deposit runs (ideal-gas runs), at the start
+// of the step; one wave
+__global__ void k_rearm(Geo g, DevBufs d) {
+    d.dts[threadIdx.x * 16] = ~0ull;
+    if (threadIdx.x == 0) {
+        const unsigned long long step = d.sc->step;
+        d.sc->dt_bits = bitsd(g.dt_max);
+        d.sc->exc_bits[(step + 1) & 1] = 0ull;
+    }
 }
 // ------------------------------------------------------------------ FFT in LDS
 // tw[k] = exp(-2 pi i k / Z), k < Z; the inverse transforms use the conjugates.
@@ -1644,7 +1652,8 @@ __global__ __launch_bounds__(256) void k_gather(Geo g, DevBufs d, int with_dt) {
     gather_body<R, SCHEME, P2>(g, d, with_dt, xcd_block(g.xcd), blockIdx.x);
 }
 
-// dt of the step: dt_max and the 64 slot minima of k_gather folded into
+// dt for bdk_optimal_dt (the step's update reduces the slots itself):
+// dt_max and the 64 slot minima of k_gather folded into
 // sc->dt_bits and logged by step; the slots are re-armed (all ones) for the
 // next reduction.  One wave.
 __device__ __forceinline__ void dt_fold_body(const Geo &g, const DevBufs &d) {
@@ -1682,7 +1691,7 @@ __device__ __forceinline__ double fmod_cell(double y, double B) {
 // (cell and offset) lies outside the particle's previous cell (<= 0 inside).
 // FUSED: capillary force gathered here (fixed dt); else read from fkap.
 template <typename R, int SCHEME, bool FUSED, bool P2>
-__device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int s) {
+__device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int s, double dt) {
     using R2 = typename V2<R>::T;
     const unsigned long long step = d.sc->step;
     const int c = d.lcell[s];
@@ -1704,7 +1713,6 @@ __device__ __forceinline__ double update_one(const Geo &g, const DevBufs &d, int
         }
     }
     const double2 n = particle_noise(g, d, pid, step);
-    const double dt = g.adaptive ? dbits(d.sc->dt_bits) : g.dt_max;
     const double sq = g.adaptive ? sqrt(2.0 * g.T * dt) : g.sq_fixed;  // same expression, correctly rounded
     const double dx = dt * (fwx + fkx) + sq * n.x;
     const double dy = dt * (fwy + fky) + sq * n.y;
@@ -1816,8 +1824,26 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     // list order), kept for the deposit beside the rebuild
     const int stride = ((g.N + blockDim.x - 1) / blockDim.x) * blockDim.x;
     for (int c = s; c <= g.ncell; c += stride) d.start_old[c] = d.start[c];
+    double dt = g.dt_max;
+    if (!FUSED && g.adaptive) {
+        // the step's dt (dm:1420-1491): dt_max and the 64 slot minima of the
+        // candidates from k_gather (all ones = none), reduced by every wave;
+        // the step's last mesh pass re-arms the slots
+        unsigned long long mm = d.dts[(threadIdx.x & 63) * 16];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const unsigned long long v = __shfl_xor(mm, o, 64);
+            if (v < mm) mm = v;
+        }
+        const unsigned long long dm = bitsd(g.dt_max);
+        dt = dbits(mm < dm ? mm : dm);
+        if (blk == 0 && threadIdx.x == 0) {
+            d.sc->dt_bits = bitsd(dt);
+            d.dtlog[d.sc->step & (kDtLog - 1)] = dt;
+        }
+    }
     double m = 0.0;
-    if (s < g.N) m = update_one<R, SCHEME, FUSED, P2>(g, d, s);
+    if (s < g.N) m = update_one<R, SCHEME, FUSED, P2>(g, d, s, dt);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0 && m > 0.0) {
@@ -2201,8 +2227,8 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int
 }
 
 // Members with a fixed dt gather the mesh force inside the update; members
-// with adaptive dt read the forces and the dt of kb_gather / kb_dt_fold, as
-// their single-simulation steps do.
+// with adaptive dt read the forces of kb_gather and reduce its dt
+// candidates, as their single-simulation steps do.
 template <typename R, bool P, bool P2>
 __global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *map, int nmap) {
     for_entries<P, false>(nmap, [&](int e_) {
@@ -2225,7 +2251,7 @@ __global__ __launch_bounds__(256) void kb_update(const RepDev *m, const int2 *ma
 }
 
 // capillary forces and dt candidates of the adaptive members (their
-// particle blocks only), then one wave per adaptive member folds its dt
+// particle blocks only)
 template <typename R, bool P, bool P2>
 __global__ __launch_bounds__(256) void kb_gather(const RepDev *m, const int2 *map, int nmap) {
     for_entries<P, true>(nmap, [&](int e_) {
@@ -2236,14 +2262,6 @@ __global__ __launch_bounds__(256) void kb_gather(const RepDev *m, const int2 *ma
         case 1: gather_body<R, 1, P2>(r.g, r.d, 1, t.y, e_); break;
         default: gather_body<R, 2, P2>(r.g, r.d, 1, t.y, e_); break;
         }
-    });
-}
-
-template <bool P>
-__global__ __launch_bounds__(64) void kb_dt_fold(const RepDev *m, const int2 *map, int nmap) {
-    for_entries<P, false>(nmap, [&](int e_) {
-        const int2 t = map[e_];
-        dt_fold_body(m[t.x].g, m[t.x].d);
     });
 }
 
@@ -2511,7 +2529,7 @@ hipError_t launch_deposit(bdk_t *h, int reset, int slot, hipStream_t st) {
 }
 
 hipError_t launch_rearm(bdk_t *h) {
-    hipLaunchKernelGGL(k_rearm, dim3(1), dim3(1), 0, h->stream, h->g, h->d.sc);
+    hipLaunchKernelGGL(k_rearm, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
     return hipGetLastError();
 }
 
@@ -2571,22 +2589,22 @@ hipError_t launch_fft(bdk_t *h) {
     return hipGetLastError();
 }
 
-template <typename R, int SCHEME> static void launch_gather_t(bdk_t *h, int with_dt) {
+template <typename R, int SCHEME> static void launch_gather_t(bdk_t *h, int with_dt, int fold) {
     if (zp2(h->g.Z))
         hipLaunchKernelGGL((k_gather<R, SCHEME, true>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d,
                            with_dt);
     else
         hipLaunchKernelGGL((k_gather<R, SCHEME, false>), dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g,
                            h->d, with_dt);
-    if (with_dt) hipLaunchKernelGGL(k_dt_fold, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
+    if (with_dt && fold) hipLaunchKernelGGL(k_dt_fold, dim3(1), dim3(64), 0, h->stream, h->g, h->d);
 }
 
-hipError_t launch_gather(bdk_t *h, int with_dt) {
+hipError_t launch_gather(bdk_t *h, int with_dt, int fold) {
     const bool f32 = h->p.precision != BDK_FP64;
     switch (h->g.scheme) {
-    case 0: f32 ? launch_gather_t<float, 0>(h, with_dt) : launch_gather_t<double, 0>(h, with_dt); break;
-    case 1: f32 ? launch_gather_t<float, 1>(h, with_dt) : launch_gather_t<double, 1>(h, with_dt); break;
-    default: f32 ? launch_gather_t<float, 2>(h, with_dt) : launch_gather_t<double, 2>(h, with_dt); break;
+    case 0: f32 ? launch_gather_t<float, 0>(h, with_dt, fold) : launch_gather_t<double, 0>(h, with_dt, fold); break;
+    case 1: f32 ? launch_gather_t<float, 1>(h, with_dt, fold) : launch_gather_t<double, 1>(h, with_dt, fold); break;
+    default: f32 ? launch_gather_t<float, 2>(h, with_dt, fold) : launch_gather_t<double, 2>(h, with_dt, fold); break;
     }
     return hipGetLastError();
 }
@@ -3243,7 +3261,7 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
     }
     if (info) {
         // capillary forces on their own (the step fuses them into the update)
-        if (kap && !h->g.adaptive) m.chk(launch_gather(h, 0));
+        if (kap && !h->g.adaptive) m.chk(launch_gather(h, 0, 0));
         const bool f32 = h->p.precision != BDK_FP64;
         if (f32) {
             hipLaunchKernelGGL(k_info_forces<float>, dim3(nblk(h->g.N, 256)), dim3(256), 0, h->stream, h->g, h->d, info);
@@ -3254,7 +3272,7 @@ hipError_t enqueue_step(bdk_t *h, std::vector<std::string> *names, std::vector<h
         }
     }
     if (h->g.adaptive) {
-        m.chk(launch_gather(h, 1));
+        m.chk(launch_gather(h, 1, 0));  // the update reduces the candidates itself
         m.mark("k_gather");
         m.chk(launch_update(h, 0));
     } else {
@@ -3348,7 +3366,7 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
         const int ad = h->g.adaptive ? 1 : 0;
         const int nb[BS_COUNT] = {(int)nblk(h->g.N, 256), (int)nblk(h->g.ncell, 256), r.scan_nb,
                                   (int)(pl.grid.x * pl.grid.y), Z / r.rb, (Z / 2) / r.rc, Z / r.rb2,
-                                  ad * (int)nblk(h->g.N, 256), ad, (int)nblk(h->g.N, tune.wca_block)};
+                                  ad * (int)nblk(h->g.N, 256), (int)nblk(h->g.N, tune.wca_block)};
         for (int k = 0; k < BS_COUNT; ++k)
             for (int b = 0; b < nb[k]; ++b) maps[k].push_back(make_int2(i, b));
         if (zp2(h->g.Z))
@@ -3426,13 +3444,12 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
             hipLaunchKernelGGL((kb_update<R, P, Q>), dim3(stage_grid(bp, nm)), dim3(256), 0, st, m, mp, nm);
         });
         break;
-    case 13:  // adaptive members: forces, dt candidates and the fold
+    case 13:  // adaptive members: forces and dt candidates (their updates reduce them)
         if (bp.nblk[BS_GATHER] > 0) {
             kinds(BS_GATHER, [&](auto q, const int2 *mp, int nm) {
                 constexpr bool Q = decltype(q)::value;
                 hipLaunchKernelGGL((kb_gather<R, P, Q>), dim3(stage_grid(bp, nm)), dim3(256), 0, st, m, mp, nm);
             });
-            hipLaunchKernelGGL(kb_dt_fold<P>, BDK_G(BS_FOLD), dim3(64), 0, st, m, bp.map[BS_FOLD], bp.nblk[BS_FOLD]);
         }
         break;
     case 6:
@@ -3457,7 +3474,7 @@ template <typename R> static void batch_stage_t(const BatchPlan &bp, int stage, 
 
 // stage 0 WCA, 1 deposit (list order), 2 r2c, 3 col, 4 c2r, 5 update,
 // 6 rank + scan, 8 deposit (slot arrays), 11 fill + sort, 12 c2r + step end,
-// 13 gather + dt fold of the adaptive members
+// 13 gather (forces and dt candidates) of the adaptive members
 hipError_t batch_stage(const BatchPlan &bp, int stage, hipStream_t st) {
     if (bp.f32)
         batch_stage_t<float>(bp, stage, st);
@@ -3484,7 +3501,7 @@ hipError_t batch_enqueue_forces(const BatchPlan &bp, hipStream_t st, hipStream_t
 }
 
 // One batched step, laid out like the single-simulation step: (the adaptive
-// members' gather and dt fold,) update, then
+// members' gather,) update, then
 // the rebuild and the WCA on st beside the slot-order deposit and the FFT
 // stages on aux (the last of which ends the step), joined.
 hipError_t batch_enqueue_step(const BatchPlan &bp, hipStream_t st, hipStream_t aux, hipEvent_t fork,
