@@ -1828,7 +1828,8 @@ __device__ __forceinline__ void update_body(const Geo &g, const DevBufs &d, int 
     if (!FUSED && g.adaptive) {
         // the step's dt (dm:1420-1491): dt_max and the 64 slot minima of the
         // candidates from k_gather (all ones = none), reduced by every wave;
-        // the step's last mesh pass re-arms the slots
+        // the step's last mesh pass re-arms the slots (k_rearm at the start
+        // of the step when no mesh pass runs)
         unsigned long long mm = d.dts[(threadIdx.x & 63) * 16];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
