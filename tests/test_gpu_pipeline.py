@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import bdk
-from common import params, positions
+from common import pair, params, positions
+from helpers import rel_rms
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +28,10 @@ CASES = [
     # keeps the sign, dm:216-325)
     (1500, 1000, 70.0, 128, bdk.TSC, bdk.FP32, 100.0, 2e-3, 10),
     (1500, 1000, 70.0, 128, bdk.CIC, bdk.FP64, 100.0, 2e-3, 10),
+    # a coarse mesh (tiles of 52 x 52 list cells): one tile stages about
+    # 1800 particles, more than the segment table holds (1536), and finds the
+    # segments by binary search; the others use the table
+    (2500, 2500, 100.0, 64, bdk.TSC, bdk.FP32, 1.0, 1e-4, 10),
 ]
 
 
@@ -49,3 +54,19 @@ def test_rebuild_side_deposit_equals_list_deposit(case):
     g, r = s.positions()
     assert np.all(np.isfinite(r))
     s.close()
+
+
+@pytest.mark.parametrize("scheme", [bdk.CIC, bdk.TSC])
+def test_segment_search_and_table_vs_oracle(scheme):
+    """The same coarse-mesh state (L = 100, Z = 64): the FP32 charge mesh and
+    the capillary forces, whose tiles find their staged particles' row
+    segments by binary search (the dense tile) or by the segment table,
+    against the FP64 oracle within 1e-5 relative RMS."""
+    print("case: coarse-mesh deposit vs oracle scheme=%d" % scheme, flush=True)
+    g, o, git, rel = pair(n1=2500, n2=2500, L=100.0, Z=64, scheme=scheme, precision=bdk.FP32, lam=20.0, seed=13)
+    fg = g.berechne_kapkraefte()
+    fo = o.kap()  # fills the oracle's meshes
+    assert rel_rms(g.mesh(bdk.MESH_RHO), o.mesh(0)) < 1e-5
+    assert rel_rms(fg, fo) < 1e-5
+    g.close()
+    o.close()
