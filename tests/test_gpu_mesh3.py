@@ -110,3 +110,42 @@ def test_factor3_graph_replay_and_batch():
     batch.close()
     for s in (a, b, c, d, e, e_alone):
         s.close()
+
+
+def test_fp64_batch_other_mesh_sizes():
+    """FP64 members on meshes of 96^2 (2^5 * 3) and 50^2 (2 * 5^2) beside a
+    power-of-two member, the 96^2 one with the reference's travel cap: 15
+    batched steps equal each member's steps alone bit for bit, and the
+    adaptive member's trajectory follows the oracle's (cells identical,
+    positions within 1e-10)."""
+    say("case: FP64 batch with meshes 96^2, 50^2 and 64^2")
+    kws = [dict(n1=600, n2=400, L=50.0, Z=96, scheme=bdk.TSC, precision=bdk.FP64, seed=51, max_travel=0.05,
+                dt_max=5e-3, T=2.0),
+           dict(n1=300, n2=300, L=30.0, Z=50, scheme=bdk.CIC, precision=bdk.FP64, seed=53),
+           dict(n1=500, n2=500, L=40.0, Z=64, scheme=bdk.NGP, precision=bdk.FP64, seed=55)]
+    alone, members, oracles = [], [], []
+    for kw in kws:
+        a, o, git, rel = pair(**kw)
+        m = bdk.RunDynamik(a.p)
+        m.set_positions(git, rel)
+        alone.append(a)
+        members.append(m)
+        oracles.append(o)
+    batch = bdk.Batch(members)
+    batch.run_steps(15)
+    batch.sync()
+    for a, m in zip(alone, members):
+        a.run_steps(15)
+        ga, ra = a.positions()
+        gm, rm = m.positions()
+        assert np.array_equal(ga, gm) and np.array_equal(ra, rm)
+    o = oracles[0]
+    for _ in range(15):
+        o.step(1.0)
+    og, orl = o.all_positions()
+    gm, rm = members[0].positions()
+    assert np.array_equal(gm, og)
+    assert np.abs(rm - orl).max() < 1e-10
+    batch.close()
+    for s in alone + members + oracles:
+        s.close()
