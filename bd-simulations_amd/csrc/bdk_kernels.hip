@@ -1034,6 +1034,39 @@ __device__ __forceinline__ void dft8(const C *v, C *y, bool inv) {
     }
 }
 
+// radix-16 butterfly: y[2m] = DFT8(v[t] + v[t+8])[m],
+// y[2m+1] = DFT8((v[t] - v[t+8]) W^t)[m], W = exp(-+2 pi i / 16)
+template <typename C>
+__device__ __forceinline__ void dft16(const C *v, C *y, bool inv) {
+    using R = decltype(C::x);
+    const R c1 = (R)0.92387953251128675613, s1 = (R)0.38268343236508977173;  // cos, sin of pi/8
+    const R h = (R)0.70710678118654752440;
+    // W^t = cos(t pi/8) -+ i sin(t pi/8), t = 1..7
+    const R wr[8] = {(R)1, c1, h, s1, (R)0, -s1, -h, -c1};
+    const R wi[8] = {(R)0, s1, h, c1, (R)1, c1, h, s1};
+    C a[8], b[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        a[t] = cadd(v[t], v[t + 8]);
+        b[t] = csub(v[t], v[t + 8]);
+    }
+    b[4] = inv ? cmul_i(b[4]) : cmul_mi(b[4]);
+#pragma unroll
+    for (int t = 1; t < 8; ++t) {
+        if (t == 4) continue;
+        const R si = inv ? wi[t] : -wi[t];
+        b[t] = cm<C>(b[t].x * wr[t] - b[t].y * si, b[t].x * si + b[t].y * wr[t]);
+    }
+    C ya[8], yb[8];
+    dft8(a, ya, inv);
+    dft8(b, yb, inv);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        y[2 * m] = ya[m];
+        y[2 * m + 1] = yb[m];
+    }
+}
+
 // LDS layout of the FFT rows: element i of a row sits at lp(i), one padding
 // element after every 8.  The output strides of the first Stockham passes
 // (8 and 64 elements) then spread over the banks instead of piling onto a
@@ -1055,8 +1088,10 @@ template <typename R, int EPT, int RAD, bool SMALL>
 __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int nrows, int stride, int logNs,
                                          const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     using C = typename V2<R>::T;
-    constexpr int LR = RAD == 8 ? 3 : (RAD == 4 ? 2 : 1);
-    constexpr int NB = EPT / RAD;
+    constexpr int LR = RAD == 16 ? 4 : (RAD == 8 ? 3 : (RAD == 4 ? 2 : 1));
+    // butterflies per thread (radix 16 at 8 elements per thread: one, on half
+    // of the threads)
+    constexpr int NB = EPT >= RAD ? EPT / RAD : 1;
     constexpr bool PRE = sizeof(R) == 4 && NB * (RAD - 1) <= 16;  // twiddles fetched before the barrier
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lognb = logn - LR, nb = 1 << lognb;
@@ -1066,22 +1101,28 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
     const int nbp = nb + (nb >> 3), Nsp = Ns + (Ns >> 3);
     C v[NB][RAD];
     C w[PRE ? NB : 1][RAD - 1];
-    // twiddles of butterfly k: tw^1, tw^2, tw^4 from the table, the other
-    // powers as products (one to two roundings deeper; 3 loads instead of 7
-    // for radix 8).  Every tiling forms them the same way -- before the
-    // barrier when the registers allow (PRE), else after it -- so the
-    // transform's rounding does not depend on the elements per thread.
+    // twiddles of butterfly k: tw^1, tw^2, tw^4 (and tw^8) from the table,
+    // the other powers as products (one to three roundings deeper; 3 loads
+    // instead of 7 for radix 8, 4 instead of 15 for radix 16).  Every tiling
+    // forms them the same way -- before the barrier when the registers allow
+    // (PRE), else after it -- so the transform's rounding does not depend on
+    // the elements per thread.
     auto twiddles = [&](int k, C *ww) {
         ww[0] = tw[k << twsh];
         if (RAD >= 4) {
             ww[1] = tw[(2 * k) << twsh];
             ww[2] = cmul(ww[0], ww[1]);
         }
-        if (RAD == 8) {
+        if (RAD >= 8) {
             ww[3] = tw[(4 * k) << twsh];
             ww[4] = cmul(ww[3], ww[0]);
             ww[5] = cmul(ww[3], ww[1]);
             ww[6] = cmul(ww[3], ww[2]);
+        }
+        if (RAD == 16) {
+            ww[7] = tw[(8 * k) << twsh];
+#pragma unroll
+            for (int t = 8; t < 15; ++t) ww[t] = cmul(ww[7], ww[t - 8]);
         }
     };
 #pragma unroll
@@ -1124,12 +1165,23 @@ __device__ __forceinline__ void fft_pass(typename V2<R>::T *buf, int logn, int n
 #pragma unroll
                 for (int t = 0; t < 4; ++t) y[t] = v[e][t];
                 dft4(y, inv);
-            } else {
+            } else if (RAD == 8) {
                 dft8(v[e], y, inv);
+            } else {
+                dft16(v[e], y, inv);
             }
-            C *q = buf + row * stride + lp(((j - k) << LR) + k);
+            if (RAD == 16 && Ns < 8) {
+                // the first radix-16 pass (Ns = 1): outputs 16 j + t straddle a
+                // padding element, so each takes its own padded index
+                C *q = buf + row * stride;
+                const int base = (j - k) << LR;
 #pragma unroll
-            for (int t = 0; t < RAD; ++t) q[t * Nsp] = y[t];
+                for (int t = 0; t < RAD; ++t) q[lp(base + k + t * Ns)] = y[t];
+            } else {
+                C *q = buf + row * stride + lp(((j - k) << LR) + k);
+#pragma unroll
+                for (int t = 0; t < RAD; ++t) q[t * Nsp] = y[t];
+            }
         }
     }
     __syncthreads();
@@ -1280,8 +1332,10 @@ __device__ __forceinline__ void lds_fft_g(typename V2<R>::T *buf, int n, int nro
 }
 
 // rows of length n: the power-of-two passes below, or lds_fft_g when n has an
-// odd factor (tw holds exp(-2 pi i k / Z), k < Z; n divides Z)
-template <typename R, int EPT>
+// odd factor (tw holds exp(-2 pi i k / Z), k < Z; n divides Z).  R16: the
+// kernel's power-of-two rows take radix-16 passes (FP32); a kernel's choice
+// does not depend on its elements per thread, so neither does its rounding.
+template <typename R, int EPT, bool R16 = false>
 __device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
                                              const typename V2<R>::T *__restrict__ tw, int logZ, bool inv);
 template <typename R, int EPT>
@@ -1293,20 +1347,40 @@ __device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int n, int nrows
         lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
 }
 // the same with the mesh kind known at compile time (P2: n and Z powers of two)
-template <typename R, int EPT, bool P2>
+template <typename R, int EPT, bool P2, bool R16 = false>
 __device__ __forceinline__ void lds_fft_t(typename V2<R>::T *buf, int n, int nrows, int stride,
                                           const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
     if (P2)
-        lds_fft_pow2<R, EPT>(buf, __ffs(n) - 1, nrows, stride, tw, __ffs(Z) - 1, inv);
+        lds_fft_pow2<R, EPT, R16>(buf, __ffs(n) - 1, nrows, stride, tw, __ffs(Z) - 1, inv);
     else
         lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
 }
 
-// radix-8 passes, then one radix-4 or radix-2 pass for the remaining bits
-template <typename R, int EPT>
+// radix-8 passes (radix-16 passes with R16), then one pass of a smaller radix
+// for the remaining bits
+template <typename R, int EPT, bool R16>
 __device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
                                              const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     int logNs = 0;
+    if constexpr (R16 && sizeof(R) == 4) {
+        // FP32 rows of at least 128: radix-16 passes, then one pass of radix
+        // 8, 4 or 2 for the remaining bits (2048 points in 3 passes instead
+        // of 4)
+        if (logn >= 7) {
+            while (logn - logNs >= 4) {
+                fft_pass<R, EPT, 16, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+                logNs += 4;
+            }
+            const int rem = logn - logNs;
+            if (rem == 3)
+                fft_pass<R, EPT, 8, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+            else if (rem == 2)
+                fft_pass<R, EPT, 4, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+            else if (rem == 1)
+                fft_pass<R, EPT, 2, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+            return;
+        }
+    }
     const bool small8 = logn - 3 < 3;  // nb = n/8 below 8
     while (logn - logNs >= 3) {
         if (small8)
@@ -1417,7 +1491,7 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
         s[row * stride + lp(k)] = ht[(size_t)(m0 + row) * Z + k];
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2>(s, Z, RC, stride, tw, Z, false);
+    lds_fft_t<R, EPT, P2, true>(s, Z, RC, stride, tw, Z, false);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = dz.q(idx), j = dz.r(idx);
         const int m = m0 + row;
@@ -1454,7 +1528,7 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
         }
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2>(s, Z, 2 * RC, stride, tw, Z, true);
+    lds_fft_t<R, EPT, P2, true>(s, Z, 2 * RC, stride, tw, Z, true);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = dz.q(idx), k = dz.r(idx);
         const int so = row * stride + lp(k);
@@ -1505,7 +1579,7 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
         }
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2>(s, Z, RB2, stride, tw, Z, true);
+    lds_fft_t<R, EPT, P2, true>(s, Z, RB2, stride, tw, Z, true);
     for (int idx = tid; idx < RB2 * Z; idx += nt) {
         const int row = dz.q(idx), l = dz.r(idx);
         fm[(size_t)(k0 + row) * Z + l] = s[row * stride + lp(l)];
