@@ -167,6 +167,7 @@ struct BatchPlan {
     int2 *map[BS_COUNT] = {};        // device maps
     int nblk[BS_COUNT] = {};
     int nblk2[BS_COUNT] = {};        // leading entries of members with a power-of-two mesh
+    int nblk16[BS_COUNT] = {};       // of those, the leading entries whose column and c2r passes are radix 16
     BatchTuning tune;
     int ept[3] = {8, 8, 8};          // the FFT elements per thread in use (at least the members' minimum)
     size_t dep_lds = 0, l1 = 0, lc = 0, l2 = 0;

@@ -1473,7 +1473,7 @@ __device__ __forceinline__ void symbols(int j, int mm, const Geo &g, const R *sq
 
 // Column pass: rows m0..m0+RC-1 of ht (length Z along x), forward FFT,
 // multiply by i*gx and i*gy, inverse FFT of both, out gt[m*Z+k] = (Fx, Fy).
-template <typename R, int EPT, bool P2>
+template <typename R, int EPT, bool P2, bool R16 = false>
 __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ ht, typename V4<R>::T *__restrict__ gt,
                                          const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
                                          const R *__restrict__ lap, const Geo &g, int RC, int stride, int blk) {
@@ -1491,7 +1491,7 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
         s[row * stride + lp(k)] = ht[(size_t)(m0 + row) * Z + k];
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2, true>(s, Z, RC, stride, tw, Z, false);
+    lds_fft_t<R, EPT, P2, R16>(s, Z, RC, stride, tw, Z, false);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = dz.q(idx), j = dz.r(idx);
         const int m = m0 + row;
@@ -1528,7 +1528,7 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
         }
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2, true>(s, Z, 2 * RC, stride, tw, Z, true);
+    lds_fft_t<R, EPT, P2, R16>(s, Z, 2 * RC, stride, tw, Z, true);
     for (int idx = tid; idx < RC * Z; idx += nt) {
         const int row = dz.q(idx), k = dz.r(idx);
         const int so = row * stride + lp(k);
@@ -1542,18 +1542,18 @@ __device__ __forceinline__ void col_body(const typename V2<R>::T *__restrict__ h
     }
 }
 
-template <typename R, int EPT, bool P2>
+template <typename R, int EPT, bool P2, bool R16 = false>
 __global__ __launch_bounds__(FFT_THREADS) void k_col(const typename V2<R>::T *__restrict__ ht,
                                                      typename V4<R>::T *__restrict__ gt,
                                                      const typename V2<R>::T *__restrict__ tw, const R *__restrict__ sq,
                                                      const R *__restrict__ lap, Geo g, int RC, int stride) {
-    col_body<R, EPT, P2>(ht, gt, tw, sq, lap, g, RC, stride, blockIdx.x);
+    col_body<R, EPT, P2, R16>(ht, gt, tw, sq, lap, g, RC, stride, blockIdx.x);
 }
 
 // Row c2r of the two real signals Fx, Fy: rows k0..k0+RB2-1, spectrum
 // C[m] = X[m] + i Y[m] completed by Hermitian symmetry, inverse FFT of
 // length Z, out fmesh[k*Z + l] = (Re, Im) = (Fx, Fy).
-template <typename R, int EPT, bool P2>
+template <typename R, int EPT, bool P2, bool R16 = false>
 __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restrict__ gt,
                                               typename V2<R>::T *__restrict__ fm,
                                               const typename V2<R>::T *__restrict__ tw, int Z, int RB2,
@@ -1579,21 +1579,21 @@ __device__ __forceinline__ void c2r_rows_body(const typename V4<R>::T *__restric
         }
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2, true>(s, Z, RB2, stride, tw, Z, true);
+    lds_fft_t<R, EPT, P2, R16>(s, Z, RB2, stride, tw, Z, true);
     for (int idx = tid; idx < RB2 * Z; idx += nt) {
         const int row = dz.q(idx), l = dz.r(idx);
         fm[(size_t)(k0 + row) * Z + l] = s[row * stride + lp(l)];
     }
 }
 
-template <typename R, int EPT, bool P2>
+template <typename R, int EPT, bool P2, bool R16 = false>
 __global__ __launch_bounds__(FFT_THREADS) void k_c2r_rows(const typename V4<R>::T *__restrict__ gt,
                                                           typename V2<R>::T *__restrict__ fm,
                                                           const typename V2<R>::T *__restrict__ tw, int Z,
                                                           int RB2, int stride, int remap, Geo g, DevBufs d,
                                                           int end_step) {
     if (end_step && blockIdx.x == 0) step_end_body(g, d);
-    c2r_rows_body<R, EPT, P2>(gt, fm, tw, Z, RB2, stride, xcd_block(remap));
+    c2r_rows_body<R, EPT, P2, R16>(gt, fm, tw, Z, RB2, stride, xcd_block(remap));
 }
 
 // ------------------------------------------------------------------ gather, update
@@ -2275,7 +2275,7 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int
     });
 }
 
-template <typename R, int EPT, bool P, bool P2>
+template <typename R, int EPT, bool P, bool P2, bool R16 = false>
 __global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int2 *map, int nmap) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
@@ -2284,11 +2284,11 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_col(const RepDev *m, const int
         const RepDev &r = m[t.x];
         const R *sq = std::is_same<R, float>::value ? (const R *)r.d.sqf : (const R *)r.d.sq;
         const R *lap = std::is_same<R, float>::value ? (const R *)r.d.lapf : (const R *)r.d.lap;
-        col_body<R, EPT, P2>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
+        col_body<R, EPT, P2, R16>((const C *)r.d.ht, (C4 *)r.d.gt, (const C *)r.d.tw, sq, lap, r.g, r.rc, r.sC, t.y);
     });
 }
 
-template <typename R, int EPT, bool P, bool P2>
+template <typename R, int EPT, bool P, bool P2, bool R16 = false>
 __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int2 *map, int nmap, int end_step) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
@@ -2296,8 +2296,8 @@ __global__ __launch_bounds__(FFT_THREADS) void kb_c2r(const RepDev *m, const int
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
         if (end_step && t.y == 0) step_end_body(r.g, r.d);
-        c2r_rows_body<R, EPT, P2>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.rb2, r.sR2,
-                                  t.y);
+        c2r_rows_body<R, EPT, P2, R16>((const C4 *)r.d.gt, (C *)r.d.fmesh, (const C *)r.d.tw, r.g.Z, r.rb2,
+                                       r.sR2, t.y);
     });
 }
 
@@ -2466,6 +2466,18 @@ hipError_t set_kernel_attributes() {
     BDK_ATTR_E(16, false)
     BDK_ATTR_E(32, false)
 #undef BDK_ATTR_E
+    // the radix-16 column and c2r passes (FP32, power-of-two meshes)
+#define BDK_ATTR_16(E)                         \
+    BDK_ATTR((k_col<float, E, true, true>))      \
+    BDK_ATTR((k_c2r_rows<float, E, true, true>)) \
+    BDK_ATTR((kb_col<float, E, false, true, true>)) \
+    BDK_ATTR((kb_col<float, E, true, true, true>))  \
+    BDK_ATTR((kb_c2r<float, E, false, true, true>)) \
+    BDK_ATTR((kb_c2r<float, E, true, true, true>))
+    BDK_ATTR_16(8)
+    BDK_ATTR_16(16)
+    BDK_ATTR_16(32)
+#undef BDK_ATTR_16
 #define BDK_ATTR_B(E, P, Q)            \
     BDK_ATTR((kb_r2c<float, E, P, Q>))  \
     BDK_ATTR((kb_r2c<double, E, P, Q>)) \
@@ -2583,6 +2595,14 @@ size_t deposit_lds_bytes(const bdk_t *h) { return dep_plan(h, 0).lds; }
 
 static bool zp2(int Z) { return (Z & (Z - 1)) == 0; }
 
+// FP32 power-of-two meshes of at least FFT_R16_MIN points per side run the
+// column and c2r passes in radix-16 kernels (fewer LDS passes); smaller
+// meshes keep the radix-8 kernels, whose registers the radix-16 code would
+// raise.  The choice depends on the mesh only, so a simulation takes the same
+// kernels alone and in a batch.
+constexpr int FFT_R16_MIN = 1024;
+static bool fft_r16(int Z) { return zp2(Z) && Z >= FFT_R16_MIN; }
+
 template <typename R, int SCHEME> static void launch_deposit_t(bdk_t *h, const DepPlan &pl, hipStream_t st) {
     if (zp2(h->g.Z))
         hipLaunchKernelGGL((k_deposit<R, SCHEME, true>), pl.grid, dim3(DP_THREADS), pl.lds, st, h->g, h->d, pl.a);
@@ -2615,7 +2635,7 @@ hipError_t launch_rearm(bdk_t *h) {
     default: CALL(32); break;   \
     }
 
-template <typename R, bool Q> static void launch_fft_tq(bdk_t *h, int only, hipStream_t st, int end_step) {
+template <typename R, bool Q, bool R16> static void launch_fft_tq(bdk_t *h, int only, hipStream_t st, int end_step) {
     using C = typename V2<R>::T;
     using C4 = typename V4<R>::T;
     const int Z = h->g.Z;
@@ -2626,10 +2646,10 @@ template <typename R, bool Q> static void launch_fft_tq(bdk_t *h, int only, hipS
     hipLaunchKernelGGL((k_r2c_rows<R, E, Q>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,          \
                        (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, c.RB, c.sR1, h->g.xcd)
 #define COL(E)                                                                                              \
-    hipLaunchKernelGGL((k_col<R, E, Q>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,         \
+    hipLaunchKernelGGL((k_col<R, E, Q, R16>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,    \
                        (const C *)h->d.ht, (C4 *)h->d.gt, (const C *)h->d.tw, sq, lap, h->g, c.RC, c.sC)
 #define C2R(E)                                                                                              \
-    hipLaunchKernelGGL((k_c2r_rows<R, E, Q>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,         \
+    hipLaunchKernelGGL((k_c2r_rows<R, E, Q, R16>), dim3(Z / c.RB2), dim3(FFT_THREADS), c.l2, st,    \
                        (const C4 *)h->d.gt, (C *)h->d.fmesh, (const C *)h->d.tw, Z, c.RB2, c.sR2, h->g.xcd, h->g, h->d,  \
                        end_step)
     if (only < 0 || only == 0) BDK_EPT_SWITCH(c.e1, R2C)
@@ -2642,10 +2662,16 @@ template <typename R, bool Q> static void launch_fft_tq(bdk_t *h, int only, hipS
 
 template <typename R> static void launch_fft_t(bdk_t *h, int only = -1, hipStream_t st = nullptr, int end_step = 0) {
     if (!st) st = h->stream;
+    if constexpr (std::is_same<R, float>::value) {
+        if (fft_r16(h->g.Z)) {
+            launch_fft_tq<R, true, true>(h, only, st, end_step);
+            return;
+        }
+    }
     if (zp2(h->g.Z))
-        launch_fft_tq<R, true>(h, only, st, end_step);
+        launch_fft_tq<R, true, false>(h, only, st, end_step);
     else
-        launch_fft_tq<R, false>(h, only, st, end_step);
+        launch_fft_tq<R, false, false>(h, only, st, end_step);
 }
 
 hipError_t launch_fft_pass(bdk_t *h, int pass, hipStream_t st, int end_step) {
@@ -3402,6 +3428,7 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
     for (int k = 0; k < BS_COUNT; ++k) {
         maps[k].clear();
         bp.nblk2[k] = 0;
+        bp.nblk16[k] = 0;
     }
     bp.nmem = n;
     bp.f32 = mem.empty() ? 1 : (mem[0]->p.precision != BDK_FP64);
@@ -3412,12 +3439,15 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
     const int *ept = bp.ept;
     bp.dep_lds = bp.l1 = bp.lc = bp.l2 = 0;
     const size_t csz = bp.f32 ? 8 : 16;
-    // entries of the members with a power-of-two mesh first: the mesh stages
-    // launch once per mesh kind present (the kind is a template parameter)
+    // entries of the members with a power-of-two mesh first, those with
+    // radix-16 column and c2r passes (FP32) before the others: the mesh
+    // stages launch once per mesh kind present (the kind is a template
+    // parameter)
+    auto cls = [&](const bdk_t *h) { return bp.f32 && fft_r16(h->g.Z) ? 0 : (zp2(h->g.Z) ? 1 : 2); };
     std::vector<int> order;
-    for (int pass = 0; pass < 2; ++pass)
+    for (int pass = 0; pass < 3; ++pass)
         for (int i = 0; i < n; ++i)
-            if (zp2(mem[i]->g.Z) == (pass == 0)) order.push_back(i);
+            if (cls(mem[i]) == pass) order.push_back(i);
     for (int oi = 0; oi < n; ++oi) {
         const int i = order[oi];
         const bdk_t *h = mem[i];
@@ -3446,6 +3476,8 @@ hipError_t batch_build(const std::vector<bdk_t *> &mem, const BatchTuning &tune,
             for (int b = 0; b < nb[k]; ++b) maps[k].push_back(make_int2(i, b));
         if (zp2(h->g.Z))
             for (int k = 0; k < BS_COUNT; ++k) bp.nblk2[k] = (int)maps[k].size();
+        if (cls(h) == 0)
+            for (int k = 0; k < BS_COUNT; ++k) bp.nblk16[k] = (int)maps[k].size();
     }
     for (int k = 0; k < BS_COUNT; ++k) bp.nblk[k] = (int)maps[k].size();
     return hipSuccess;
@@ -3472,6 +3504,14 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
         if (n2 > 0) f(std::true_type(), (const int2 *)bp.map[S], n2);
         if (nall > n2) f(std::false_type(), (const int2 *)bp.map[S] + n2, nall - n2);
     };
+    // the same for the column and c2r passes, with the radix-16 members
+    // (leading) apart: f(kind, radix 16, map, entries)
+    auto kinds16 = [&](int S, auto &&f) {
+        const int n16 = std::is_same<R, float>::value ? bp.nblk16[S] : 0, n2 = bp.nblk2[S], nall = bp.nblk[S];
+        if (n16 > 0) f(std::true_type(), std::true_type(), (const int2 *)bp.map[S], n16);
+        if (n2 > n16) f(std::true_type(), std::false_type(), (const int2 *)bp.map[S] + n16, n2 - n16);
+        if (nall > n2) f(std::false_type(), std::false_type(), (const int2 *)bp.map[S] + n2, nall - n2);
+    };
 #define BDK_G(S) dim3(stage_grid(bp, bp.nblk[S]))
     switch (stage) {
     case 0:
@@ -3495,20 +3535,24 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
         });
         break;
     case 3:
-        kinds(BS_COL, [&](auto q, const int2 *mp, int nm) {
+        kinds16(BS_COL, [&](auto q, auto r16, const int2 *mp, int nm) {
             constexpr bool Q = decltype(q)::value;
-#define COL(E) hipLaunchKernelGGL((kb_col<R, E, P, Q>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.lc, st, m, mp, nm)
+            constexpr bool R16 = decltype(r16)::value && std::is_same<R, float>::value;
+#define COL(E)                                                                                                 \
+    hipLaunchKernelGGL((kb_col<R, E, P, Q, R16>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.lc, st, m, mp, \
+                       nm)
             BDK_EPT_SWITCH2(bp.ept[1], COL)
 #undef COL
         });
         break;
     case 4:
     case 12:  // 12: also the end of the step
-        kinds(BS_C2R, [&](auto q, const int2 *mp, int nm) {
+        kinds16(BS_C2R, [&](auto q, auto r16, const int2 *mp, int nm) {
             constexpr bool Q = decltype(q)::value;
+            constexpr bool R16 = decltype(r16)::value && std::is_same<R, float>::value;
 #define C2R(E)                                                                                                  \
-    hipLaunchKernelGGL((kb_c2r<R, E, P, Q>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.l2, st, m, mp, nm, \
-                       stage == 12 ? 1 : 0)
+    hipLaunchKernelGGL((kb_c2r<R, E, P, Q, R16>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.l2, st, m, mp, \
+                       nm, stage == 12 ? 1 : 0)
             BDK_EPT_SWITCH2(bp.ept[2], C2R)
 #undef C2R
         });
