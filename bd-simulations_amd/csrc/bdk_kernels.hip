@@ -19,13 +19,15 @@
 //   k_r2c_rows  row FFTs of the real mesh, transposed half-spectrum out;
 //   k_col       column FFT, Green's function x gradient (dm:1210-1216), two
 //               inverse column FFTs (Fx, Fy);
-//   k_c2r_rows  inverse row FFT of (Fx + i Fy) -> real Fx, Fy mesh.
+//   k_c2r_rows  inverse row FFT of (Fx + i Fy) -> real Fx, Fy mesh
+//               (Stockham passes in LDS: radix 8, radix 16 for the column
+//               and c2r passes of FP32 meshes of 1024 points and more).
 // Every force, gather and update kernel runs over list positions q and
 // reads the list-ordered state; k_update writes the new state to slot q.
 // The slots are therefore re-sorted by cell every step: consecutive threads
 // handle particles of neighbouring cells for the whole run.
-// With a finite max_travel, k_gather (forces + dt min-reduction, dm:1420-1491)
-// runs before a non-fused k_update.
+// With a finite max_travel, k_gather (forces and the dt candidates,
+// dm:1420-1491) runs before a non-fused k_update, which reduces them.
 //
 // Position arithmetic (x = git*B + rel, mesh indices, weights, refresh) is
 // compiled without FP contraction so it rounds like the reference's x86-64
