@@ -21,7 +21,8 @@
 //               inverse column FFTs (Fx, Fy);
 //   k_c2r_rows  inverse row FFT of (Fx + i Fy) -> real Fx, Fy mesh
 //               (Stockham passes in LDS: radix 8, radix 16 for the column
-//               and c2r passes of FP32 meshes of 1024 points and more).
+//               and c2r passes of FP32 meshes of 1024 points and more, whose
+//               r2c rows end in one radix-16 pass instead of radix 8 and 2).
 // Every force, gather and update kernel runs over list positions q and
 // reads the list-ordered state; k_update writes the new state to slot q.
 // The slots are therefore re-sorted by cell every step: consecutive threads
@@ -1337,7 +1338,7 @@ __device__ __forceinline__ void lds_fft_g(typename V2<R>::T *buf, int n, int nro
 // odd factor (tw holds exp(-2 pi i k / Z), k < Z; n divides Z).  R16: the
 // kernel's power-of-two rows take radix-16 passes (FP32); a kernel's choice
 // does not depend on its elements per thread, so neither does its rounding.
-template <typename R, int EPT, bool R16 = false>
+template <typename R, int EPT, bool R16 = false, bool R16T = false>
 __device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
                                              const typename V2<R>::T *__restrict__ tw, int logZ, bool inv);
 template <typename R, int EPT>
@@ -1349,21 +1350,34 @@ __device__ __forceinline__ void lds_fft(typename V2<R>::T *buf, int n, int nrows
         lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
 }
 // the same with the mesh kind known at compile time (P2: n and Z powers of two)
-template <typename R, int EPT, bool P2, bool R16 = false>
+template <typename R, int EPT, bool P2, bool R16 = false, bool R16T = false>
 __device__ __forceinline__ void lds_fft_t(typename V2<R>::T *buf, int n, int nrows, int stride,
                                           const typename V2<R>::T *__restrict__ tw, int Z, bool inv) {
     if (P2)
-        lds_fft_pow2<R, EPT, R16>(buf, __ffs(n) - 1, nrows, stride, tw, __ffs(Z) - 1, inv);
+        lds_fft_pow2<R, EPT, R16, R16T>(buf, __ffs(n) - 1, nrows, stride, tw, __ffs(Z) - 1, inv);
     else
         lds_fft_g<R, EPT>(buf, n, nrows, stride, tw, Z, inv);
 }
 
 // radix-8 passes (radix-16 passes with R16), then one pass of a smaller radix
 // for the remaining bits
-template <typename R, int EPT, bool R16>
+template <typename R, int EPT, bool R16, bool R16T>
 __device__ __forceinline__ void lds_fft_pow2(typename V2<R>::T *buf, int logn, int nrows, int stride,
                                              const typename V2<R>::T *__restrict__ tw, int logZ, bool inv) {
     int logNs = 0;
+    if constexpr (R16T && sizeof(R) == 4) {
+        // radix-8 passes, with the last 4 bits (when radix 8 would leave one
+        // bit for a radix-2 pass) in one radix-16 pass: 1024 points in passes
+        // of 8, 8 and 16
+        if (logn >= 7 && logn % 3 == 1) {
+            while (logn - logNs > 4) {
+                fft_pass<R, EPT, 8, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+                logNs += 3;
+            }
+            fft_pass<R, EPT, 16, false>(buf, logn, nrows, stride, logNs, tw, logZ, inv);
+            return;
+        }
+    }
     if constexpr (R16 && sizeof(R) == 4) {
         // FP32 rows of at least 128: radix-16 passes, then one pass of radix
         // 8, 4 or 2 for the remaining bits (2048 points in 3 passes instead
@@ -1408,8 +1422,9 @@ constexpr int FFT_THREADS = 256;
 
 // Row r2c: rows k0..k0+RB-1 of the real mesh -> half spectrum X[m], m<Z/2,
 // with X[0] packing (DC, Nyquist) (both real), written transposed:
-// ht[m*Z + k].  Length-Z/2 complex FFT of (x[2n], x[2n+1]) + split.
-template <typename R, int EPT, bool P2>
+// ht[m*Z + k].  Length-Z/2 complex FFT of (x[2n], x[2n+1]) + split; with
+// R16 (FP32) a row length of 2^(3m+1) ends in one radix-16 pass.
+template <typename R, int EPT, bool P2, bool R16 = false>
 __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
                                               const typename V2<R>::T *__restrict__ tw, int Z, int RB,
                                               int stride, int blk) {
@@ -1425,7 +1440,7 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
         s[row * stride + lp(m)] = ((const C *)(rho + (size_t)(k0 + row) * Z))[m];
     }
     __syncthreads();
-    lds_fft_t<R, EPT, P2>(s, n, RB, stride, tw, Z, false);
+    lds_fft_t<R, EPT, P2, false, R16>(s, n, RB, stride, tw, Z, false);
     const int half = n >> 1;
     for (int idx = tid; idx < RB * (half + 1); idx += nt) {
         const int row = idx / (half + 1), m = idx - row * (half + 1);
@@ -1449,11 +1464,11 @@ __device__ __forceinline__ void r2c_rows_body(const R *__restrict__ rho, typenam
     }
 }
 
-template <typename R, int EPT, bool P2>
+template <typename R, int EPT, bool P2, bool R16 = false>
 __global__ __launch_bounds__(FFT_THREADS) void k_r2c_rows(const R *__restrict__ rho, typename V2<R>::T *__restrict__ ht,
                                                           const typename V2<R>::T *__restrict__ tw, int Z,
                                                           int RB, int stride, int remap) {
-    r2c_rows_body<R, EPT, P2>(rho, ht, tw, Z, RB, stride, xcd_block(remap));
+    r2c_rows_body<R, EPT, P2, R16>(rho, ht, tw, Z, RB, stride, xcd_block(remap));
 }
 
 // spectral factors of one (j, m') pair: gx = sq[j] G / Z / Z, gy = sq[m'] G / Z / Z,
@@ -2267,13 +2282,13 @@ __global__ __launch_bounds__(DP_THREADS) void kb_deposit(const RepDev *m, const 
     });
 }
 
-template <typename R, int EPT, bool P, bool P2>
+template <typename R, int EPT, bool P, bool P2, bool R16 = false>
 __global__ __launch_bounds__(FFT_THREADS) void kb_r2c(const RepDev *m, const int2 *map, int nmap) {
     using C = typename V2<R>::T;
     for_entries<P, true>(nmap, [&](int e_) {
         const int2 t = map[e_];
         const RepDev &r = m[t.x];
-        r2c_rows_body<R, EPT, P2>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.rb, r.sR1, t.y);
+        r2c_rows_body<R, EPT, P2, R16>((const R *)r.d.rho, (C *)r.d.ht, (const C *)r.d.tw, r.g.Z, r.rb, r.sR1, t.y);
     });
 }
 
@@ -2470,6 +2485,9 @@ hipError_t set_kernel_attributes() {
 #undef BDK_ATTR_E
     // the radix-16 column and c2r passes (FP32, power-of-two meshes)
 #define BDK_ATTR_16(E)                         \
+    BDK_ATTR((k_r2c_rows<float, E, true, true>)) \
+    BDK_ATTR((kb_r2c<float, E, false, true, true>)) \
+    BDK_ATTR((kb_r2c<float, E, true, true, true>))  \
     BDK_ATTR((k_col<float, E, true, true>))      \
     BDK_ATTR((k_c2r_rows<float, E, true, true>)) \
     BDK_ATTR((kb_col<float, E, false, true, true>)) \
@@ -2598,7 +2616,8 @@ size_t deposit_lds_bytes(const bdk_t *h) { return dep_plan(h, 0).lds; }
 static bool zp2(int Z) { return (Z & (Z - 1)) == 0; }
 
 // FP32 power-of-two meshes of at least FFT_R16_MIN points per side run the
-// column and c2r passes in radix-16 kernels (fewer LDS passes); smaller
+// column and c2r passes in radix-16 kernels and the r2c rows with a radix-16
+// last pass where radix 8 would leave one bit (fewer LDS passes); smaller
 // meshes keep the radix-8 kernels, whose registers the radix-16 code would
 // raise.  The choice depends on the mesh only, so a simulation takes the same
 // kernels alone and in a batch.
@@ -2645,7 +2664,7 @@ template <typename R, bool Q, bool R16> static void launch_fft_tq(bdk_t *h, int 
     const R *sq = std::is_same<R, float>::value ? (const R *)h->d.sqf : (const R *)h->d.sq;
     const R *lap = std::is_same<R, float>::value ? (const R *)h->d.lapf : (const R *)h->d.lap;
 #define R2C(E)                                                                                              \
-    hipLaunchKernelGGL((k_r2c_rows<R, E, Q>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,          \
+    hipLaunchKernelGGL((k_r2c_rows<R, E, Q, R16>), dim3(Z / c.RB), dim3(FFT_THREADS), c.l1, st,     \
                        (const R *)h->d.rho, (C *)h->d.ht, (const C *)h->d.tw, Z, c.RB, c.sR1, h->g.xcd)
 #define COL(E)                                                                                              \
     hipLaunchKernelGGL((k_col<R, E, Q, R16>), dim3((Z / 2) / c.RC), dim3(FFT_THREADS), c.lc, st,    \
@@ -3529,9 +3548,12 @@ template <typename R, bool P> static void batch_stage_tp(const BatchPlan &bp, in
         });
         break;
     case 2:
-        kinds(BS_R2C, [&](auto q, const int2 *mp, int nm) {
+        kinds16(BS_R2C, [&](auto q, auto r16, const int2 *mp, int nm) {
             constexpr bool Q = decltype(q)::value;
-#define R2C(E) hipLaunchKernelGGL((kb_r2c<R, E, P, Q>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.l1, st, m, mp, nm)
+            constexpr bool R16 = decltype(r16)::value && std::is_same<R, float>::value;
+#define R2C(E)                                                                                                 \
+    hipLaunchKernelGGL((kb_r2c<R, E, P, Q, R16>), dim3(stage_grid(bp, nm)), dim3(FFT_THREADS), bp.l1, st, m, mp, \
+                       nm)
             BDK_EPT_SWITCH2(bp.ept[0], R2C)
 #undef R2C
         });
