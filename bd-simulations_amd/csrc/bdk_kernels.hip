@@ -1699,6 +1699,14 @@ __device__ __forceinline__ void gather_body(const Geo &g, const DevBufs &d, int 
         const int gx = div_nc(c, g), gy = c - gx * g.nc;
         const double2 r = d.lrel[s];
         const int pid = d.lpid[s];
+        // the WCA force and the noise before the mesh gather: their loads
+        // overlap the gather's instead of waiting behind the fkap store
+        R2 fw;
+        double2 n;
+        if (with_dt) {
+            fw = ((const R2 *)d.fwca)[s];
+            n = particle_noise(g, d, pid, d.sc->step);
+        }
         double fx = 0.0, fy = 0.0;
         if (!g.idgas) gather_kap<R, SCHEME, P2>(gx * g.B + r.x, gy * g.B + r.y, pid >= g.n1, (const R2 *)d.fmesh, g, fx, fy);
         R2 o;
@@ -1706,8 +1714,6 @@ __device__ __forceinline__ void gather_body(const Geo &g, const DevBufs &d, int 
         o.y = (R)fy;
         ((R2 *)d.fkap)[s] = o;
         if (with_dt) {
-            const R2 fw = ((const R2 *)d.fwca)[s];
-            const double2 n = particle_noise(g, d, pid, d.sc->step);
             double c0 = dt_candidate((double)fw.x + (double)o.x, n.x, g);
             double c1 = dt_candidate((double)fw.y + (double)o.y, n.y, g);
             if (c0 < cand) cand = c0;
