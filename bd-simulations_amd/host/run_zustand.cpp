@@ -254,7 +254,7 @@ void RunZustand::init(int nr_) {
     if (!dyn.mt) bdk_mt_create(cfg_.seed + (uint64_t)nr, &dyn.mt);
     if (cfg_.startpos_methode != 1) {
         std::cout << "Fehler: startpos_methode=" << cfg_.startpos_methode
-                  << ", this build implements 1 (random start, dm:1647-1678)" << std::endl;
+                  << ", this library implements 1 (random start, dm:1647-1678)" << std::endl;
         std::exit(1);
     }
     init_zufallspos();

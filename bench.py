@@ -330,7 +330,7 @@ def main():
                     help="run each kernel REPS times plus a calibration copy and exit (for --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs one simulation on one GPU; --gpus %d is not supported "
+        print("bench.py: the library runs one simulation on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

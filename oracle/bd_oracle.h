@@ -3,7 +3,7 @@
  * Gully85/BD-Simulations (RunZustand::RunDynamik, dynamik_methoden.cpp,
  * gridRoutinen.cpp, allgemeine_methoden.cpp, zufall.cpp, auswertung.cpp).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity reference for the HIP
  * path: only the tests, the smoke check and the CPU-baseline leg of the
  * benchmark may load it.  The product library never links or calls it.
  *

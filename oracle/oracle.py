@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by the tests, the smoke check and the
-CPU-baseline leg of the benchmark, as the checker.  The product (the HIP
+CPU-baseline leg of the benchmark, as the reference result.  The product (the HIP
 library and its host code) never imports this module.
 """
 import ctypes as C
